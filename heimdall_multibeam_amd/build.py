@@ -1,0 +1,65 @@
+"""Builds the native libraries in-tree.
+
+  libheimdall_mi355x.so  the search (hd_* C ABI + gfx950 kernels), hipcc
+  libhd_synth.so         synthetic filterbank generator, g++/OpenMP
+
+Every HIP source is compiled for gfx950 only, with -ffp-contract=off (the
+reference performs no fused multiply-adds, and results are compared bit for
+bit) and with packed FP32 instructions disabled.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libheimdall_mi355x.so")
+SYNTH = os.path.join(HERE, "libhd_synth.so")
+
+HIP_SOURCES = ["hd_api.cpp", "k_dedisperse.hip", "k_normalise.hip", "k_filter.hip",
+               "k_cluster.hip", "k_stages.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed:\n%s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return r
+
+
+def _compile(src):
+    os.makedirs(OBJ, exist_ok=True)
+    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    lang = ["-x", "hip"] if src.endswith(".hip") else []
+    _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", out])
+    return out
+
+
+def build(verbose=False):
+    srcs = [s for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    tmp = SYNTH + ".tmp"
+    _run(["g++", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-std=c++17",
+          "-o", tmp, os.path.join(CSRC, "synth.cpp")])
+    os.replace(tmp, SYNTH)
+    if verbose:
+        print("built", LIB, SYNTH)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,1003 @@
+// hd_api.cpp -- the Heimdall C interface on MI355X.
+//
+// Host orchestration of one gulp (Pipeline/pipeline.cu:233-1087 in the
+// reference): the filterbank goes to HBM once, every stage runs over all DM
+// trials at once on the pipeline's own HIP stream, the DM-time plane and the
+// boxcar outputs never leave the device, and only the candidate groups come
+// back to the host, where the .cand file is written exactly as the reference
+// writes it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <arpa/inet.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <netdb.h>
+#include <sstream>
+#include <string>
+#include <sys/socket.h>
+#include <unistd.h>
+#include <vector>
+
+#define HAVE_PSRDADA 1  // read hd_params through the layout that includes dada_id
+#include "hd/default_params.h"
+#include "hd/mi355x.h"
+#include "hd/parse_command_line.h"
+#include "hd/pipeline.h"
+#include "hd/stages.h"
+#include "hd_internal.hpp"
+#include "hd_plan.hpp"
+
+using hd::SeriesShape;
+using hd::WidthTable;
+using hd::BlockRmsShape;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// device allocations that only grow
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t want = n ? n : 16;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+unsigned log2_pow2(unsigned v) {  // get_filter_index (pipeline.cu:126-136)
+  unsigned r = 0;
+  while ((1u << (r + 1)) <= v && r < 31) ++r;
+  return r;
+}
+
+enum Stage { ST_UNPACK, ST_DEDISP, ST_BASELINE, ST_RMS, ST_SCAN, ST_BLOCKRMS, ST_GIANTS,
+             ST_CLUSTER, ST_TOTAL, ST_COUNT };
+
+}  // namespace
+
+struct hd_pipeline_t {
+  hd_params params;
+  std::string output_dir, sigproc_file, coinc_host;
+  std::vector<hd_range_t> zaps;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool output_enabled = true;
+  // plan
+  std::vector<float> dm_list;
+  std::vector<float> kc;
+  size_t max_delay = 0;
+  // device buffers
+  DevBuf fb, x, live, delays, plane, pts, rms, S, partial, wrms, winv;
+  DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
+  DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm;
+  DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
+  DevBuf o_peak, o_ind, o_begin, o_end, o_filter, o_dm, o_members;
+  unsigned* h_pinned = nullptr;  // small pinned staging word(s)
+  size_t sample_cap = 1u << 22;
+  // delay table state for the current live-channel set
+  std::vector<int> live_list;
+  std::vector<int> killmask_cached;
+  size_t wlen = 0;
+  // results of the last call
+  size_t n_giants = 0;
+  std::vector<float> r_peak, r_dms;
+  std::vector<hd_size> r_ind, r_begin, r_end, r_filter, r_dm, r_members;
+  std::string cand_text;
+  double timings[ST_COUNT] = {0};
+  hipEvent_t ev[ST_COUNT + 1] = {};
+};
+
+// ---------------------------------------------------------------------------
+// params, errors, command line (default_params.C, error.C, parse_command_line.C)
+
+extern "C" void hd_set_default_params(hd_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->verbosity = 0;
+  p->dada_id = 0;
+  p->sigproc_file = nullptr;
+  p->yield_cpu = false;
+  p->nsamps_gulp = 262144;
+  p->dm_gulp_size = 2048;
+  p->baseline_length = 2.0f;
+  p->beam = 0;
+  p->nbeams = 1;
+  p->override_beam = false;
+  p->nchans = 1024;
+  p->dt = 64e-6f;
+  p->f0 = 1581.804688f;
+  p->df = -.390625f;
+  p->dm_min = 0.0f;
+  p->dm_max = 1000.0f;
+  p->dm_tol = 1.25f;
+  p->dm_pulse_width = 40;
+  p->dm_nbits = 32;
+  p->use_scrunching = true;
+  p->scrunch_tol = 1.15f;
+  p->rfi_tol = 5.0f;
+  p->rfi_min_beams = 8;
+  p->boxcar_max = 4096;
+  p->detect_thresh = 6.0f;
+  p->cand_sep_time = 3;
+  p->cand_sep_filter = 3;
+  p->cand_sep_dm = 200;
+  p->cand_rfi_dm_cut = (hd_size)1.5;
+  p->max_giant_rate = 0;
+  p->min_tscrunch_width = 4096;
+  p->num_channel_zaps = 0;
+  p->channel_zaps = nullptr;
+  p->coincidencer_host = nullptr;
+  p->coincidencer_port = -1;
+  p->beam_count = 13;
+  p->gpu_id = 0;
+  p->utc_start = 0;
+  p->spectra_per_second = 0;
+  p->output_dir = ".";
+}
+
+extern "C" const char* hd_get_error_string(hd_error error) {
+  switch (error) {
+    case HD_NO_ERROR: return "No error";
+    case HD_MEM_ALLOC_FAILED: return "Memory allocation failed";
+    case HD_MEM_COPY_FAILED: return "Memory copy failed";
+    case HD_INVALID_DEVICE_INDEX: return "Invalid device index";
+    case HD_DEVICE_ALREADY_SET: return "Device is already set and cannot be changed";
+    case HD_INVALID_PIPELINE: return "Invalid pipeline";
+    case HD_INVALID_POINTER: return "Invalid pointer";
+    case HD_INVALID_STRIDE: return "Invalid stride";
+    case HD_TOO_FEW_NSAMPS: return "No. samples < maximum delay";
+    case HD_PRIOR_GPU_ERROR: return "Prior GPU error.";
+    case HD_INTERNAL_GPU_ERROR: return "Internal GPU error. Please contact the author(s).";
+    case HD_TOO_MANY_EVENTS: return "Too many events";
+    case HD_UNKNOWN_ERROR: return "Unknown error. Please contact the author(s).";
+    default: return "Invalid error code";
+  }
+}
+
+extern "C" hd_error throw_error(hd_error error) { return error; }
+
+extern "C" void hd_print_usage(void) {
+  hd_params p;
+  hd_set_default_params(&p);
+  std::cout << "Usage: heimdall [options]\n"
+            << "    -k  key                  use PSRDADA hexidecimal key\n"
+            << "    -f  filename             process specified SIGPROC filterbank file\n"
+            << "    -vVgG                    increase verbosity level\n"
+            << "    -yield_cpu               yield CPU during GPU operations\n"
+            << "    -gpu_id ID               run on specified GPU\n"
+            << "    -nsamps_gulp num         number of samples to be read at a time ["
+            << p.nsamps_gulp << "]\n"
+            << "    -baseline_length num     number of seconds over which to smooth the baseline ["
+            << p.baseline_length << "]\n"
+            << "    -beam ##                 over-ride beam number\n"
+            << "    -nbeams ##               give number of beams [def 1]\n"
+            << "    -output_dir path         create all output files in specified path\n"
+            << "    -dm min max              min and max DM\n"
+            << "    -dm_tol num              SNR loss tolerance between each DM trial ["
+            << p.dm_tol << "]\n"
+            << "    -coincidencer host:port  connect to the coincidencer on the specified host and port\n"
+            << "    -zap_chans start end     zap all channels between start and end channels (end exclusive)\n"
+            << "    -max_giant_rate nevents  limit the maximum number of individual detections per minute to nevents\n"
+            << "    -dm_pulse_width num      expected intrinsic width of the pulse signal in microseconds ["
+            << p.dm_pulse_width << "]\n"
+            << "    -dm_nbits num            number of bits per sample in dedispersed time series ["
+            << p.dm_nbits << "]\n"
+            << "    -scrunching num          1 = use adaptive dm-dependent time scrunching (not supported; treated as 0)\n"
+            << "    -scrunching_tol num      smearing tolerance factor for time scrunching\n"
+            << "    -rfi_tol num             RFI exicision threshold limits [" << p.rfi_tol << "]\n"
+            << "    -rfi_min_beams num       minimum beams for coincident RFI [" << p.rfi_min_beams
+            << "]\n"
+            << "    -boxcar_max num          maximum boxcar width in samples [" << p.boxcar_max
+            << "]\n"
+            << "    -min_tscrunch_width num  vary between high quality (large value) and high performance (low value)\n";
+}
+
+extern "C" int hd_parse_command_line(int argc, char* argv[], hd_params* params) {
+  int i = 0;
+  auto need = [&](int k) { return i + k < argc; };
+  while (++i < argc) {
+    const std::string a = argv[i];
+    if (a == "-h") { hd_print_usage(); return -1; }
+    if (a == "-v") params->verbosity = std::max(params->verbosity, 1);
+    else if (a == "-V") params->verbosity = std::max(params->verbosity, 2);
+    else if (a == "-g") params->verbosity = std::max(params->verbosity, 3);
+    else if (a == "-G") params->verbosity = std::max(params->verbosity, 4);
+    else if (a == "-k" && need(1)) { unsigned k = 0; std::sscanf(argv[++i], "%x", &k); params->dada_id = (key_t)k; }
+    else if (a == "-f" && need(1)) params->sigproc_file = strdup(argv[++i]);
+    else if (a == "-yield_cpu" && need(1)) params->yield_cpu = std::atoi(argv[++i]) != 0;
+    else if (a == "-nsamps_gulp" && need(1)) params->nsamps_gulp = std::atoi(argv[++i]);
+    else if (a == "-baseline_length" && need(1)) params->baseline_length = std::atof(argv[++i]);
+    else if (a == "-dm" && need(2)) { params->dm_min = std::atof(argv[++i]); params->dm_max = std::atof(argv[++i]); }
+    else if (a == "-dm_tol" && need(1)) params->dm_tol = std::atof(argv[++i]);
+    else if (a == "-dm_pulse_width" && need(1)) params->dm_pulse_width = std::atof(argv[++i]);
+    else if (a == "-dm_nbits" && need(1)) params->dm_nbits = std::atoi(argv[++i]);
+    else if (a == "-gpu_id" && need(1)) params->gpu_id = std::atoi(argv[++i]);
+    else if (a == "-scrunching" && need(1)) params->use_scrunching = std::atoi(argv[++i]) != 0;
+    else if (a == "-scrunch_tol" && need(1)) params->scrunch_tol = std::atof(argv[++i]);
+    else if (a == "-rfi_tol" && need(1)) params->rfi_tol = std::atof(argv[++i]);
+    else if (a == "-rfi_min_beams" && need(1)) params->rfi_min_beams = std::atoi(argv[++i]);
+    else if (a == "-boxcar_max" && need(1)) params->boxcar_max = std::atoi(argv[++i]);
+    else if (a == "-detect_thresh" && need(1)) params->detect_thresh = std::atof(argv[++i]);
+    else if (a == "-beam" && need(1)) { params->beam = std::atoi(argv[++i]) - 1; params->override_beam = true; }
+    else if (a == "-nbeams" && need(1)) params->nbeams = std::atoi(argv[++i]);
+    else if (a == "-beam_count" && need(1)) params->beam_count = std::atoi(argv[++i]);
+    else if (a == "-cand_sep_time" && need(1)) params->cand_sep_time = std::atoi(argv[++i]);
+    else if (a == "-cand_sep_filter" && need(1)) params->cand_sep_filter = std::atoi(argv[++i]);
+    else if (a == "-cand_sep_dm_trial" && need(1)) params->cand_sep_dm = std::atoi(argv[++i]);
+    else if (a == "-cand_rfi_dm_cut" && need(1)) params->cand_rfi_dm_cut = (hd_size)std::atof(argv[++i]);
+    else if (a == "-max_giant_rate" && need(1)) params->max_giant_rate = std::atof(argv[++i]);
+    else if (a == "-output_dir" && need(1)) params->output_dir = strdup(argv[++i]);
+    else if (a == "-min_tscrunch_width" && need(1)) params->min_tscrunch_width = std::atoi(argv[++i]);
+    else if (a == "-coincidencer" && need(1)) {
+      std::istringstream iss(argv[++i]);
+      std::string host, port;
+      std::getline(iss, host, ':');
+      std::getline(iss, port);
+      params->coincidencer_host = strdup(host.c_str());
+      params->coincidencer_port = std::atoi(port.c_str());
+    } else if (a == "-zap_chans" && need(2)) {
+      unsigned iz = params->num_channel_zaps++;
+      params->channel_zaps = (hd_range_t*)realloc(params->channel_zaps,
+                                                  sizeof(hd_range_t) * params->num_channel_zaps);
+      params->channel_zaps[iz].start = std::atoi(argv[++i]);
+      params->channel_zaps[iz].end = std::atoi(argv[++i]);
+    } else {
+      std::cerr << "WARNING: Unknown parameter '" << a << "'" << std::endl;
+    }
+  }
+  if (params->sigproc_file == nullptr && params->dada_id == 0) {
+    std::cerr << "ERROR: no input mechanism specified" << std::endl;
+    hd_print_usage();
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" hd_error hd_apply_manual_killmasks(int* h_killmask, hd_size nchans,
+                                              unsigned int num_channel_zaps,
+                                              const hd_range_t* channel_zaps) {
+  if (num_channel_zaps && !channel_zaps) return HD_INVALID_POINTER;
+  for (unsigned i = 0; i < num_channel_zaps; ++i)
+    for (hd_size j = channel_zaps[i].start; j < channel_zaps[i].end; ++j)
+      if (j < nchans) h_killmask[j] = 0;
+  return HD_NO_ERROR;
+}
+
+extern "C" hd_size hd_generate_dm_list(hd_float* out, hd_size cap, hd_float dm_min,
+                                       hd_float dm_max, hd_float dt, hd_float pulse_width_us,
+                                       hd_float f0, hd_float df, hd_size nchans, hd_float tol) {
+  const std::vector<float> l = hdplan::dm_list(dm_min, dm_max, dt, pulse_width_us, f0, df, nchans, tol);
+  for (size_t i = 0; i < l.size() && i < cap && out; ++i) out[i] = l[i];
+  return l.size();
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+hd_error gpu_err(hipError_t e) {
+  if (e == hipSuccess) return HD_NO_ERROR;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return HD_MEM_ALLOC_FAILED;
+  if (e == hipErrorInvalidDevice) return HD_INVALID_DEVICE_INDEX;
+  std::cerr << "HIP error: " << hipGetErrorString(e) << std::endl;
+  return HD_INTERNAL_GPU_ERROR;
+}
+
+#define HD_CHECK(expr)                                  \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return gpu_err(_e);           \
+  } while (0)
+
+struct DeviceGuard {  // binds the pipeline's device, restores the caller's
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+hd_error build_live_and_delays(hd_pipeline pl) {
+  const hd_params& p = pl->params;
+  std::vector<int> km(p.nchans, 1);
+  hd_apply_manual_killmasks(km.data(), p.nchans, (unsigned)pl->zaps.size(), pl->zaps.data());
+  if (km == pl->killmask_cached && !pl->live_list.empty()) return HD_NO_ERROR;
+  pl->killmask_cached = km;
+  pl->live_list.clear();
+  for (size_t c = 0; c < p.nchans; ++c)
+    if (km[c]) pl->live_list.push_back((int)c);
+  const size_t n_dm = pl->dm_list.size();
+  const size_t nl = pl->live_list.size();
+  std::vector<int> d(n_dm * std::max<size_t>(nl, 1));
+  for (size_t m = 0; m < n_dm; ++m)
+    for (size_t l = 0; l < nl; ++l)
+      d[m * nl + l] = hdplan::delay(pl->dm_list[m], pl->kc[pl->live_list[l]]);
+  // LDS window: widest (last - first) delay span of a DM block, dword aligned
+  const int DB = hd::dedisp_tile_dms();
+  int spread = 0;
+  for (size_t m0 = 0; m0 < n_dm; m0 += DB) {
+    const size_t ml = std::min(m0 + DB, n_dm) - 1;
+    for (size_t l = 0; l < nl; ++l)
+      spread = std::max(spread, d[ml * nl + l] - (d[m0 * nl + l] & ~3));
+  }
+  pl->wlen = round_up((size_t)spread + hd::dedisp_tile_time() + 8, 4);
+  HD_CHECK(pl->live.ensure(std::max<size_t>(nl, 1) * sizeof(int)));
+  HD_CHECK(pl->delays.ensure(d.size() * sizeof(int)));
+  HD_CHECK(hipMemcpy(pl->live.p, pl->live_list.data(), nl * sizeof(int), hipMemcpyHostToDevice));
+  HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
+  return HD_NO_ERROR;
+}
+
+// Shapes of one call (pipeline.cu:488-499, 572-573, 626-692; remove_baseline.cu:30-46)
+struct CallShape {
+  SeriesShape sh;
+  WidthTable wt;
+  BlockRmsShape bs;
+  size_t bc692;
+  bool skip;  // DM loop breaks before the first trial (no giants)
+};
+
+hd_error make_shapes(hd_pipeline pl, size_t nsamps, size_t nbeams, CallShape& cs) {
+  const hd_params& p = pl->params;
+  std::memset(&cs, 0, sizeof(cs));
+  const size_t max_delay = pl->max_delay;
+  const size_t W = p.boxcar_max;
+  SeriesShape& sh = cs.sh;
+  sh.nbeams = (unsigned)nbeams;
+  sh.stride = nsamps / nbeams;
+  sh.nout = nsamps - max_delay;
+  cs.skip = (max_delay >= sh.stride) || (max_delay + W > sh.stride);
+  if (cs.skip) return HD_NO_ERROR;
+  sh.bc = sh.stride - max_delay;
+  const int TB = hd::dedisp_tile_time();
+  sh.ppitch = round_up(sh.nout + 1, (size_t)TB);
+  sh.spitch = round_up(sh.nout + 1, 64);
+  // baseline (pipeline.cu:572-573 then remove_baseline.cu:36-46)
+  const float cur_dt = p.dt * 1.0f;
+  const size_t smooth = (size_t)(p.baseline_length / (2 * cur_dt));
+  const float oversample = 2;
+  sh.sc = (size_t)((double)(oversample * (float)sh.stride / (float)(2 * smooth)) + 0.5);
+  if (sh.sc) {
+    sh.nscr = (int)(size_t)(std::log((double)(sh.stride / sh.sc)) / std::log(5.));
+    sh.count_round = (size_t)(std::pow(5., (double)sh.nscr) * (double)sh.sc);
+    if (sh.count_round < 2) return HD_INVALID_STRIDE;
+    int Ld = 0;
+    while (Ld < std::min(sh.nscr, 3) && sh.count_round / hd::pow5_host(Ld) > 12288) ++Ld;
+    if (sh.count_round / hd::pow5_host(Ld) > 12288) return HD_INVALID_STRIDE;
+    sh.bl_direct = Ld;
+    sh.step_rs_beam = (float)(sh.stride - 1) / (float)sh.count_round;
+    sh.step_rs_last = (float)(sh.bc - 1) / (float)(sh.count_round - 1);
+    sh.step_bl_beam = (float)(sh.sc - 1) / (float)sh.stride;
+    sh.step_bl_last = (float)(sh.sc - 1) / (float)(sh.bc - 1);
+    // The last beam's stretch back could read one point past the pyramid
+    // (median_filter.cu:462 with j+1 == sample_count), i.e. a stale scrunch
+    // buffer entry.  That case is refused rather than guessed.
+    for (size_t i = sh.bc; i-- > 0;) {
+      const float x = (float)(unsigned)i * sh.step_bl_last;
+      const unsigned j = (unsigned)x;
+      const float fr = x - (float)j;
+      if ((size_t)j + 1 < sh.sc) break;
+      if (fr > 1e-5f) return HD_INVALID_STRIDE;
+    }
+  }
+  {
+    int Ld = 0;
+    while (sh.bc / hd::pow5_host(Ld) > 12288 && sh.bc >= hd::pow5_host(Ld + 1) && Ld < 3) ++Ld;
+    if (sh.bc / hd::pow5_host(Ld) > 12288) return HD_INVALID_STRIDE;
+    sh.rms_direct = Ld;
+    sh.rms_delta = (float)(sh.stride - sh.bc);
+  }
+  // widths (pipeline.cu:657-692)
+  WidthTable& wt = cs.wt;
+  wt.off = (unsigned)(W / 2);
+  const size_t max_nsamps_filtered = sh.nout + 1 - W;
+  const size_t rel_min_ts = std::max<size_t>(p.min_tscrunch_width, 1);
+  int nw = 0;
+  for (size_t w = 1; w <= W; w *= 2) {
+    if (nw >= hd::kMaxWidths) return HD_INVALID_STRIDE;
+    hd::WidthInfo& wi = wt.wi[nw++];
+    const size_t ts = std::max<size_t>(2 * w / rel_min_ts, 1);
+    wi.w = (unsigned)w;
+    wi.ts = (unsigned)ts;
+    wi.ahead = (unsigned)((w - 1) / 2 + 1);
+    wi.behind = (unsigned)(w / 2);
+    wi.n_w = (unsigned)((max_nsamps_filtered - 1) / ts + 1);
+    wi.filter_idx = log2_pow2((unsigned)w);
+    wi.merge_dist = (unsigned)(p.cand_sep_time * (w / ts));
+    wi.rel_off = (unsigned)((W / 2) / ts);
+  }
+  wt.nw = nw;
+  // The first width must produce fresh output at every block-RMS position;
+  // otherwise its RMS would read the previous trial's buffer contents.
+  if (nw == 0 || wt.wi[0].ts != 1) return HD_INVALID_STRIDE;
+  cs.bc692 = sh.stride - (max_delay + W);
+  BlockRmsShape& bs = cs.bs;
+  bs.bc = cs.bc692;
+  bs.n = nbeams * cs.bc692;
+  if (bs.n == 0) return HD_TOO_FEW_NSAMPS;
+  bs.delta = (float)(sh.stride - cs.bc692);
+  bs.L0 = 0;
+  while (bs.L0 < 5 && hd::pow5_host(bs.L0 + 1) <= bs.n) ++bs.L0;
+  bs.chunk = hd::pow5_host(bs.L0);
+  bs.np = bs.n / bs.chunk;
+  return HD_NO_ERROR;
+}
+
+void record(hd_pipeline pl, int st) { (void)hipEventRecord(pl->ev[st], pl->stream); }
+
+// ostream float formatting as the reference's candidate writer uses it.
+void write_cand_lines(hd_pipeline pl, std::ostream& os, size_t nsamps, size_t nbeams,
+                      size_t first_idx, size_t processed) {
+  const hd_params& p = pl->params;
+  const unsigned nsamps_beam = (unsigned)(nsamps / nbeams);      // pipeline.cu:903
+  const unsigned nsamps_valid = (unsigned)(processed / nbeams);   // :905
+  for (size_t i = 0; i < pl->r_peak.size(); ++i) {
+    const hd_size ind = pl->r_ind[i];
+    const hd_size beam_idx = (hd_size)std::floor((float)ind / (float)nsamps_beam);
+    const hd_size beam_start = beam_idx * nsamps_beam + (p.boxcar_max / 2);
+    const hd_size beam_end = beam_start + nsamps_valid;
+    if (!(beam_start <= ind && ind < beam_end)) continue;
+    const hd_size samp_offset = ind - (nsamps_beam * beam_idx);
+    const hd_size samp_idx = first_idx + samp_offset;
+    const hd_float snr = std::isinf(pl->r_peak[i]) ? 0.0f : pl->r_peak[i];
+    os << snr << "\t" << samp_idx << "\t" << p.dt * samp_idx << "\t" << pl->r_filter[i] << "\t"
+       << pl->r_dm[i] << "\t" << pl->r_dms[i] << "\t" << pl->r_members[i] << "\t"
+       << first_idx + (pl->r_begin[i] % nsamps_beam) << "\t"
+       << first_idx + (pl->r_end[i] % nsamps_beam) << "\t" << (p.beam + beam_idx + 1) << "\t"
+       << "\n";
+  }
+}
+
+hd_error emit_output(hd_pipeline pl, size_t nsamps, size_t nbeams, size_t first_idx,
+                     size_t processed) {
+  const hd_params& p = pl->params;
+  std::ostringstream lines;
+  write_cand_lines(pl, lines, nsamps, nbeams, first_idx, processed);
+  pl->cand_text = lines.str();
+  if (!pl->output_enabled) return HD_NO_ERROR;
+  // Output name (pipeline.cu:893-901).  The reference divides by
+  // spectra_per_second unconditionally; a zero rate is taken as offset 0.
+  char buffer[64];
+  const time_t off = p.spectra_per_second ? (time_t)(first_idx / p.spectra_per_second) : 0;
+  time_t now = p.utc_start + off;
+  std::strftime(buffer, 64, "%Y-%m-%d-%H:%M:%S", std::gmtime(&now));
+  std::ostringstream ss;
+  ss << std::setw(2) << std::setfill('0') << (p.beam + 1);
+  if (!pl->coinc_host.empty() && p.coincidencer_port != -1) {
+    // TCP branch (pipeline.cu:907-995): header line then one line per event
+    const unsigned nsamps_beam = (unsigned)(nsamps / nbeams);
+    const unsigned nsamps_valid = (unsigned)(processed / nbeams);
+    unsigned n_events = 0;
+    for (size_t i = 0; i < pl->r_ind.size(); ++i) {
+      const hd_size ind = pl->r_ind[i];
+      const hd_size beam_idx = (hd_size)std::floor((float)ind / (float)nsamps_beam);
+      const hd_size beam_start = beam_idx * nsamps_beam + (p.boxcar_max / 2);
+      if (beam_start <= ind && ind < beam_start + nsamps_valid) ++n_events;
+    }
+    std::ostringstream msg;
+    char ubuf[64];
+    time_t utc = p.utc_start;
+    std::strftime(ubuf, 64, "%Y-%m-%d-%H:%M:%S", std::gmtime(&utc));
+    msg << ubuf << " " << buffer << " " << first_idx << " " << p.beam << " " << nbeams << " "
+        << n_events << std::endl;
+    std::ostringstream ev;
+    for (size_t i = 0; i < pl->r_ind.size(); ++i) {
+      const hd_size ind = pl->r_ind[i];
+      const hd_size beam_idx = (hd_size)std::floor((float)ind / (float)nsamps_beam);
+      const hd_size beam_start = beam_idx * nsamps_beam + (p.boxcar_max / 2);
+      if (!(beam_start <= ind && ind < beam_start + nsamps_valid)) continue;
+      const hd_size samp_idx = first_idx + ind - (nsamps_beam * beam_idx);
+      const hd_float snr = std::isinf(pl->r_peak[i]) ? 0.0f : pl->r_peak[i];
+      ev << snr << "\t" << samp_idx << "\t" << samp_idx * p.dt << "\t" << pl->r_filter[i] << "\t"
+         << pl->r_dm[i] << "\t" << pl->r_dms[i] << "\t" << pl->r_members[i] << "\t"
+         << first_idx + (pl->r_begin[i] % nsamps_beam) << "\t"
+         << first_idx + (pl->r_end[i] % nsamps_beam) << "\t" << (p.beam + beam_idx + 1)
+         << std::endl;
+    }
+    addrinfo hints{}, *res = nullptr;
+    hints.ai_family = AF_INET;
+    hints.ai_socktype = SOCK_STREAM;
+    const std::string port = std::to_string(p.coincidencer_port);
+    if (getaddrinfo(pl->coinc_host.c_str(), port.c_str(), &hints, &res) == 0 && res) {
+      int fd = socket(res->ai_family, res->ai_socktype, res->ai_protocol);
+      if (fd >= 0 && connect(fd, res->ai_addr, res->ai_addrlen) == 0) {
+        const std::string a = msg.str(), b = ev.str();
+        (void)!write(fd, a.data(), a.size());
+        (void)!write(fd, b.data(), b.size());
+      } else {
+        std::cerr << "SocketException was caught:Could not connect to port." << "\n";
+      }
+      if (fd >= 0) close(fd);
+      freeaddrinfo(res);
+    } else {
+      std::cerr << "SocketException was caught:Could not resolve host." << "\n";
+    }
+    return HD_NO_ERROR;
+  }
+  const std::string filename = pl->output_dir + "/" + std::string(buffer) + "_" + ss.str() + ".cand";
+  std::ofstream cand_file(filename.c_str(), std::ios::out);
+  if (cand_file.good()) cand_file << pl->cand_text;
+  else std::cout << "Skipping dump due to bad file open on " << filename << std::endl;
+  cand_file.close();
+  return HD_NO_ERROR;
+}
+
+// The device part of one gulp.  `d_fb` is the filterbank in device memory.
+hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbits,
+                  size_t first_idx, size_t nbeams, size_t* processed, bool* too_many) {
+  const hd_params& p = pl->params;
+  hipStream_t s = pl->stream;
+  *too_many = false;
+  pl->n_giants = 0;
+  pl->r_peak.clear(); pl->r_ind.clear(); pl->r_begin.clear(); pl->r_end.clear();
+  pl->r_filter.clear(); pl->r_dm.clear(); pl->r_members.clear(); pl->r_dms.clear();
+  for (double& t : pl->timings) t = 0;
+  const size_t W = p.boxcar_max;
+  const size_t max_delay = pl->max_delay;
+  // pipeline.cu:370-372
+  *processed = (nsamps < nbeams * (W + max_delay)) ? 0 : nsamps - nbeams * (W + max_delay);
+  if (max_delay >= nsamps) return HD_TOO_FEW_NSAMPS;
+  CallShape cs;
+  hd_error err = make_shapes(pl, nsamps, nbeams, cs);
+  if (err != HD_NO_ERROR) return err;
+  err = build_live_and_delays(pl);
+  if (err != HD_NO_ERROR) return err;
+  if (cs.skip) return HD_NO_ERROR;
+  const SeriesShape& sh = cs.sh;
+  const WidthTable& wt = cs.wt;
+  const int n_dm = (int)pl->dm_list.size();
+  const int n_live = (int)pl->live_list.size();
+  const int TB = hd::dedisp_tile_time();
+
+  record(pl, 0);
+  // unpack to channel-major bytes
+  const size_t xpitch = round_up(sh.ppitch + max_delay + pl->wlen + 64, 256);
+  HD_CHECK(pl->x.ensure((size_t)std::max(n_live, 1) * xpitch));
+  HD_CHECK(hd::launch_unpack(d_fb, p.nchans * nbits / 8, (int)nbits, nsamps, pl->live.as<int>(),
+                             n_live, pl->x.as<uint8_t>(), xpitch, s));
+  record(pl, 1);
+  // dedisperse
+  HD_CHECK(pl->plane.ensure((size_t)n_dm * sh.ppitch * sizeof(float)));
+  hd::DedispGeom g{n_dm, n_live, sh.nout, xpitch, sh.ppitch, (int)pl->wlen};
+  (void)TB;
+  HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), g,
+                                 pl->plane.as<float>(), s));
+  record(pl, 2);
+  // baseline points, per-beam RMS, normalise
+  HD_CHECK(pl->pts.ensure(((size_t)n_dm * nbeams * std::max<size_t>(sh.sc, 1) + 64) * sizeof(float)));
+  HD_CHECK(pl->rms.ensure((size_t)n_dm * nbeams * sizeof(float)));
+  HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh, n_dm, pl->pts.as<float>(), s));
+  record(pl, 3);
+  HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh, n_dm,
+                               pl->rms.as<float>(), s));
+  HD_CHECK(hd::launch_normalise(pl->plane.as<float>(), pl->pts.as<float>(), pl->rms.as<float>(),
+                                sh, n_dm, s));
+  record(pl, 4);
+  // prefix sums
+  HD_CHECK(pl->S.ensure((size_t)n_dm * sh.spitch * sizeof(float)));
+  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), sh.ppitch, sh.nout, n_dm, pl->S.as<float>(),
+                           sh.spitch, s));
+  record(pl, 5);
+  // block RMS: the ts == 1 widths together, then each decimated width in order
+  const BlockRmsShape& bs = cs.bs;
+  HD_CHECK(pl->partial.ensure((size_t)n_dm * wt.nw * std::max<size_t>(bs.np, 1) * sizeof(float)));
+  HD_CHECK(pl->wrms.ensure((size_t)n_dm * wt.nw * sizeof(float)));
+  HD_CHECK(pl->winv.ensure((size_t)n_dm * wt.nw * sizeof(float)));
+  int n_fresh = 0;
+  while (n_fresh < wt.nw && wt.wi[n_fresh].ts == 1) ++n_fresh;
+  HD_CHECK(hd::launch_block_rms_partials(pl->S.as<float>(), sh, wt, bs, 0, n_fresh, n_dm,
+                                         pl->partial.as<float>(), pl->winv.as<float>(), s));
+  HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), bs, wt.nw, 0, n_fresh, n_dm,
+                                       pl->wrms.as<float>(), pl->winv.as<float>(), s));
+  for (int j = n_fresh; j < wt.nw; ++j) {
+    HD_CHECK(hd::launch_block_rms_partials(pl->S.as<float>(), sh, wt, bs, j, 1, n_dm,
+                                           pl->partial.as<float>(), pl->winv.as<float>(), s));
+    HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), bs, wt.nw, j, 1, n_dm,
+                                         pl->wrms.as<float>(), pl->winv.as<float>(), s));
+  }
+  record(pl, 6);
+  // threshold + ordered compaction
+  hd::GiantBuffers gb{};
+  gb.chunk = 4096;
+  gb.nchunks_per_w = (wt.wi[0].n_w + gb.chunk - 1) / gb.chunk;
+  const size_t nkeys = (size_t)n_dm * wt.nw * gb.nchunks_per_w;
+  HD_CHECK(pl->ccount.ensure(nkeys * sizeof(unsigned)));
+  HD_CHECK(pl->coff.ensure(nkeys * sizeof(unsigned)));
+  HD_CHECK(pl->cpos.ensure(nkeys * sizeof(unsigned)));
+  HD_CHECK(pl->counter.ensure(sizeof(unsigned) * 4));
+  HD_CHECK(pl->scan_tmp.ensure(hd::exclusive_scan_tmp_words(std::max<size_t>(nkeys, 1 << 20)) *
+                               sizeof(unsigned)));
+  unsigned nsamp = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    HD_CHECK(pl->skey.ensure(pl->sample_cap * sizeof(uint64_t)));
+    HD_CHECK(pl->sval.ensure(pl->sample_cap * sizeof(float)));
+    gb.skey = pl->skey.as<uint64_t>();
+    gb.sval = pl->sval.as<float>();
+    gb.scap = pl->sample_cap;
+    gb.counter = pl->counter.as<unsigned>();
+    gb.ccount = pl->ccount.as<unsigned>();
+    gb.coff = pl->coff.as<unsigned>();
+    gb.cpos = pl->cpos.as<unsigned>();
+    gb.scan_tmp = pl->scan_tmp.as<unsigned>();
+    HD_CHECK(hipMemsetAsync(gb.counter, 0, sizeof(unsigned), s));
+    HD_CHECK(hd::launch_threshold(pl->S.as<float>(), sh.spitch, sh, wt, pl->winv.as<float>(),
+                                  n_dm, p.detect_thresh, cs.bc692, gb, s));
+    HD_CHECK(hipMemcpyAsync(pl->h_pinned, gb.counter, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipStreamSynchronize(s));
+    nsamp = pl->h_pinned[0];
+    if (nsamp <= pl->sample_cap) break;
+    pl->sample_cap = round_up((size_t)nsamp + nsamp / 4, 1 << 20);
+  }
+  if (nsamp > pl->sample_cap) return HD_MEM_ALLOC_FAILED;
+  size_t n_giants = 0;
+  if (nsamp > 0) {
+    HD_CHECK(hd::launch_exclusive_scan_u32(gb.ccount, gb.cpos, nkeys, gb.scan_tmp, s));
+    HD_CHECK(pl->okey.ensure((size_t)nsamp * sizeof(uint64_t)));
+    HD_CHECK(pl->oval.ensure((size_t)nsamp * sizeof(float)));
+    HD_CHECK(pl->head.ensure((size_t)nsamp * sizeof(unsigned)));
+    HD_CHECK(pl->segid.ensure((size_t)nsamp * sizeof(unsigned)));
+    HD_CHECK(pl->scan_tmp.ensure(hd::exclusive_scan_tmp_words(std::max<size_t>(nkeys, nsamp)) *
+                                 sizeof(unsigned)));
+    gb.scan_tmp = pl->scan_tmp.as<unsigned>();
+    gb.okey = pl->okey.as<uint64_t>();
+    gb.oval = pl->oval.as<float>();
+    gb.head = pl->head.as<unsigned>();
+    gb.segid = pl->segid.as<unsigned>();
+    HD_CHECK(hd::launch_order_samples(gb, nkeys, s));
+    HD_CHECK(hd::launch_segment_heads(gb, wt, nsamp, s));
+    HD_CHECK(hd::launch_exclusive_scan_u32(gb.head, gb.segid, nsamp, gb.scan_tmp, s));
+    HD_CHECK(hipMemcpyAsync(pl->h_pinned, gb.segid + (nsamp - 1), sizeof(unsigned),
+                            hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipMemcpyAsync(pl->h_pinned + 1, gb.head + (nsamp - 1), sizeof(unsigned),
+                            hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipStreamSynchronize(s));
+    n_giants = (size_t)pl->h_pinned[0] + pl->h_pinned[1];
+    HD_CHECK(pl->g_peak.ensure(n_giants * sizeof(float)));
+    HD_CHECK(pl->g_ind.ensure(n_giants * sizeof(uint64_t)));
+    HD_CHECK(pl->g_begin.ensure(n_giants * sizeof(uint64_t)));
+    HD_CHECK(pl->g_end.ensure(n_giants * sizeof(uint64_t)));
+    HD_CHECK(pl->g_filter.ensure(n_giants * sizeof(uint32_t)));
+    HD_CHECK(pl->g_dm.ensure(n_giants * sizeof(uint32_t)));
+    hd::GiantOut go{pl->g_peak.as<float>(), pl->g_ind.as<uint64_t>(), pl->g_begin.as<uint64_t>(),
+                    pl->g_end.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
+                    pl->g_dm.as<uint32_t>()};
+    HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, go, s));
+  }
+  pl->n_giants = n_giants;
+  record(pl, 7);
+  // giant-rate limit (pipeline.cu:780-787): the running total only grows, so
+  // the limit trips during the loop exactly when the final total exceeds it.
+  if (p.max_giant_rate != 0) {
+    const float mins = (float)((double)((float)nsamps * p.dt) / 60.0);
+    if ((float)n_giants / mins > p.max_giant_rate) *too_many = true;
+  }
+  if (*too_many || n_giants == 0) {
+    record(pl, 8);
+    return HD_NO_ERROR;
+  }
+  // clustering and merging (pipeline.cu:808-887)
+  const unsigned ng = (unsigned)n_giants;
+  size_t npad = 1;
+  while (npad < ng) npad <<= 1;
+  HD_CHECK(pl->c_keys.ensure(npad * sizeof(uint64_t)));
+  HD_CHECK(pl->c_lo.ensure(ng * sizeof(uint32_t)));
+  HD_CHECK(pl->c_hi.ensure(ng * sizeof(uint32_t)));
+  HD_CHECK(pl->c_labels.ensure(ng * sizeof(int32_t)));
+  HD_CHECK(pl->c_flag.ensure(sizeof(unsigned) * 4));
+  HD_CHECK(pl->c_best.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->c_members.ensure(ng * sizeof(unsigned)));
+  HD_CHECK(pl->c_rflag.ensure(ng * sizeof(unsigned)));
+  HD_CHECK(pl->c_rpos.ensure(ng * sizeof(unsigned)));
+  HD_CHECK(pl->c_tmp.ensure(hd::exclusive_scan_tmp_words(ng) * sizeof(unsigned)));
+  hd::ClusterBuffers cb{pl->c_keys.as<uint64_t>(), pl->c_lo.as<uint32_t>(), pl->c_hi.as<uint32_t>(),
+                        pl->c_labels.as<int32_t>(), pl->c_flag.as<unsigned>(),
+                        pl->c_best.as<uint64_t>(), pl->c_members.as<unsigned>(),
+                        pl->c_rflag.as<unsigned>(), pl->c_rpos.as<unsigned>(),
+                        pl->c_tmp.as<unsigned>(), npad};
+  const unsigned max_filter = wt.wi[wt.nw - 1].filter_idx;
+  int iters = 0;
+  HD_CHECK(hd::launch_cluster(pl->g_ind.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
+                              pl->g_dm.as<uint32_t>(), nullptr, ng, p.cand_sep_time,
+                              p.cand_sep_filter, p.cand_sep_dm, nsamps / nbeams, max_filter, cb,
+                              s, pl->h_pinned, &iters));
+  HD_CHECK(pl->o_peak.ensure(ng * sizeof(float)));
+  HD_CHECK(pl->o_ind.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->o_begin.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->o_end.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->o_filter.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->o_dm.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->o_members.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(hd::launch_merge(pl->g_peak.as<float>(), pl->g_ind.as<uint64_t>(),
+                            pl->g_begin.as<uint64_t>(), pl->g_end.as<uint64_t>(),
+                            pl->g_filter.as<uint32_t>(), pl->g_dm.as<uint32_t>(), nullptr, ng, cb,
+                            pl->o_peak.as<float>(), pl->o_ind.as<uint64_t>(),
+                            pl->o_begin.as<uint64_t>(), pl->o_end.as<uint64_t>(),
+                            pl->o_filter.as<uint64_t>(), pl->o_dm.as<uint64_t>(),
+                            pl->o_members.as<uint64_t>(), s));
+  HD_CHECK(hipMemcpyAsync(pl->h_pinned, cb.root_pos + (ng - 1), sizeof(unsigned),
+                          hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->h_pinned + 1, cb.root_flag + (ng - 1), sizeof(unsigned),
+                          hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipStreamSynchronize(s));
+  const size_t n_groups = (size_t)pl->h_pinned[0] + pl->h_pinned[1];
+  pl->r_peak.resize(n_groups);
+  pl->r_ind.resize(n_groups);
+  pl->r_begin.resize(n_groups);
+  pl->r_end.resize(n_groups);
+  pl->r_filter.resize(n_groups);
+  pl->r_dm.resize(n_groups);
+  pl->r_members.resize(n_groups);
+  pl->r_dms.resize(n_groups);
+  static_assert(sizeof(hd_size) == sizeof(uint64_t), "hd_size must be 64-bit");
+  HD_CHECK(hipMemcpyAsync(pl->r_peak.data(), pl->o_peak.p, n_groups * 4, hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->r_ind.data(), pl->o_ind.p, n_groups * 8, hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->r_begin.data(), pl->o_begin.p, n_groups * 8, hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->r_end.data(), pl->o_end.p, n_groups * 8, hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->r_filter.data(), pl->o_filter.p, n_groups * 8, hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->r_dm.data(), pl->o_dm.p, n_groups * 8, hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->r_members.data(), pl->o_members.p, n_groups * 8, hipMemcpyDeviceToHost, s));
+  record(pl, 8);
+  HD_CHECK(hipStreamSynchronize(s));
+  for (size_t g2 = 0; g2 < n_groups; ++g2) pl->r_dms[g2] = pl->dm_list[pl->r_dm[g2]];  // :872-875
+  return HD_NO_ERROR;
+}
+
+hd_error finish_call(hd_pipeline pl, hd_error err, size_t nsamps, size_t nbeams,
+                     size_t first_idx, size_t processed, bool too_many) {
+  if (err != HD_NO_ERROR) return err;
+  (void)hipEventSynchronize(pl->ev[8]);
+  static const int stage_ev[ST_COUNT][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5},
+                                            {5, 6}, {6, 7}, {7, 8}, {0, 8}};
+  for (int k = 0; k < ST_COUNT; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pl->ev[stage_ev[k][0]], pl->ev[stage_ev[k][1]]) == hipSuccess)
+      pl->timings[k] = ms;
+  }
+  if (too_many) {  // no clustering, empty output (pipeline.cu:808)
+    pl->r_peak.clear(); pl->r_ind.clear(); pl->r_begin.clear(); pl->r_end.clear();
+    pl->r_filter.clear(); pl->r_dm.clear(); pl->r_members.clear(); pl->r_dms.clear();
+  }
+  hd_error e = emit_output(pl, nsamps, nbeams, first_idx, processed);
+  if (e != HD_NO_ERROR) return e;
+  if (pl->params.verbosity >= 1 && pl->output_enabled) {
+    const double* t = pl->timings;
+    std::cout << "Unpack time:             " << t[ST_UNPACK] * 1e-3 << "\n"
+              << "Dedispersion time:       " << t[ST_DEDISP] * 1e-3 << "\n"
+              << "Baselining time:         " << t[ST_BASELINE] * 1e-3 << "\n"
+              << "Normalisation time:      " << t[ST_RMS] * 1e-3 << "\n"
+              << "Filtering time:          " << (t[ST_SCAN] + t[ST_BLOCKRMS]) * 1e-3 << "\n"
+              << "Find giants time:        " << t[ST_GIANTS] * 1e-3 << "\n"
+              << "Process candidates time: " << t[ST_CLUSTER] * 1e-3 << "\n"
+              << "Total time:              " << t[ST_TOTAL] * 1e-3 << std::endl;
+  }
+  return too_many ? HD_TOO_MANY_EVENTS : HD_NO_ERROR;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" hd_error hd_create_pipeline(hd_pipeline* pipeline_, hd_params params) {
+  if (!pipeline_) return HD_INVALID_POINTER;
+  *pipeline_ = nullptr;
+  if (params.nchans == 0 || !(params.dt > 0)) return HD_INVALID_STRIDE;
+  if (!(params.df < 0)) return HD_INVALID_STRIDE;  // delays assume channel 0 is the top
+  if (params.dm_nbits != 32) return HD_INVALID_NBITS;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HD_INVALID_DEVICE_INDEX;
+  if (params.gpu_id < 0 || params.gpu_id >= ndev) return HD_INVALID_DEVICE_INDEX;
+  hd_pipeline_t* pl = new (std::nothrow) hd_pipeline_t();
+  if (!pl) return HD_MEM_ALLOC_FAILED;
+  pl->params = params;
+  if (params.output_dir) pl->output_dir = params.output_dir;
+  else pl->output_dir = ".";
+  if (params.coincidencer_host) pl->coinc_host = params.coincidencer_host;
+  if (params.num_channel_zaps && params.channel_zaps)
+    pl->zaps.assign(params.channel_zaps, params.channel_zaps + params.num_channel_zaps);
+  pl->params.output_dir = nullptr;
+  pl->params.coincidencer_host = nullptr;
+  pl->params.channel_zaps = nullptr;
+  pl->params.sigproc_file = nullptr;
+  pl->device = params.gpu_id;
+  DeviceGuard dg(pl->device);
+  if (params.verbosity >= 1) {
+    std::cout << "Process " << params.beam << " using GPU " << params.gpu_id << std::endl;
+    if (params.use_scrunching)
+      std::cout << "Adaptive time scrunching is not supported; searching at full resolution"
+                << std::endl;
+  }
+  hipError_t e = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&pl->h_pinned, 64, hipHostMallocDefault);
+  for (int k = 0; k <= ST_COUNT && e == hipSuccess; ++k) e = hipEventCreate(&pl->ev[k]);
+  if (e != hipSuccess) {
+    hd_destroy_pipeline(pl);
+    return gpu_err(e);
+  }
+  pl->dm_list = hdplan::dm_list(params.dm_min, params.dm_max, params.dt, params.dm_pulse_width,
+                                params.f0, params.df, params.nchans, params.dm_tol);
+  pl->kc = hdplan::delay_slopes(params.nchans, params.dt, params.f0, params.df);
+  pl->max_delay = 0;
+  for (size_t c = 0; c < params.nchans; ++c) {
+    const int d = hdplan::delay(pl->dm_list.back(), pl->kc[c]);
+    if (d > 0 && (size_t)d > pl->max_delay) pl->max_delay = (size_t)d;
+  }
+  *pipeline_ = pl;
+  return HD_NO_ERROR;
+}
+
+extern "C" hd_size hd_get_max_overlap(hd_pipeline pl, hd_size nbeams) {
+  if (!pl) return 0;
+  return (pl->params.boxcar_max + pl->max_delay) * nbeams;  // pipeline.cu:214-219
+}
+
+extern "C" hd_error hd_preallocate(hd_pipeline pl, hd_size nsamps, hd_size nbeams) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (nbeams == 0) return HD_INVALID_STRIDE;
+  DeviceGuard dg(pl->device);
+  const size_t max_alloc = nsamps + hd_get_max_overlap(pl, nbeams);
+  const size_t n_dm = pl->dm_list.size();
+  const size_t ppitch = round_up(max_alloc + 1, (size_t)hd::dedisp_tile_time());
+  HD_CHECK(pl->fb.ensure(max_alloc * pl->params.nchans));
+  HD_CHECK(pl->plane.ensure(n_dm * ppitch * sizeof(float)));
+  HD_CHECK(pl->S.ensure(n_dm * round_up(max_alloc + 1, 64) * sizeof(float)));
+  return HD_NO_ERROR;
+}
+
+extern "C" hd_error hd_execute(hd_pipeline pl, const hd_byte* h_filterbank, hd_size nsamps,
+                               hd_size nbits, hd_size first_idx, hd_size nbeams,
+                               hd_size* nsamps_processed) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (!h_filterbank || !nsamps_processed) return HD_INVALID_POINTER;
+  if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8)) return HD_INVALID_NBITS;
+  if (nbeams == 0 || nsamps < nbeams) return HD_INVALID_STRIDE;
+  DeviceGuard dg(pl->device);
+  const size_t bytes = nsamps * pl->params.nchans * nbits / 8;
+  HD_CHECK(pl->fb.ensure(bytes));
+  HD_CHECK(hipMemcpyAsync(pl->fb.p, h_filterbank, bytes, hipMemcpyHostToDevice, pl->stream));
+  bool too_many = false;
+  size_t processed = 0;
+  hd_error err = run_gulp(pl, pl->fb.as<hd_byte>(), nsamps, nbits, first_idx, nbeams, &processed,
+                          &too_many);
+  *nsamps_processed = processed;
+  return finish_call(pl, err, nsamps, nbeams, first_idx, processed, too_many);
+}
+
+extern "C" hd_error hd_execute_device(hd_pipeline pl, const hd_byte* d_filterbank,
+                                      hd_size nsamps, hd_size nbits, hd_size first_idx,
+                                      hd_size nbeams, hd_size* nsamps_processed) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (!d_filterbank || !nsamps_processed) return HD_INVALID_POINTER;
+  if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8)) return HD_INVALID_NBITS;
+  if (nbeams == 0 || nsamps < nbeams) return HD_INVALID_STRIDE;
+  DeviceGuard dg(pl->device);
+  bool too_many = false;
+  size_t processed = 0;
+  hd_error err = run_gulp(pl, d_filterbank, nsamps, nbits, first_idx, nbeams, &processed,
+                          &too_many);
+  *nsamps_processed = processed;
+  return finish_call(pl, err, nsamps, nbeams, first_idx, processed, too_many);
+}
+
+extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
+  if (!pl) return;
+  {
+    DeviceGuard dg(pl->device);
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    DevBuf* bufs[] = {&pl->fb, &pl->x, &pl->live, &pl->delays, &pl->plane, &pl->pts, &pl->rms,
+                      &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
+                      &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->okey, &pl->oval,
+                      &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,
+                      &pl->g_begin, &pl->g_end, &pl->g_filter, &pl->g_dm, &pl->c_keys,
+                      &pl->c_lo, &pl->c_hi, &pl->c_labels, &pl->c_flag, &pl->c_best,
+                      &pl->c_members, &pl->c_rflag, &pl->c_rpos, &pl->c_tmp, &pl->o_peak,
+                      &pl->o_ind, &pl->o_begin, &pl->o_end, &pl->o_filter, &pl->o_dm,
+                      &pl->o_members};
+    for (DevBuf* b : bufs) b->release();
+    for (hipEvent_t& ev : pl->ev)
+      if (ev) (void)hipEventDestroy(ev);
+    if (pl->h_pinned) (void)hipHostFree(pl->h_pinned);
+    if (pl->stream) (void)hipStreamDestroy(pl->stream);
+  }
+  delete pl;
+}
+
+// ---------------------------------------------------------------------------
+// accessors (hd/mi355x.h)
+namespace {
+template <typename T>
+void copy_out(T* dst, const std::vector<T>& src, size_t n) {
+  if (dst) std::memcpy(dst, src.data(), n * sizeof(T));
+}
+}  // namespace
+
+extern "C" hd_size hd_get_last_candidates(hd_pipeline pl, hd_size cap, hd_float* peaks,
+                                          hd_size* inds, hd_size* begins, hd_size* ends,
+                                          hd_size* filter_inds, hd_size* dm_inds,
+                                          hd_size* members, hd_float* dms) {
+  if (!pl) return 0;
+  const size_t n = pl->r_peak.size();
+  const size_t k = std::min<size_t>(n, cap);
+  copy_out(peaks, pl->r_peak, k);
+  copy_out(inds, pl->r_ind, k);
+  copy_out(begins, pl->r_begin, k);
+  copy_out(ends, pl->r_end, k);
+  copy_out(filter_inds, pl->r_filter, k);
+  copy_out(dm_inds, pl->r_dm, k);
+  copy_out(members, pl->r_members, k);
+  copy_out(dms, pl->r_dms, k);
+  return n;
+}
+
+extern "C" hd_size hd_get_last_giants(hd_pipeline pl, hd_size cap, hd_float* peaks,
+                                      hd_size* inds, hd_size* begins, hd_size* ends,
+                                      hd_size* filter_inds, hd_size* dm_inds) {
+  if (!pl) return 0;
+  const size_t n = pl->n_giants;
+  const size_t k = std::min<size_t>(n, cap);
+  if (k == 0) return n;
+  DeviceGuard dg(pl->device);
+  std::vector<uint32_t> tmp(k);
+  if (peaks) (void)hipMemcpy(peaks, pl->g_peak.p, k * sizeof(float), hipMemcpyDeviceToHost);
+  if (inds) (void)hipMemcpy(inds, pl->g_ind.p, k * 8, hipMemcpyDeviceToHost);
+  if (begins) (void)hipMemcpy(begins, pl->g_begin.p, k * 8, hipMemcpyDeviceToHost);
+  if (ends) (void)hipMemcpy(ends, pl->g_end.p, k * 8, hipMemcpyDeviceToHost);
+  if (filter_inds) {
+    (void)hipMemcpy(tmp.data(), pl->g_filter.p, k * 4, hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < k; ++i) filter_inds[i] = tmp[i];
+  }
+  if (dm_inds) {
+    (void)hipMemcpy(tmp.data(), pl->g_dm.p, k * 4, hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < k; ++i) dm_inds[i] = tmp[i];
+  }
+  return n;
+}
+
+extern "C" hd_size hd_get_last_cand_text(hd_pipeline pl, char* buf, hd_size cap) {
+  if (!pl) return 0;
+  const size_t n = pl->cand_text.size();
+  if (buf && cap) {
+    const size_t k = std::min<size_t>(n, cap - 1);
+    std::memcpy(buf, pl->cand_text.data(), k);
+    buf[k] = 0;
+  }
+  return n;
+}
+
+extern "C" hd_size hd_get_dm_count(hd_pipeline pl) { return pl ? pl->dm_list.size() : 0; }
+extern "C" hd_size hd_get_max_delay(hd_pipeline pl) { return pl ? pl->max_delay : 0; }
+extern "C" hd_size hd_get_dm_list(hd_pipeline pl, hd_float* out, hd_size cap) {
+  if (!pl) return 0;
+  const size_t n = pl->dm_list.size();
+  if (out) std::memcpy(out, pl->dm_list.data(), std::min<size_t>(n, cap) * sizeof(float));
+  return n;
+}
+extern "C" hd_size hd_get_last_timings(hd_pipeline pl, double* ms, hd_size cap) {
+  if (!pl || !ms) return 0;
+  const size_t k = std::min<size_t>(ST_COUNT, cap);
+  for (size_t i = 0; i < k; ++i) ms[i] = pl->timings[i];
+  return k;
+}
+extern "C" void hd_set_output_enabled(hd_pipeline pl, int enabled) {
+  if (pl) pl->output_enabled = enabled != 0;
+}

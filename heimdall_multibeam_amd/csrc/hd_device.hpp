@@ -1,0 +1,158 @@
+// hd_device.hpp -- device helpers shared by the search kernels (gfx950).
+//
+// All arithmetic here mirrors the reference's float/int conversions one for
+// one; the library is compiled with -ffp-contract=off so that no expression
+// is fused into an FMA (the reference's host build has none).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hd {
+
+// ---------------------------------------------------------------------------
+// Medians.  The reference's comparison networks (Pipeline/median_filter.cu:
+// 21-75) return the order statistic (median4: mean of the middle two).
+// Written here as min/max networks: the extreme elements of a pair of pairs
+// can never be the median, which leaves a median of three.
+__device__ __forceinline__ float med3(float a, float b, float c) {
+  return fmaxf(fminf(a, b), fminf(fmaxf(a, b), c));
+}
+__device__ __forceinline__ float med4(float a, float b, float c, float d) {
+  const float lo = fmaxf(fminf(a, b), fminf(c, d));
+  const float hi = fminf(fmaxf(a, b), fmaxf(c, d));
+  return 0.5f * (lo + hi);
+}
+__device__ __forceinline__ float med5(float a, float b, float c, float d, float e) {
+  const float lo = fmaxf(fminf(a, b), fminf(c, d));
+  const float hi = fminf(fmaxf(a, b), fmaxf(c, d));
+  return med3(e, lo, hi);
+}
+// Tail of a median-of-5 pyramid once fewer than five values remain
+// (Pipeline/median_filter.cu:217-235,295-311).
+__device__ __forceinline__ float med_tail(const float* v, int n) {
+  if (n == 1) return v[0];
+  if (n == 2) return 0.5f * (v[0] + v[1]);
+  if (n == 3) return med3(v[0], v[1], v[2]);
+  return med4(v[0], v[1], v[2], v[3]);
+}
+
+// (double)x > 1e-5 for a float x is the same test as x > 1e-5f: 1e-5f is the
+// float just below 1e-5 and no float lies between the two.
+__device__ __forceinline__ bool frac_gt(float x) { return x > 1e-5f; }
+
+// linear_stretch_functor (Pipeline/median_filter.cu:451-464) for output i;
+// in_end is the element one past the input, read when rounding lands the
+// last output at the end of the input.
+struct Stretch {
+  float step;  // float(in_count-1)/(out_count-1)
+  __device__ __forceinline__ float operator()(const float* in, unsigned i) const {
+    const float x = (float)i * step;
+    const unsigned j = (unsigned)x;
+    const float fr = x - (float)j;
+    if (frac_gt(fr)) {
+      const float a = in[j];
+      const float d = in[j + 1] - a;
+      const float p = fr * d;
+      return a + p;
+    }
+    return in[j] + 0.f;
+  }
+};
+
+// linear_stretch_beam_functor (Pipeline/median_filter.cu:481-499): the index
+// is carried through float across beams.
+struct StretchBeam {
+  float step;              // float(in_beam-1)/out_beam
+  unsigned out_count;      // outputs per beam
+  __device__ __forceinline__ float operator()(const float* in, unsigned i) const {
+    const unsigned beam = i / out_count;
+    const float x = (float)i * step;
+    const float xb = x + (float)beam;
+    const unsigned k = (unsigned)xb;
+    const float o = (x - (float)k) + (float)beam;
+    if (frac_gt(o)) {
+      const float a = in[k];
+      const float d = in[k + 1] - a;
+      const float p = o * d;
+      return a + p;
+    }
+    return in[k] + 0.f;
+  }
+};
+
+// reblock_abs_beam_functor index (Pipeline/get_rms.cu:214-229):
+// (size_t)((float)beam*delta + (float)o), beam = o/length.
+__device__ __forceinline__ size_t reblock_pos(unsigned o, size_t length, float delta) {
+  const size_t beam = o / length;
+  const float p = (float)beam * delta;
+  const float q = p + (float)o;
+  return (size_t)q;
+}
+
+// Orderable key of a float: unsigned compare of keys matches float compare
+// (for non-NaN values).
+__device__ __forceinline__ uint32_t float_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+  const uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide helpers.  The block size is a compile-time constant BS and
+// every thread calls these.
+
+// Reduces `n` values in `src` by median-of-5 groups `levels` times, ping-
+// ponging with `dst`; returns a pointer to the buffer holding the result
+// (n / 5^levels values).  Requires n >= 5^levels.
+template <int BS>
+__device__ float* lds_pyramid(float* src, float* dst, int n, int levels) {
+  for (int l = 0; l < levels; ++l) {
+    const int m = n / 5;
+    __syncthreads();
+    for (int j = threadIdx.x; j < m; j += BS) {
+      const float* p = src + 5 * j;
+      dst[j] = med5(p[0], p[1], p[2], p[3], p[4]);
+    }
+    float* t = src;
+    src = dst;
+    dst = t;
+    n = m;
+  }
+  __syncthreads();
+  return src;
+}
+
+// Median of 5^L consecutive leaves starting at `first` (L in 0..3), leaves
+// produced by the functor `leaf(k)`.
+template <typename Leaf>
+__device__ __forceinline__ float tree_value(const Leaf& leaf, size_t first, int L) {
+  if (L == 0) return leaf(first);
+  if (L == 1) return med5(leaf(first), leaf(first + 1), leaf(first + 2), leaf(first + 3),
+                          leaf(first + 4));
+  if (L == 2) {
+    float m[5];
+#pragma unroll
+    for (int g = 0; g < 5; ++g) {
+      const size_t b = first + 5 * g;
+      m[g] = med5(leaf(b), leaf(b + 1), leaf(b + 2), leaf(b + 3), leaf(b + 4));
+    }
+    return med5(m[0], m[1], m[2], m[3], m[4]);
+  }
+  float n2[5];
+  for (int h = 0; h < 5; ++h) {
+    float m[5];
+#pragma unroll
+    for (int g = 0; g < 5; ++g) {
+      const size_t b = first + 25 * h + 5 * g;
+      m[g] = med5(leaf(b), leaf(b + 1), leaf(b + 2), leaf(b + 3), leaf(b + 4));
+    }
+    n2[h] = med5(m[0], m[1], m[2], m[3], m[4]);
+  }
+  return med5(n2[0], n2[1], n2[2], n2[3], n2[4]);
+}
+
+}  // namespace hd

@@ -1,0 +1,167 @@
+// hd_internal.hpp -- shapes and kernel launchers shared between the host
+// orchestration (hd_api.cpp) and the kernel sources (*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace hd {
+
+// Geometry of one gulp's DM series (one hd_execute call, scrunch 1).
+// Names follow Pipeline/pipeline.cu:488-499 and remove_baseline.cu:21-46.
+struct SeriesShape {
+  unsigned nbeams;
+  size_t stride;       // cur_beam_stride = nsamps/nbeams
+  size_t nout;         // nsamps_computed = nsamps - max_delay
+  size_t bc;           // cur_beam_nsamps = stride - max_delay
+  size_t ppitch;       // plane row pitch in floats (>= nout+1, zero past nout)
+  size_t spitch;       // prefix-sum row pitch in floats (>= nout+1)
+  // baseline (remove_baseline.cu:38-46)
+  size_t sc;           // sample_count
+  size_t count_round;  // 5^nscr * sample_count
+  int nscr;            // nscrunches
+  int bl_direct;       // pyramid levels computed straight from the leaves
+  float step_rs_beam;  // float(stride-1)/count_round   (beams 0..nb-2)
+  float step_rs_last;  // float(bc-1)/(count_round-1)   (last beam)
+  float step_bl_beam;  // float(sc-1)/stride            (beams 0..nb-2)
+  float step_bl_last;  // float(sc-1)/(bc-1)            (last beam)
+  // per-beam RMS (get_rms.cu:66-100)
+  int rms_direct;
+  float rms_delta;     // float(stride - bc)
+};
+
+// One boxcar width of the filter loop (pipeline.cu:657-692).
+struct WidthInfo {
+  unsigned w;          // rel_filter_width
+  unsigned ts;         // rel_tscrunch_width
+  unsigned ahead;      // (w-1)/2+1
+  unsigned behind;     // w/2
+  unsigned n_w;        // cur_nsamps_filtered
+  unsigned filter_idx; // log2 w
+  unsigned merge_dist; // cand_sep_time * (w/ts)
+  unsigned rel_off;    // (boxcar_max/2)/ts
+};
+
+constexpr int kMaxWidths = 24;
+
+inline size_t pow5_host(int L) {
+  size_t p = 1;
+  for (int i = 0; i < L; ++i) p *= 5;
+  return p;
+}
+
+struct WidthTable {
+  int nw;
+  unsigned off;        // boxcar_max/2
+  WidthInfo wi[kMaxWidths];
+};
+
+// Block-RMS pyramid geometry (get_rms.cu:104-131) for one width.
+struct BlockRmsShape {
+  size_t n;            // pooled count = nbeams * bc692
+  size_t bc;           // bc692 = stride - (max_delay + boxcar_max)
+  float delta;         // float(stride - bc692)
+  int L0;              // levels computed per chunk
+  size_t chunk;        // 5^L0
+  size_t np;           // n / 5^L0 partial values
+};
+
+// ---- launchers (return hipError_t of the launch) ----
+hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
+                         const int* d_live, int n_live, uint8_t* x, size_t xpitch,
+                         hipStream_t s);
+
+struct DedispGeom {
+  int n_dm, n_live;
+  size_t nout;         // outputs per DM
+  size_t xpitch;       // bytes per unpacked channel row
+  size_t ppitch;       // floats per output row
+  int wlen;            // LDS window bytes per channel (multiple of 4)
+};
+hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const DedispGeom& g,
+                             float* plane, hipStream_t s);
+int dedisp_tile_time();     // outputs per block in time
+int dedisp_tile_dms();      // DMs per block
+
+hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
+                                  float* pts, hipStream_t s);
+hipError_t launch_beam_rms(const float* plane, const float* pts, const SeriesShape& sh,
+                           int n_dm, float* rms, hipStream_t s);
+hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
+                            const SeriesShape& sh, int n_dm, hipStream_t s);
+hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series, float* S,
+                       size_t spitch, hipStream_t s);
+
+hipError_t launch_block_rms_partials(const float* S, const SeriesShape& sh, const WidthTable& wt,
+                                     const BlockRmsShape& bs, int first_w, int n_w_group,
+                                     int n_dm, float* partial, const float* winv,
+                                     hipStream_t s);
+hipError_t launch_block_rms_finish(const float* partial, const BlockRmsShape& bs, int nw_total,
+                                   int first_w, int n_w_group, int n_dm, float* wrms,
+                                   float* winv, hipStream_t s);
+
+struct GiantBuffers {
+  uint64_t* skey;      // (dm*NW+w)<<32 | i   (sample records, ordered later)
+  float* sval;
+  size_t scap;         // sample capacity
+  unsigned* counter;   // samples reserved (atomic)
+  unsigned* ccount;    // per chunk key
+  unsigned* coff;      // per chunk key: offset in the unordered buffer
+  unsigned* cpos;      // per chunk key: offset in the ordered buffer
+  uint64_t* okey;      // ordered samples
+  float* oval;
+  unsigned* head;      // per ordered sample
+  unsigned* segid;
+  unsigned* scan_tmp;
+  size_t nchunks_per_w;
+  size_t chunk;        // outputs per chunk
+};
+hipError_t launch_threshold(const float* S, size_t spitch, const SeriesShape& sh,
+                            const WidthTable& wt, const float* winv, int n_dm, float thresh,
+                            size_t bc692, GiantBuffers& gb, hipStream_t s);
+hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n,
+                                     unsigned* tmp, hipStream_t s);
+size_t exclusive_scan_tmp_words(size_t n);
+hipError_t launch_order_samples(GiantBuffers& gb, size_t nkeys, hipStream_t s);
+hipError_t launch_segment_heads(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
+                                hipStream_t s);
+// giants SoA outputs
+struct GiantOut {
+  float* peak;
+  uint64_t* ind;
+  uint64_t* begin;
+  uint64_t* end;
+  uint32_t* filter;
+  uint32_t* dm;
+};
+hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
+                                 GiantOut go, hipStream_t s);
+
+// clustering / merge
+struct ClusterBuffers {
+  uint64_t* sort_keys;   // padded power of two
+  uint32_t* lo;
+  uint32_t* hi;
+  int32_t* labels;
+  unsigned* flag;
+  uint64_t* best;
+  unsigned* members;
+  unsigned* root_flag;
+  unsigned* root_pos;
+  unsigned* scan_tmp;
+  size_t sort_n;
+};
+hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uint32_t* dm,
+                          const uint64_t* members_in, unsigned n, uint64_t time_tol,
+                          uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam,
+                          unsigned max_filter, ClusterBuffers& cb, hipStream_t s,
+                          unsigned* h_flag_pinned, int* iterations);
+hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* begin,
+                        const uint64_t* end, const uint32_t* filter, const uint32_t* dm,
+                        const uint64_t* members_in, unsigned n, ClusterBuffers& cb,
+                        float* g_peak, uint64_t* g_ind, uint64_t* g_begin, uint64_t* g_end,
+                        uint64_t* g_filter, uint64_t* g_dm, uint64_t* g_members,
+                        hipStream_t s);
+
+}  // namespace hd

@@ -1,0 +1,217 @@
+// k_cluster.hip -- candidate clustering and merging.
+//
+// Clustering (Pipeline/label_candidate_clusters.cu:141-197, are_coincident.cuh:
+// 46-68).  The reference relabels every giant i to the minimum label over all
+// coincident giants in one parallel pass and then traces label chains; with a
+// single thread (the sequential semantics this build adopts, since the
+// parallel pass is order dependent) that is
+//     L[i] = min(i, min{ L[j] : j < i, coincident(i, j) }),
+// i.e. L[i] is the smallest index reachable from i along coincident links to
+// earlier giants.  Here that fixed point is reached by repeated in-place
+// relaxation with pointer jumping (L[i] <- min(L[i], L[L[i]], L[j])): every
+// label stays a reachable index and only decreases, so the iteration stops at
+// the unique fixed point.  Neighbours are found through a sort of the giants
+// by sample index, so each giant only visits giants within the widest time
+// tolerance.
+//
+// Merging (merge_candidates.cu:21-72,102-131): giants sharing a root form a
+// group, ordered by root; its fields come from the first (lowest index) giant
+// with the maximum S/N and its member count is the sum.  A 64-bit atomic max
+// over (orderable S/N, ~index) picks that giant.
+#include "hd_device.hpp"
+#include "hd_internal.hpp"
+
+namespace hd {
+
+namespace {
+
+__global__ void k_sort_init(const uint64_t* __restrict__ ind, unsigned n, uint64_t* keys,
+                            size_t npad) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npad) return;
+  keys[k] = (k < n) ? ((ind[k] << 32) | (uint64_t)k) : ~0ull;
+}
+
+__global__ void k_bitonic_step(uint64_t* keys, size_t npad, size_t j, size_t kk) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  const size_t l = i ^ j;
+  if (l <= i) return;
+  const uint64_t a = keys[i], b = keys[l];
+  const bool up = ((i & kk) == 0);
+  if ((a > b) == up) {
+    keys[i] = b;
+    keys[l] = a;
+  }
+}
+
+__device__ __forceinline__ unsigned lower_bound_ind(const uint64_t* keys, unsigned n,
+                                                    uint64_t v) {
+  unsigned lo = 0, hi = n;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi) >> 1;
+    if ((keys[mid] >> 32) < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_windows(const uint64_t* __restrict__ ind, unsigned n,
+                          const uint64_t* __restrict__ keys, uint64_t span, uint32_t* lo,
+                          uint32_t* hi, int32_t* labels) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t t = ind[i];
+  lo[i] = lower_bound_ind(keys, n, t > span ? t - span : 0);
+  hi[i] = lower_bound_ind(keys, n, t + span + 1);
+  labels[i] = (int32_t)i;
+}
+
+// are_coincident_beam (are_coincident.cuh:46-68) with the reference's int
+// casts: time tolerance scaled by the wider of the two boxcars.
+__device__ __forceinline__ bool coincident(uint64_t ii, uint64_t ij, uint32_t fi, uint32_t fj,
+                                           uint32_t di, uint32_t dj, uint64_t time_tol,
+                                           uint64_t filter_tol, uint64_t dm_tol,
+                                           uint64_t nsamps_beam) {
+  if (ii / nsamps_beam != ij / nsamps_beam) return false;
+  const int wi = 1 << (int)fi, wj = 1 << (int)fj;
+  const uint64_t tt = time_tol * (uint64_t)(wi > wj ? wi : wj);
+  const int dt = abs((int)ij - (int)ii);
+  const int df = abs((int)fj - (int)fi);
+  const int dd = abs((int)dj - (int)di);
+  return (uint64_t)dt <= tt && (uint64_t)df <= filter_tol && (uint64_t)dd <= dm_tol;
+}
+
+__global__ void k_relax(const uint64_t* __restrict__ ind, const uint32_t* __restrict__ filter,
+                        const uint32_t* __restrict__ dm, unsigned n,
+                        const uint64_t* __restrict__ keys, const uint32_t* __restrict__ lo,
+                        const uint32_t* __restrict__ hi, uint64_t time_tol, uint64_t filter_tol,
+                        uint64_t dm_tol, uint64_t nsamps_beam, int32_t* labels,
+                        unsigned* flag) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t cur = __atomic_load_n(&labels[i], __ATOMIC_RELAXED);
+  int32_t best = cur;
+  const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
+  if (jump < best) best = jump;
+  const uint64_t ii = ind[i];
+  const uint32_t fi = filter[i], di = dm[i];
+  for (unsigned p = lo[i]; p < hi[i]; ++p) {
+    const unsigned j = (unsigned)(keys[p] & 0xffffffffull);
+    if (j >= i) continue;
+    if (!coincident(ii, ind[j], fi, filter[j], di, dm[j], time_tol, filter_tol, dm_tol,
+                    nsamps_beam))
+      continue;
+    const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
+    if (lj < best) best = lj;
+  }
+  if (best < cur) {
+    __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
+    *flag = 1u;
+  }
+}
+
+__global__ void k_merge_init(unsigned n, const int32_t* __restrict__ labels, uint64_t* best,
+                             unsigned* members, unsigned* root_flag) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  best[i] = 0;
+  members[i] = 0;
+  root_flag[i] = (labels[i] == (int32_t)i) ? 1u : 0u;
+}
+
+__global__ void k_merge_accum(unsigned n, const int32_t* __restrict__ labels,
+                              const float* __restrict__ peak,
+                              const uint64_t* __restrict__ members_in, uint64_t* best,
+                              unsigned* members) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned r = (unsigned)labels[i];
+  const uint64_t key = ((uint64_t)float_key(peak[i]) << 32) | (uint64_t)(0xffffffffu - i);
+  atomicMax((unsigned long long*)&best[r], (unsigned long long)key);
+  atomicAdd(&members[r], members_in ? (unsigned)members_in[i] : 1u);
+}
+
+__global__ void k_merge_write(unsigned n, const unsigned* __restrict__ root_flag,
+                              const unsigned* __restrict__ root_pos,
+                              const uint64_t* __restrict__ best,
+                              const unsigned* __restrict__ members, const float* peak,
+                              const uint64_t* ind, const uint64_t* begin, const uint64_t* end,
+                              const uint32_t* filter, const uint32_t* dm, float* g_peak,
+                              uint64_t* g_ind, uint64_t* g_begin, uint64_t* g_end,
+                              uint64_t* g_filter, uint64_t* g_dm, uint64_t* g_members) {
+  const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || !root_flag[r]) return;
+  const unsigned g = root_pos[r];
+  const unsigned src = 0xffffffffu - (unsigned)(best[r] & 0xffffffffull);
+  g_peak[g] = peak[src];
+  g_ind[g] = ind[src];
+  g_begin[g] = begin[src];
+  g_end[g] = end[src];
+  g_filter[g] = filter[src];
+  g_dm[g] = dm[src];
+  g_members[g] = members[r];
+}
+
+}  // namespace
+
+hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uint32_t* dm,
+                          const uint64_t* members_in, unsigned n, uint64_t time_tol,
+                          uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam,
+                          unsigned max_filter, ClusterBuffers& cb, hipStream_t s,
+                          unsigned* h_flag_pinned, int* iterations) {
+  (void)members_in;
+  *iterations = 0;
+  if (n == 0) return hipSuccess;
+  size_t npad = 1;
+  while (npad < n) npad <<= 1;
+  const unsigned T = 256;
+  hipLaunchKernelGGL(k_sort_init, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s, ind, n,
+                     cb.sort_keys, npad);
+  for (size_t kk = 2; kk <= npad; kk <<= 1)
+    for (size_t j = kk >> 1; j > 0; j >>= 1)
+      hipLaunchKernelGGL(k_bitonic_step, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s,
+                         cb.sort_keys, npad, j, kk);
+  const uint64_t span = time_tol << max_filter;
+  hipLaunchKernelGGL(k_windows, dim3((n + T - 1) / T), dim3(T), 0, s, ind, n, cb.sort_keys, span,
+                     cb.lo, cb.hi, cb.labels);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  for (int it = 0; it < 100000; ++it) {
+    e = hipMemsetAsync(cb.flag, 0, sizeof(unsigned), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_relax, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter, dm, n,
+                       cb.sort_keys, cb.lo, cb.hi, time_tol, filter_tol, dm_tol, nsamps_beam,
+                       cb.labels, cb.flag);
+    e = hipMemcpyAsync(h_flag_pinned, cb.flag, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    *iterations = it + 1;
+    if (*h_flag_pinned == 0) break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* begin,
+                        const uint64_t* end, const uint32_t* filter, const uint32_t* dm,
+                        const uint64_t* members_in, unsigned n, ClusterBuffers& cb,
+                        float* g_peak, uint64_t* g_ind, uint64_t* g_begin, uint64_t* g_end,
+                        uint64_t* g_filter, uint64_t* g_dm, uint64_t* g_members,
+                        hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned T = 256;
+  const dim3 grid((n + T - 1) / T);
+  hipLaunchKernelGGL(k_merge_init, grid, dim3(T), 0, s, n, cb.labels, cb.best, cb.members,
+                     cb.root_flag);
+  hipLaunchKernelGGL(k_merge_accum, grid, dim3(T), 0, s, n, cb.labels, peak, members_in, cb.best,
+                     cb.members);
+  hipError_t e = launch_exclusive_scan_u32(cb.root_flag, cb.root_pos, n, cb.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_merge_write, grid, dim3(T), 0, s, n, cb.root_flag, cb.root_pos, cb.best,
+                     cb.members, peak, ind, begin, end, filter, dm, g_peak, g_ind, g_begin,
+                     g_end, g_filter, g_dm, g_members);
+  return hipGetLastError();
+}
+
+}  // namespace hd

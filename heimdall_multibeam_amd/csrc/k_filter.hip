@@ -1,0 +1,409 @@
+// k_filter.hip -- boxcar filtering, block RMS and giant finding for all
+// (DM, width) pairs of a gulp.
+//
+// The boxcar output of width w at index i is never materialised; every
+// consumer evaluates it from the prefix sums (Pipeline/matched_filter.cu:
+// 43-64):  F_w[i] = S[off + ahead + i*ts] - S[off - behind + i*ts].
+//
+//   block RMS  GetRMSPlanMB::exec_multibeam pooled form (get_rms.cu:104-131):
+//              each block reduces a 5^L0-leaf chunk of |F_w| to one value for
+//              every width of a group at once (the chunk's prefix sums are
+//              staged in LDS once); a finishing wave per (DM, width) runs the
+//              remaining levels and the small-count rule, then
+//              rms = (float)((double)med * 1.4862) and inv = 1.0f/rms.
+//              Widths with ts > 1 read what the reference's filtered buffer
+//              holds past the fresh decimated values: the normalised output of
+//              the latest earlier width that reached that position
+//              (pipeline.cu:680-712 reuse d_filtered_series).
+//   threshold  GiantFinder's selection (find_giants.cu:47-60,137-183): per
+//              (DM, width, 4096-output chunk) the samples above threshold are
+//              compacted in time order and appended to an unordered buffer;
+//              the per-chunk counts are scanned so that a second pass lays the
+//              samples out in the reference's order (DM, width, time).
+//   segments   head flags where a new giant starts (adjacent gap > merge_dist,
+//              find_giants.cu:79-87,197-210), their scan, and one thread per
+//              giant reducing its run to the first maximum, begin and end
+//              (:260-303), with the index rescaling of pipeline.cu:755-775.
+#include "hd_device.hpp"
+#include "hd_internal.hpp"
+
+namespace hd {
+
+namespace {
+
+constexpr int FB_BS = 256;
+
+__device__ __forceinline__ float raw_val(const float* S, const WidthInfo& wi, unsigned off,
+                                         size_t i) {
+  const size_t base = i * wi.ts;
+  const float a = S[off + wi.ahead + base];
+  const float b = S[off - wi.behind + base];
+  return a - b;
+}
+
+// What the reference's filtered buffer holds at `pos` while width j is being
+// processed: fresh output below n_w, otherwise the normalised value left by
+// the latest earlier width that wrote `pos`.
+__device__ __forceinline__ float filtered_at(const float* S, const WidthTable& wt,
+                                             const float* winv, int j, size_t pos) {
+  if (pos < wt.wi[j].n_w) return raw_val(S, wt.wi[j], wt.off, pos);
+  for (int jj = j - 1; jj >= 0; --jj)
+    if (pos < wt.wi[jj].n_w) return raw_val(S, wt.wi[jj], wt.off, pos) * winv[jj];
+  return 0.f;
+}
+
+template <int BS>
+__device__ float lds_finish_f(float* A, float* B, size_t n) {
+  __shared__ float result;
+  while (n > 4) {
+    const size_t m = n / 5;
+    __syncthreads();
+    for (size_t j = threadIdx.x; j < m; j += BS) {
+      const float* p = A + 5 * j;
+      B[j] = med5(p[0], p[1], p[2], p[3], p[4]);
+    }
+    float* t = A;
+    A = B;
+    B = t;
+    n = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) result = (n >= 1) ? med_tail(A, (int)n) : 0.f;
+  __syncthreads();
+  return result;
+}
+
+// One block per (chunk, DM): level-L0 medians of |F_w| for widths
+// [first_w, first_w + n_group).  `tile` selects the LDS-staged path, valid
+// when every width of the group has ts == 1.
+__global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
+    const float* __restrict__ Sall, size_t spitch, WidthTable wt, BlockRmsShape bs, int first_w,
+    int n_group, int tile_ok, unsigned amax, unsigned bmax, const float* __restrict__ winv_all,
+    float* __restrict__ partial) {
+  extern __shared__ float lds[];
+  const size_t c = blockIdx.x;
+  const int m = blockIdx.y;
+  const float* S = Sall + (size_t)m * spitch;
+  const float* winv = winv_all + (size_t)m * wt.nw;
+  const size_t k0 = c * bs.chunk;
+  const size_t pos0 = reblock_pos((unsigned)k0, bs.bc, bs.delta);
+  const size_t posl = reblock_pos((unsigned)(k0 + bs.chunk - 1), bs.bc, bs.delta);
+  const bool use_tile = tile_ok && (posl - pos0 == bs.chunk - 1);
+  const size_t tlen = bs.chunk + amax + bmax;
+  float* T = lds;
+  float* A = lds + (use_tile ? tlen : 0);
+  const size_t n1 = bs.chunk / 5;
+  float* B = A + (n1 ? n1 : 1);
+  if (use_tile) {
+    const float* src = S + pos0 + wt.off - bmax;
+    for (size_t t = threadIdx.x; t < tlen; t += FB_BS) T[t] = src[t];
+    __syncthreads();
+  }
+  for (int g = 0; g < n_group; ++g) {
+    const int j = first_w + g;
+    const WidthInfo wi = wt.wi[j];
+    auto leaf = [&](size_t k) -> float {
+      if (use_tile) {
+        const size_t r = k - k0;  // pos - pos0
+        const float a = T[r + bmax + wi.ahead];
+        const float b = T[r + bmax - wi.behind];
+        return fabsf(a - b);
+      }
+      const size_t pos = reblock_pos((unsigned)k, bs.bc, bs.delta);
+      return fabsf(filtered_at(S, wt, winv, j, pos));
+    };
+    float value;
+    if (bs.L0 == 0) {
+      value = leaf(k0);
+    } else {
+      for (size_t q = threadIdx.x; q < n1; q += FB_BS) {
+        const size_t b = k0 + 5 * q;
+        A[q] = med5(leaf(b), leaf(b + 1), leaf(b + 2), leaf(b + 3), leaf(b + 4));
+      }
+      float* res = lds_pyramid<FB_BS>(A, B, (int)n1, bs.L0 - 1);
+      value = res[0];
+    }
+    if (threadIdx.x == 0) partial[((size_t)m * wt.nw + j) * bs.np + c] = value;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void k_block_rms_finish(const float* __restrict__ partial,
+                                                         BlockRmsShape bs, int nw_total,
+                                                         int first_w, float* __restrict__ wrms,
+                                                         float* __restrict__ winv) {
+  extern __shared__ float lds[];
+  const int j = first_w + blockIdx.x;
+  const int m = blockIdx.y;
+  const float* p = partial + ((size_t)m * nw_total + j) * bs.np;
+  float* A = lds;
+  float* B = lds + bs.np;
+  for (size_t q = threadIdx.x; q < bs.np; q += 64) A[q] = p[q];
+  __syncthreads();
+  const float med = lds_finish_f<64>(A, B, bs.np);
+  if (threadIdx.x == 0) {
+    const float r = (float)((double)med * 1.4862);  // get_rms.cu:127-128
+    wrms[(size_t)m * nw_total + j] = r;
+    winv[(size_t)m * nw_total + j] = 1.0f / r;      // get_rms.cu:195
+  }
+}
+
+// Block-wide exclusive scan of one unsigned per thread; returns the prefix,
+// writes the total to *total.
+template <int BS>
+__device__ unsigned block_exclusive_scan(unsigned v, unsigned* total) {
+  __shared__ unsigned s[BS];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int d = 1; d < BS; d <<= 1) {
+    const unsigned t = (threadIdx.x >= (unsigned)d) ? s[threadIdx.x - d] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  const unsigned incl = s[threadIdx.x];
+  *total = s[BS - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+constexpr int TH_PER = 16;
+constexpr int TH_CHUNK = FB_BS * TH_PER;  // 4096 outputs per block
+
+__global__ __launch_bounds__(FB_BS) void k_threshold(
+    const float* __restrict__ Sall, size_t spitch, size_t stride, size_t bc692, WidthTable wt,
+    const float* __restrict__ winv_all, float thresh, GiantBuffers gb) {
+  const size_t c = blockIdx.x;
+  const int j = blockIdx.y;
+  const int m = blockIdx.z;
+  const WidthInfo wi = wt.wi[j];
+  const float* S = Sall + (size_t)m * spitch;
+  const float inv = winv_all[(size_t)m * wt.nw + j];
+  const size_t ib = c * TH_CHUNK + (size_t)threadIdx.x * TH_PER;
+  float vals[TH_PER];
+  unsigned mask = 0;
+#pragma unroll
+  for (int q = 0; q < TH_PER; ++q) {
+    const size_t i = ib + q;
+    vals[q] = 0.f;
+    if (i < wi.n_w) {
+      const float v = raw_val(S, wi, wt.off, i) * inv;  // get_rms.cu:194-197
+      vals[q] = v;
+      const size_t samp = (size_t)(unsigned)i % stride;  // find_giants.cu:55-58
+      if (samp < bc692 && v > thresh) mask |= 1u << q;
+    }
+  }
+  const unsigned cnt = __popc(mask);
+  unsigned total;
+  const unsigned rank = block_exclusive_scan<FB_BS>(cnt, &total);
+  __shared__ unsigned sbase;
+  const size_t key = ((size_t)m * wt.nw + j) * gb.nchunks_per_w + c;
+  if (threadIdx.x == 0) {
+    unsigned base = 0;
+    if (total) base = atomicAdd(gb.counter, total);
+    sbase = base;
+    gb.ccount[key] = total;
+    gb.coff[key] = base;
+  }
+  __syncthreads();
+  if (!mask) return;
+  const size_t base = sbase;
+  if (base + total > gb.scap) return;  // overflow: detected by the host
+  unsigned r = rank;
+  const uint64_t dmw = (uint64_t)m * wt.nw + j;
+#pragma unroll
+  for (int q = 0; q < TH_PER; ++q) {
+    if (mask & (1u << q)) {
+      gb.skey[base + r] = (dmw << 32) | (uint64_t)(ib + q);
+      gb.sval[base + r] = vals[q];
+      ++r;
+    }
+  }
+}
+
+// ---- device-wide exclusive scan of unsigned ints (1024 per block) ----
+constexpr int SC_BS = 256;
+constexpr int SC_PER = 4;
+constexpr int SC_TILE = SC_BS * SC_PER;
+
+__global__ __launch_bounds__(SC_BS) void k_scan_blocks(const unsigned* __restrict__ in,
+                                                       unsigned* __restrict__ out, size_t n,
+                                                       unsigned* __restrict__ sums) {
+  const size_t b0 = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_PER;
+  unsigned v[SC_PER];
+  unsigned local = 0;
+#pragma unroll
+  for (int q = 0; q < SC_PER; ++q) {
+    v[q] = (b0 + q < n) ? in[b0 + q] : 0u;
+    local += v[q];
+  }
+  unsigned total;
+  unsigned pre = block_exclusive_scan<SC_BS>(local, &total);
+#pragma unroll
+  for (int q = 0; q < SC_PER; ++q) {
+    if (b0 + q < n) out[b0 + q] = pre;
+    pre += v[q];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SC_BS) void k_scan_add(unsigned* __restrict__ out, size_t n,
+                                                    const unsigned* __restrict__ offs) {
+  const size_t b0 = (size_t)blockIdx.x * SC_TILE;
+  const unsigned add = offs[blockIdx.x];
+  for (int q = threadIdx.x; q < SC_TILE; q += SC_BS)
+    if (b0 + q < n) out[b0 + q] += add;
+}
+
+__global__ void k_order_samples(GiantBuffers gb, size_t nkeys) {
+  const size_t key = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (key >= nkeys) return;
+  const unsigned cnt = gb.ccount[key];
+  if (!cnt) return;
+  const size_t src = gb.coff[key];
+  const size_t dst = gb.cpos[key];
+  for (unsigned q = 0; q < cnt; ++q) {
+    gb.okey[dst + q] = gb.skey[src + q];
+    gb.oval[dst + q] = gb.sval[src + q];
+  }
+}
+
+__global__ void k_segment_heads(GiantBuffers gb, WidthTable wt, unsigned nsamp) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsamp) return;
+  const uint64_t key = gb.okey[k];
+  unsigned h = 1;
+  if (k > 0) {
+    const uint64_t prev = gb.okey[k - 1];
+    if ((prev >> 32) == (key >> 32)) {
+      const unsigned j = (unsigned)((key >> 32) % (uint64_t)wt.nw);
+      const uint64_t i = key & 0xffffffffull;
+      const uint64_t ip = prev & 0xffffffffull;
+      h = (i > ip + wt.wi[j].merge_dist) ? 1u : 0u;
+    }
+  }
+  gb.head[k] = h;
+}
+
+__global__ void k_segment_reduce(GiantBuffers gb, WidthTable wt, unsigned nsamp, GiantOut go) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsamp || !gb.head[k]) return;
+  const unsigned g = gb.segid[k];
+  const uint64_t key = gb.okey[k];
+  const uint64_t dmw = key >> 32;
+  const unsigned j = (unsigned)(dmw % (uint64_t)wt.nw);
+  const unsigned m = (unsigned)(dmw / (uint64_t)wt.nw);
+  float peak = gb.oval[k];
+  uint64_t pind = key & 0xffffffffull;
+  const uint64_t first = pind;
+  uint64_t last = pind;
+  for (unsigned kk = k + 1; kk < nsamp && !gb.head[kk]; ++kk) {
+    const float v = gb.oval[kk];
+    const uint64_t i = gb.okey[kk] & 0xffffffffull;
+    if (!(peak >= v)) {  // maximum_first keeps the earlier of equal peaks
+      peak = v;
+      pind = i;
+    }
+    last = i;
+  }
+  const WidthInfo wi = wt.wi[j];
+  const uint64_t ro = wi.rel_off, sc = wi.ts;
+  go.peak[g] = peak;
+  go.ind[g] = (pind + ro) * sc;
+  go.begin[g] = (first + ro) * sc;
+  go.end[g] = (last + 1 + ro) * sc;
+  go.filter[g] = wi.filter_idx;
+  go.dm[g] = m;
+}
+
+}  // namespace
+
+hipError_t launch_block_rms_partials(const float* S, const SeriesShape& sh, const WidthTable& wt,
+                                     const BlockRmsShape& bs, int first_w, int n_group, int n_dm,
+                                     float* partial, const float* winv, hipStream_t s) {
+  if (bs.np == 0 || n_group == 0) return hipSuccess;
+  int tile_ok = 1;
+  unsigned amax = 0, bmax = 0;
+  for (int g = 0; g < n_group; ++g) {
+    const WidthInfo& wi = wt.wi[first_w + g];
+    if (wi.ts != 1) tile_ok = 0;
+    if (wi.ahead > amax) amax = wi.ahead;
+    if (wi.behind > bmax) bmax = wi.behind;
+  }
+  const size_t tlen = bs.chunk + amax + bmax;
+  const size_t n1 = bs.chunk / 5;
+  if ((tlen + n1 + n1 / 5 + 8) * sizeof(float) > 65536) tile_ok = 0;  // keep LDS <= 64 KiB
+  size_t lds = ((tile_ok ? tlen : 0) + (n1 ? n1 : 1) + n1 / 5 + 8) * sizeof(float);
+  hipLaunchKernelGGL(k_block_rms_partials, dim3((unsigned)bs.np, n_dm), dim3(FB_BS), lds, s, S,
+                     sh.spitch, wt, bs, first_w, n_group, tile_ok, amax, bmax, winv, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_rms_finish(const float* partial, const BlockRmsShape& bs, int nw_total,
+                                   int first_w, int n_group, int n_dm, float* wrms, float* winv,
+                                   hipStream_t s) {
+  if (n_group == 0) return hipSuccess;
+  const size_t lds = (bs.np + bs.np / 5 + 8) * sizeof(float);
+  if (lds > 65536) return hipErrorInvalidValue;  // pooled count beyond ~40M samples
+  hipLaunchKernelGGL(k_block_rms_finish, dim3(n_group, n_dm), dim3(64), lds, s, partial, bs,
+                     nw_total, first_w, wrms, winv);
+  return hipGetLastError();
+}
+
+hipError_t launch_threshold(const float* S, size_t spitch, const SeriesShape& sh,
+                            const WidthTable& wt, const float* winv, int n_dm, float thresh,
+                            size_t bc692, GiantBuffers& gb, hipStream_t s) {
+  gb.chunk = TH_CHUNK;
+  dim3 grid((unsigned)gb.nchunks_per_w, wt.nw, n_dm);
+  hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), 0, s, S, spitch, sh.stride, bc692, wt, winv,
+                     thresh, gb);
+  return hipGetLastError();
+}
+
+size_t exclusive_scan_tmp_words(size_t n) {
+  size_t words = 0;
+  while (n > 1) {
+    const size_t nb = (n + SC_TILE - 1) / SC_TILE;
+    words += 2 * nb + 2;
+    n = nb;
+  }
+  return words + 4;
+}
+
+hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n, unsigned* tmp,
+                                     hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t nb = (n + SC_TILE - 1) / SC_TILE;
+  unsigned* sums = tmp;
+  unsigned* offs = tmp + nb + 1;
+  hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(SC_BS), 0, s, in, out, n, sums);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || nb == 1) return e;
+  e = launch_exclusive_scan_u32(sums, offs, nb, tmp + 2 * nb + 2, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SC_BS), 0, s, out, n, offs);
+  return hipGetLastError();
+}
+
+hipError_t launch_order_samples(GiantBuffers& gb, size_t nkeys, hipStream_t s) {
+  hipLaunchKernelGGL(k_order_samples, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s, gb,
+                     nkeys);
+  return hipGetLastError();
+}
+
+hipError_t launch_segment_heads(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
+                                hipStream_t s) {
+  if (!nsamp) return hipSuccess;
+  hipLaunchKernelGGL(k_segment_heads, dim3((nsamp + 255) / 256), dim3(256), 0, s, gb, wt, nsamp);
+  return hipGetLastError();
+}
+
+hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
+                                 GiantOut go, hipStream_t s) {
+  if (!nsamp) return hipSuccess;
+  hipLaunchKernelGGL(k_segment_reduce, dim3((nsamp + 255) / 256), dim3(256), 0, s, gb, wt, nsamp,
+                     go);
+  return hipGetLastError();
+}
+
+}  // namespace hd

@@ -1,0 +1,215 @@
+// k_normalise.hip -- baseline removal, per-beam RMS, normalisation and the
+// matched filter's prefix sum, over all DM series of a gulp at once.
+//
+//   baseline   RemoveBaselinePlan::exec (Pipeline/remove_baseline.cu:21-160):
+//              one block per (beam, DM) resamples the beam to count_round
+//              points, median-of-5 scrunches them down to sample_count points
+//              and stores those points; the stretched baseline is recomputed
+//              from them wherever it is needed (never materialised).
+//   beam rms   GetRMSPlanMB::exec_multibeam per-beam form (get_rms.cu:66-100):
+//              median-of-5 pyramid of |x - baseline| per (beam, DM) block.
+//   normalise  normalise_multibeam (get_rms.cu:202-212): z = (x-baseline)/rms,
+//              written in place over the DM plane.
+//   scan       MatchedFilterPlan::prep (matched_filter.cu:21-32): the exclusive
+//              prefix sum in the reference's sequential order, one lane per
+//              series, 64x64 tiles transposed through LDS so that HBM reads
+//              and writes stay coalesced.
+#include "hd_device.hpp"
+#include "hd_internal.hpp"
+
+namespace hd {
+
+namespace {
+
+constexpr int NB_BS = 256;
+
+__device__ __forceinline__ size_t pow5(int L) {
+  size_t p = 1;
+  for (int i = 0; i < L; ++i) p *= 5;
+  return p;
+}
+
+// Finishes a median-of-5 pyramid held in A (n values) with the reference's
+// loop `for (size = n; size > 1; size /= 5)`: groups of five while size > 4,
+// then the small-count rule.  Returns the final value to every thread.
+template <int BS>
+__device__ float lds_finish(float* A, float* B, size_t n) {
+  __shared__ float result;
+  while (n > 4) {
+    const size_t m = n / 5;
+    __syncthreads();
+    for (size_t j = threadIdx.x; j < m; j += BS) {
+      const float* p = A + 5 * j;
+      B[j] = med5(p[0], p[1], p[2], p[3], p[4]);
+    }
+    float* t = A;
+    A = B;
+    B = t;
+    n = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) result = (n >= 1) ? med_tail(A, (int)n) : 0.f;
+  __syncthreads();
+  return result;
+}
+
+// The stretched baseline at series position pos (remove_baseline.cu:88-90):
+// beams 0..nb-2 through linear_stretch_beam over their full stride, the last
+// beam through linear_stretch over bc samples.
+__device__ __forceinline__ float baseline_at(const float* P, const SeriesShape& sh, size_t pos) {
+  if (sh.sc == 0) return 0.f;
+  const size_t split = (size_t)(sh.nbeams - 1) * sh.stride;
+  if (pos < split) {
+    StretchBeam f{sh.step_bl_beam, (unsigned)sh.stride};
+    return f(P, (unsigned)pos);
+  }
+  Stretch f{sh.step_bl_last};
+  return f(P + (size_t)(sh.nbeams - 1) * sh.sc, (unsigned)(pos - split));
+}
+
+__global__ __launch_bounds__(NB_BS) void k_baseline_points(const float* __restrict__ plane,
+                                                           SeriesShape sh,
+                                                           float* __restrict__ pts) {
+  extern __shared__ float lds[];
+  const unsigned b = blockIdx.x;
+  const int m = blockIdx.y;
+  const float* d = plane + (size_t)m * sh.ppitch;
+  const int Ld = sh.bl_direct;
+  const size_t per = pow5(Ld);
+  const size_t nl = sh.count_round / per;
+  float* A = lds;
+  float* B = lds + nl;
+  const bool last = (b == sh.nbeams - 1);
+  const float* dl = d + (size_t)(sh.nbeams - 1) * sh.stride;
+  const size_t gbase = (size_t)b * sh.count_round;
+  // resampled value i of beam b (remove_baseline.cu:73-75)
+  auto leaf = [&](size_t i) -> float {
+    if (last) {
+      Stretch f{sh.step_rs_last};
+      return f(dl, (unsigned)i);
+    }
+    StretchBeam f{sh.step_rs_beam, (unsigned)sh.count_round};
+    return f(d, (unsigned)(gbase + i));
+  };
+  for (size_t j = threadIdx.x; j < nl; j += NB_BS) A[j] = tree_value(leaf, j * per, Ld);
+  float* res = lds_pyramid<NB_BS>(A, B, (int)nl, sh.nscr - Ld);
+  float* o = pts + ((size_t)m * sh.nbeams + b) * sh.sc;
+  for (size_t j = threadIdx.x; j < sh.sc; j += NB_BS) o[j] = res[j];
+}
+
+__global__ __launch_bounds__(NB_BS) void k_beam_rms(const float* __restrict__ plane,
+                                                    const float* __restrict__ pts,
+                                                    SeriesShape sh, float* __restrict__ rms) {
+  extern __shared__ float lds[];
+  const unsigned b = blockIdx.x;
+  const int m = blockIdx.y;
+  const float* d = plane + (size_t)m * sh.ppitch;
+  const float* P = pts + (size_t)m * sh.nbeams * sh.sc;
+  const int Ld = sh.rms_direct;
+  const size_t per = pow5(Ld);
+  const size_t nl = sh.bc / per;
+  float* A = lds;
+  float* B = lds + nl;
+  const size_t obase = (size_t)b * sh.bc;
+  // |x - baseline| at the reblocked position (get_rms.cu:81, 214-229)
+  auto leaf = [&](size_t k) -> float {
+    const size_t pos = reblock_pos((unsigned)(obase + k), sh.bc, sh.rms_delta);
+    const float y = d[pos] - baseline_at(P, sh, pos);
+    return fabsf(y);
+  };
+  for (size_t j = threadIdx.x; j < nl; j += NB_BS) A[j] = tree_value(leaf, j * per, Ld);
+  const float med = lds_finish<NB_BS>(A, B, nl);
+  if (threadIdx.x == 0) rms[(size_t)m * sh.nbeams + b] = med * 1.4862f;  // get_rms.cu:94-97
+}
+
+// z = (x - baseline) / rms[pos / stride], in place (remove_baseline.cu:99-102,
+// get_rms.cu:183-184).  Position nout (one past the series) is left as is.
+__global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
+                                                   const float* __restrict__ pts,
+                                                   const float* __restrict__ rms,
+                                                   SeriesShape sh) {
+  const int m = blockIdx.y;
+  float* d = plane + (size_t)m * sh.ppitch;
+  const float* P = pts + (size_t)m * sh.nbeams * sh.sc;
+  const float* R = rms + (size_t)m * sh.nbeams;
+  for (size_t pos = (size_t)blockIdx.x * 256 + threadIdx.x; pos < sh.nout;
+       pos += (size_t)gridDim.x * 256) {
+    const float y = d[pos] - baseline_at(P, sh, pos);
+    const unsigned beam = (unsigned)((unsigned)pos / sh.stride);
+    d[pos] = y / R[beam];
+  }
+}
+
+// Sequential exclusive scan per series: S[0] = 0, S[k] = S[k-1] + z[k-1].
+__global__ __launch_bounds__(64) void k_scan(const float* __restrict__ z, size_t zpitch,
+                                             size_t count, int n_series,
+                                             float* __restrict__ S, size_t spitch) {
+  __shared__ float tile[64][65];
+  const int lane = threadIdx.x;
+  const int r0 = blockIdx.x * 64;
+  const int myrow = r0 + lane;
+  float run = 0.f;
+  if (myrow < n_series) S[(size_t)myrow * spitch] = 0.f;
+  for (size_t t0 = 0; t0 < count; t0 += 64) {
+    const size_t t = t0 + lane;
+#pragma unroll 8
+    for (int r = 0; r < 64; ++r) {
+      const int row = r0 + r;
+      float v = 0.f;
+      if (row < n_series && t < count) v = z[(size_t)row * zpitch + t];
+      tile[r][lane] = v;
+    }
+    __syncthreads();
+    const int ncol = (int)((count - t0) < 64 ? (count - t0) : 64);
+    for (int c = 0; c < ncol; ++c) {
+      run = run + tile[lane][c];
+      tile[lane][c] = run;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int r = 0; r < 64; ++r) {
+      const int row = r0 + r;
+      if (row < n_series && t < count) S[(size_t)row * spitch + t + 1] = tile[r][lane];
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
+                                  float* pts, hipStream_t s) {
+  if (sh.sc == 0) return hipSuccess;
+  const size_t nl = sh.count_round / pow5_host(sh.bl_direct);
+  const size_t lds = (nl + nl / 5 + 8) * sizeof(float);
+  hipLaunchKernelGGL(k_baseline_points, dim3(sh.nbeams, n_dm), dim3(NB_BS), lds, s, plane, sh,
+                     pts);
+  return hipGetLastError();
+}
+
+hipError_t launch_beam_rms(const float* plane, const float* pts, const SeriesShape& sh,
+                           int n_dm, float* rms, hipStream_t s) {
+  const size_t nl = sh.bc / pow5_host(sh.rms_direct);
+  const size_t lds = (nl + nl / 5 + 8) * sizeof(float);
+  hipLaunchKernelGGL(k_beam_rms, dim3(sh.nbeams, n_dm), dim3(NB_BS), lds, s, plane, pts, sh,
+                     rms);
+  return hipGetLastError();
+}
+
+hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
+                            const SeriesShape& sh, int n_dm, hipStream_t s) {
+  size_t gx = (sh.nout + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(k_normalise, dim3((unsigned)gx, n_dm), dim3(256), 0, s, plane, pts, rms,
+                     sh);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series, float* S,
+                       size_t spitch, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan, dim3((n_series + 63) / 64), dim3(64), 0, s, z, zpitch, count,
+                     n_series, S, spitch);
+  return hipGetLastError();
+}
+
+}  // namespace hd

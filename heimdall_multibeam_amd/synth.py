@@ -1,0 +1,170 @@
+"""Seeded synthetic filterbanks for the search workloads (host side).
+
+Wraps libhd_synth.so (csrc/synth.cpp).  Every workload is described by a
+`Workload` (observing setup + search parameters) and turned into bytes by
+`make_gulp`, which draws the injected pulses from numpy's PCG64 with the
+given seed and places each at a trial DM so it lines up with the plan's
+delay table.
+"""
+import ctypes as C
+import dataclasses
+import os
+from typing import List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SYNTH_LIB = os.path.join(HERE, "libhd_synth.so")
+
+
+class _Pulse(C.Structure):
+    _fields_ = [("t0", C.c_double), ("dm", C.c_float), ("width", C.c_float), ("snr", C.c_float)]
+
+
+class _Tone(C.Structure):
+    _fields_ = [("chan", C.c_int), ("amp_sigma", C.c_float), ("period", C.c_float),
+                ("phase", C.c_float)]
+
+
+class _Spec(C.Structure):
+    _fields_ = [("nsamps", C.c_size_t), ("nchans", C.c_size_t), ("nbits", C.c_size_t),
+                ("dt", C.c_float), ("f0", C.c_float), ("df", C.c_float), ("seed", C.c_uint64),
+                ("mean", C.c_float), ("sigma", C.c_float), ("dead", C.POINTER(C.c_int)),
+                ("dead_value", C.c_float), ("pulses", C.POINTER(_Pulse)),
+                ("npulses", C.c_size_t), ("bursts", C.POINTER(_Pulse)), ("nbursts", C.c_size_t),
+                ("tones", C.POINTER(_Tone)), ("ntones", C.c_size_t)]
+
+
+_lib = None
+
+
+def _synth():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SYNTH_LIB):
+            raise RuntimeError("libhd_synth.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(SYNTH_LIB)
+        L.hd_synth_filterbank.restype = C.c_int
+        L.hd_synth_filterbank.argtypes = [C.POINTER(_Spec), C.c_void_p]
+        L.hd_synth_delay.restype = C.c_int
+        L.hd_synth_delay.argtypes = [C.c_float, C.c_size_t, C.c_float, C.c_float, C.c_float]
+        L.hd_synth_dm_list.restype = C.c_size_t
+        L.hd_synth_dm_list.argtypes = [C.POINTER(C.c_float), C.c_size_t, C.c_float, C.c_float,
+                                       C.c_float, C.c_float, C.c_float, C.c_float, C.c_size_t,
+                                       C.c_float]
+        _lib = L
+    return _lib
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    nchans: int
+    nbits: int
+    dt: float            # seconds
+    f0: float            # MHz, channel 0
+    df: float            # MHz per channel
+    nsamps: int          # per beam
+    dm_min: float = 0.0
+    dm_max: float = 2000.0
+    boxcar_max: int = 4096
+    detect_thresh: float = 6.0
+    n_pulses: int = 0
+    pulse_dm: tuple = (0.0, 2000.0)
+    pulse_width: tuple = (1.0, 32.0)
+    pulse_snr: tuple = (8.0, 30.0)
+    tone_frac: float = 0.0
+    tone_sigma: float = 10.0
+    n_bursts: int = 0
+    burst_snr: float = 20.0
+    mean: float = 128.0
+    sigma: float = 16.0
+
+
+# The workloads the search is measured and checked on.
+CONFIGS = {
+    # 1 beam, small: 512 ch, 256 us, gulp 65536, DM 0-500, boxcars 1..256
+    "small_512ch": Workload("small_512ch", 512, 8, 256e-6, 1500.0, -0.5, 65536, 0.0, 500.0, 256,
+                            6.0, 10, (50.0, 450.0), (1.0, 32.0), (8.0, 30.0)),
+    # 1 beam, full size: 1024 ch over 1182-1518 MHz, 64 us, gulp 262144, DM 0-2000,
+    # boxcars 1..4096, 5% tone channels at 10 sigma, 3 zero-DM bursts at 20 sigma
+    "full_1024ch": Workload("full_1024ch", 1024, 8, 64e-6, 1517.8359375, -0.328125, 262144, 0.0,
+                            2000.0, 4096, 6.0, 50, (0.0, 2000.0), (1.0, 512.0), (6.0, 50.0),
+                            0.05, 10.0, 3, 20.0),
+}
+
+
+def trial_dms(w: Workload, dm_tol: float = 1.25, pulse_width_us: float = 40.0) -> np.ndarray:
+    """The plan's DM trials for a workload (same recurrence as the library)."""
+    L = _synth()
+    fp = C.POINTER(C.c_float)
+    n = L.hd_synth_dm_list(None, 0, w.dm_min, w.dm_max, w.dt, pulse_width_us, w.f0, w.df,
+                           w.nchans, dm_tol)
+    out = np.zeros(n, np.float32)
+    L.hd_synth_dm_list(out.ctypes.data_as(fp), n, w.dm_min, w.dm_max, w.dt, pulse_width_us,
+                       w.f0, w.df, w.nchans, dm_tol)
+    return out
+
+
+def delay_samples(w: Workload, dm: float, chan: int) -> int:
+    return _synth().hd_synth_delay(dm, chan, w.dt, w.f0, w.df)
+
+
+def make_gulp(w: Workload, seed: int, nsamps: Optional[int] = None, dm_trials=None,
+              dead: Optional[np.ndarray] = None, dead_value: float = 128.0,
+              n_pulses: Optional[int] = None, return_truth: bool = False):
+    """Bytes [nsamps][nchans*nbits/8] for one beam of workload `w`."""
+    ns = int(nsamps if nsamps is not None else w.nsamps)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if dm_trials is None:
+        dm_trials = trial_dms(w)
+    dm_trials = np.asarray(dm_trials, np.float32)
+    max_delay = max(delay_samples(w, float(dm_trials[-1]), w.nchans - 1), 0)
+    npul = w.n_pulses if n_pulses is None else n_pulses
+    pulses = []
+    lo_t = w.boxcar_max
+    hi_t = max(ns - max_delay - w.boxcar_max, lo_t + 1)
+    for _ in range(npul):
+        dm = rng.uniform(*w.pulse_dm)
+        dm = float(dm_trials[int(np.argmin(np.abs(dm_trials - dm)))])
+        width = float(rng.uniform(*w.pulse_width))
+        snr = float(rng.uniform(*w.pulse_snr))
+        t0 = float(rng.integers(lo_t, hi_t))
+        pulses.append((t0, dm, width, snr))
+    tones = []
+    if w.tone_frac > 0:
+        ntone = int(round(w.tone_frac * w.nchans))
+        chans = rng.choice(w.nchans, ntone, replace=False)
+        for c in chans:
+            tones.append((int(c), w.tone_sigma, float(rng.uniform(20.0, 2000.0)),
+                          float(rng.uniform(0, 2 * np.pi))))
+    bursts = []
+    for _ in range(w.n_bursts):
+        bursts.append((float(rng.integers(lo_t, hi_t)), 0.0, float(rng.uniform(1.0, 16.0)),
+                       w.burst_snr))
+    out = np.empty(ns * w.nchans * w.nbits // 8, np.uint8)
+    spec = _Spec()
+    spec.nsamps = ns
+    spec.nchans = w.nchans
+    spec.nbits = w.nbits
+    spec.dt, spec.f0, spec.df = w.dt, w.f0, w.df
+    spec.seed = seed
+    spec.mean, spec.sigma = w.mean, w.sigma
+    keep = []
+    if dead is not None:
+        dead = np.ascontiguousarray(dead, np.int32)
+        keep.append(dead)
+        spec.dead = dead.ctypes.data_as(C.POINTER(C.c_int))
+    spec.dead_value = dead_value
+    P = (_Pulse * max(len(pulses), 1))(*[_Pulse(*p) for p in pulses])
+    B = (_Pulse * max(len(bursts), 1))(*[_Pulse(*p) for p in bursts])
+    T = (_Tone * max(len(tones), 1))(*[_Tone(*t) for t in tones])
+    spec.pulses, spec.npulses = P, len(pulses)
+    spec.bursts, spec.nbursts = B, len(bursts)
+    spec.tones, spec.ntones = T, len(tones)
+    rc = _synth().hd_synth_filterbank(C.byref(spec), out.ctypes.data)
+    if rc != 0:
+        raise ValueError("hd_synth_filterbank failed")
+    if return_truth:
+        return out, {"pulses": pulses, "tones": tones, "bursts": bursts}
+    return out

@@ -1,0 +1,57 @@
+/* hd/mi355x.h -- additive entry points of this implementation.
+ *
+ * None of these has a counterpart in the reference interface; they expose
+ * what the reference keeps inside hd_execute (candidates, giants, stage
+ * timings) and let callers hand over device-resident filterbanks or several
+ * beams of different lengths in one call.
+ */
+#pragma once
+
+#include "hd/types.h"
+#include "hd/error.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Candidate groups of the last hd_execute* call, before the per-beam valid
+ * window filter applied when writing the .cand file.  Copies at most `cap`
+ * entries into the host arrays (any may be NULL) and returns the count. */
+hd_size hd_get_last_candidates(hd_pipeline pl, hd_size cap, hd_float* peaks, hd_size* inds,
+                               hd_size* begins, hd_size* ends, hd_size* filter_inds,
+                               hd_size* dm_inds, hd_size* members, hd_float* dms);
+
+/* Giants of the last call in the reference's order (DM, then width, then
+ * time).  Same conventions as hd_get_last_candidates. */
+hd_size hd_get_last_giants(hd_pipeline pl, hd_size cap, hd_float* peaks, hd_size* inds,
+                           hd_size* begins, hd_size* ends, hd_size* filter_inds,
+                           hd_size* dm_inds);
+
+/* The .cand text the last call wrote (or would have written), NUL-terminated;
+ * returns the length needed. */
+hd_size hd_get_last_cand_text(hd_pipeline pl, char* buf, hd_size cap);
+
+/* hd_execute on a filterbank already in device memory (same layout as the
+ * host buffer of hd_execute).  The stream is the pipeline's own. */
+hd_error hd_execute_device(hd_pipeline pl, const hd_byte* d_filterbank, hd_size nsamps,
+                           hd_size nbits, hd_size first_idx, hd_size nbeams,
+                           hd_size* nsamps_processed);
+
+/* Number of DM trials and the maximum delay (samples) of the plan. */
+hd_size hd_get_dm_count(hd_pipeline pl);
+hd_size hd_get_max_delay(hd_pipeline pl);
+/* Copies the DM trial list (at most cap values); returns the count. */
+hd_size hd_get_dm_list(hd_pipeline pl, hd_float* out, hd_size cap);
+
+/* Stage timings (milliseconds, HIP events) of the last call, in the order
+ * unpack, dedisperse, baseline, rms, scan, block_rms, giants, cluster,
+ * total.  Returns the number of values written (at most cap). */
+hd_size hd_get_last_timings(hd_pipeline pl, double* ms, hd_size cap);
+
+/* When enabled, hd_execute* skips the .cand file / socket output (used by
+ * benchmarks and tests that read candidates through the accessors). */
+void hd_set_output_enabled(hd_pipeline pl, int enabled);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,105 @@
+/* hd/stages.h -- the search's stage functions over device memory.
+ *
+ * The reference exposes these stages as C++ classes over Thrust vectors
+ * (Pipeline/hd/*.h).  Here each is an extern "C" function over raw device
+ * pointers, batched over `n_series` DM series laid out `series_stride`
+ * floats apart, running on the HIP stream passed as `stream` (NULL = the
+ * default stream).  Argument order and meaning follow the reference call
+ * each one replaces.  All return an hd_error.
+ */
+#pragma once
+
+#include "hd/types.h"
+#include "hd/error.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Giant list in device memory (structure of arrays, capacity entries each).
+ * `count` receives the number of giants found; when it exceeds `capacity`
+ * only the first `capacity` are stored. */
+typedef struct hd_giants {
+  hd_float* peaks;
+  hd_size*  inds;
+  hd_size*  begins;
+  hd_size*  ends;
+  hd_size   capacity;
+  hd_size   count;
+} hd_giants;
+
+/* replaces RemoveBaselinePlan::exec (Pipeline/remove_baseline.cu:21-160,
+ * hd/remove_baseline.h:19-23) */
+hd_error hd_remove_baseline(hd_float* d_data, hd_size beam_stride, hd_size beam_count,
+                            hd_size smooth_radius, hd_size nbeams,
+                            hd_size n_series, hd_size series_stride, void* stream);
+
+/* replaces GetRMSPlanMB::exec_multibeam per-beam form (Pipeline/get_rms.cu:66-100):
+ * d_rms receives nbeams values per series. */
+hd_error hd_get_rms_multibeam(const hd_float* d_data, hd_float* d_rms, hd_size beam_stride,
+                              hd_size beam_count, hd_size nbeams,
+                              hd_size n_series, hd_size series_stride, void* stream);
+
+/* replaces normalise_multibeam(d, d_rms, beam_stride, cur_nsamps, nbeams)
+ * (Pipeline/get_rms.cu:202-212) */
+hd_error hd_normalise_multibeam(hd_float* d_data, const hd_float* d_rms, hd_size beam_stride,
+                                hd_size cur_nsamps, hd_size nbeams,
+                                hd_size n_series, hd_size series_stride, void* stream);
+
+/* replaces GetRMSPlanMB::exec_multibeam pooled form (Pipeline/get_rms.cu:104-131):
+ * d_block_rms receives one value per series. */
+hd_error hd_block_rms(const hd_float* d_data, hd_float* d_block_rms, hd_size beam_stride,
+                      hd_size beam_count, hd_size nbeams,
+                      hd_size n_series, hd_size series_stride, void* stream);
+
+/* replaces normalise_multibeam(d, block_rms, count) (Pipeline/get_rms.cu:189-200),
+ * with one block_rms per series read from device memory. */
+hd_error hd_normalise_block(hd_float* d_data, const hd_float* d_block_rms, hd_size count,
+                            hd_size n_series, hd_size series_stride, void* stream);
+
+/* replaces MatchedFilterPlan<float>::prep (Pipeline/matched_filter.cu:21-32):
+ * d_scanned receives count+1 prefix sums per series (scanned_stride apart). */
+hd_error hd_matched_filter_prep(const hd_float* d_in, hd_size count, hd_float* d_scanned,
+                                hd_size n_series, hd_size series_stride,
+                                hd_size scanned_stride, void* stream);
+
+/* replaces MatchedFilterPlan<float>::exec (Pipeline/matched_filter.cu:37-67):
+ * writes ceil((scanned_count - max_width)/tscrunch) values per series. */
+hd_error hd_matched_filter_exec(const hd_float* d_scanned, hd_size scanned_count,
+                                hd_size max_width, hd_size width, hd_size tscrunch,
+                                hd_float* d_out, hd_size n_series, hd_size scanned_stride,
+                                hd_size out_stride, void* stream);
+
+/* replaces GiantFinder::exec (Pipeline/find_giants.cu:101-316) for one series;
+ * giants are written in time order from index 0 of `out`. */
+hd_error hd_find_giants(const hd_float* d_data, hd_size count, hd_size nbeams,
+                        hd_float thresh, hd_size merge_dist, hd_size beam_stride,
+                        hd_size beam_count, hd_giants* out, void* stream);
+
+/* replaces label_candidate_clusters (Pipeline/label_candidate_clusters.cu:203-394),
+ * with the reference's single-thread result (see DESIGN notes in the source). */
+hd_error hd_label_candidate_clusters(hd_size count, const struct ConstRawCandidates* d_cands,
+                                     hd_size time_tol, hd_size filter_tol, hd_size dm_tol,
+                                     hd_size nsamps_beam, hd_size* d_labels,
+                                     hd_size* label_count, void* stream);
+
+/* replaces merge_candidates (Pipeline/merge_candidates.cu:74-134) */
+hd_error hd_merge_candidates(hd_size count, const hd_size* d_labels,
+                             const struct ConstRawCandidates* d_cands,
+                             const struct RawCandidates* d_groups, void* stream);
+
+/* replaces apply_manual_killmasks (Pipeline/clean_filterbank_rfi.cu:530-545):
+ * zeroes killmask[j] for j in [start, end) of each range (end exclusive). */
+hd_error hd_apply_manual_killmasks(int* h_killmask, hd_size nchans,
+                                   unsigned int num_channel_zaps,
+                                   const hd_range_t* channel_zaps);
+
+/* replaces dedisp_generate_dm_list (called at Pipeline/pipeline.cu:178-182):
+ * returns the number of trials and writes at most `cap` of them. */
+hd_size hd_generate_dm_list(hd_float* out, hd_size cap, hd_float dm_min, hd_float dm_max,
+                            hd_float dt, hd_float pulse_width_us, hd_float f0, hd_float df,
+                            hd_size nchans, hd_float tol);
+
+#ifdef __cplusplus
+}
+#endif

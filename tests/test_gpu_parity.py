@@ -1,0 +1,120 @@
+"""End-to-end parity of hd_execute on the GPU against the CPU oracle.
+
+Same seeded filterbank into both; the bar is bit-exact: giants (peak S/N,
+sample, begin, end, filter, DM index) in the reference's order, candidate
+groups, and the .cand text.
+"""
+import numpy as np
+import pytest
+
+from tests._cases import SMALL_128, TINY, oracle_params_for, params_for
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(O, w, seed, nbeams=1, nbits=8, first_idx=0, gpu_kw=None, orc_kw=None,
+              fb=None, nsamps=None):
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import synth
+    gpu_kw = gpu_kw or {}
+    orc_kw = orc_kw or {}
+    if fb is None:
+        beams = [synth.make_gulp(w, seed + b) for b in range(nbeams)]
+        fb = np.concatenate(beams)
+        nsamps = w.nsamps * nbeams
+    print("case", w.name, "seed", seed, "nbeams", nbeams, "nbits", nbits, flush=True)
+    pl = H.Pipeline(params_for(w, **gpu_kw))
+    pl.set_output(False)
+    allow = (H.HD_NO_ERROR, H.HD_TOO_MANY_EVENTS)
+    err, processed = pl.execute(fb, nsamps, nbits, first_idx, nbeams, allow=allow)
+    g = pl.giants()
+    c = pl.candidates()
+    text = pl.cand_text()
+    dms = pl.dm_list()
+    md = pl.max_delay()
+    pl.close()
+    p = oracle_params_for(O, w, **orc_kw)
+    r = O.execute(p, fb, nsamps, nbits, first_idx, nbeams)
+    return dict(err=err, processed=processed, giants=g, cands=c, text=text, dms=dms,
+                max_delay=md), r
+
+
+def _compare(gpu, r, O, w):
+    assert gpu["max_delay"] == r.max_delay
+    assert len(gpu["dms"]) == r.n_dm
+    assert gpu["processed"] == r.nsamps_processed
+    og = r.giants
+    gg = gpu["giants"]
+    print("giants gpu", len(gg["peak"]), "oracle", len(og), "groups", len(r.groups), flush=True)
+    assert len(gg["peak"]) == len(og)
+    for f in ("ind", "begin", "end", "filter", "dm"):
+        np.testing.assert_array_equal(gg[f], og[f].astype(np.uint64), err_msg=f)
+    np.testing.assert_array_equal(gg["peak"].view(np.uint32), og["peak"].view(np.uint32))
+    gc = gpu["cands"]
+    assert len(gc["peak"]) == len(r.groups)
+    for f in ("ind", "begin", "end", "filter", "dm", "members"):
+        np.testing.assert_array_equal(gc[f], r.groups[f].astype(np.uint64), err_msg=f)
+    np.testing.assert_array_equal(gc["peak"].view(np.uint32), r.groups["peak"].view(np.uint32))
+    np.testing.assert_array_equal(gc["dms"], r.group_dms)
+    assert gpu["text"] == r.text
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_tiny_single_beam(oracle, seed):
+    gpu, r = _run_both(oracle, TINY, seed)
+    assert len(r.giants) > 0
+    _compare(gpu, r, oracle, TINY)
+
+
+def test_small_with_rfi(oracle):
+    gpu, r = _run_both(oracle, SMALL_128, 3)
+    assert len(r.groups) > 0
+    _compare(gpu, r, oracle, SMALL_128)
+
+
+@pytest.mark.parametrize("nbeams", [2, 3])
+def test_multibeam_stf(oracle, nbeams):
+    gpu, r = _run_both(oracle, TINY, 10, nbeams=nbeams)
+    _compare(gpu, r, oracle, TINY)
+
+
+def test_killmask_zaps(oracle):
+    zaps = [(3, 9), (40, 41)]
+    km = np.ones(TINY.nchans, np.int32)
+    for a, b in zaps:
+        km[a:b] = 0
+    gpu, r = _run_both(oracle, TINY, 5, gpu_kw=dict(channel_zaps=zaps), orc_kw=dict(killmask=km))
+    _compare(gpu, r, oracle, TINY)
+
+
+@pytest.mark.parametrize("nbits", [2, 4])
+def test_subbyte_input(oracle, nbits):
+    import dataclasses
+    from heimdall_multibeam_amd import synth
+    mean, sigma = (1.5, 0.9) if nbits == 2 else (7.5, 2.5)
+    w = dataclasses.replace(TINY, nbits=nbits, mean=mean, sigma=sigma, name="tiny_%dbit" % nbits)
+    gpu, r = _run_both(oracle, w, 7, nbits=nbits)
+    _compare(gpu, r, oracle, w)
+
+
+def test_decimated_widths(oracle):
+    # min_tscrunch_width below boxcar_max: the widest widths are decimated and
+    # their block RMS reads the previous width's normalised output
+    gpu, r = _run_both(oracle, TINY, 11, gpu_kw=dict(min_tscrunch_width=64),
+                       orc_kw=dict(min_tscrunch_width=64))
+    _compare(gpu, r, oracle, TINY)
+
+
+def test_giant_rate_limit(oracle):
+    import heimdall_multibeam_amd as H
+    gpu, r = _run_both(oracle, TINY, 1, gpu_kw=dict(max_giant_rate=1.0),
+                       orc_kw=dict(max_giant_rate=1.0))
+    assert r.error == H.HD_TOO_MANY_EVENTS
+    assert gpu["err"] == H.HD_TOO_MANY_EVENTS
+    assert gpu["text"] == r.text == ""
+
+
+def test_first_idx_and_beam_offset(oracle):
+    gpu, r = _run_both(oracle, TINY, 4, first_idx=123456, gpu_kw=dict(beam=2),
+                       orc_kw=dict(beam=2))
+    _compare(gpu, r, oracle, TINY)

@@ -1,0 +1,310 @@
+"""Pins of the CPU oracle against the reference fixture and against what the
+mathematics fixes (CPU only).
+
+Each check is chosen so that a plausible slip in the oracle (a dropped term,
+an off-by-one window, a wrong rounding, a transposed operand) fails it.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+f32 = np.float32
+
+
+# ---------------------------------------------------------------- DM list
+def test_dm_list_matches_reference_fixture(oracle):
+    """Share/dm2000.list: trials 1..1434 for the reference defaults with -dm 0 2000
+    (nchans 1024, dt 64 us, f0 1581.804688, df -0.390625, width 40 us, tol 1.25)."""
+    dms = oracle.dm_list(f32(0), f32(2000), f32(64e-6), f32(40), f32(1581.804688),
+                         f32(-0.390625), 1024, f32(1.25))
+    ref = open(os.path.join(GOLDEN, "dm2000.list")).read().split()
+    assert len(ref) == 1434
+    assert dms[0] == 0.0
+    assert len(dms) == 1435  # trial 0 plus the 1434 listed
+    got = ["%g" % v for v in dms[1:]]
+    mism = [(i, a, b) for i, (a, b) in enumerate(zip(got, ref)) if a != b]
+    assert not mism, mism[:5]
+
+
+def test_dm_list_monotone_and_spacing(oracle):
+    dms = oracle.dm_list(f32(0), f32(3000), f32(64e-6), f32(40), f32(1599.95), f32(-0.0976),
+                         4096, f32(1.25))
+    assert np.all(np.diff(dms) > 0)
+    assert dms[-2] < 3000 <= dms[-1]
+
+
+# ---------------------------------------------------------------- medians
+def test_medians_are_order_statistics(oracle):
+    for vals in itertools.product(range(4), repeat=5):
+        v = [f32(x) for x in vals]
+        assert oracle.median5(*v) == np.sort(v)[2]
+    for vals in itertools.product(range(4), repeat=4):
+        v = [f32(x) * f32(0.5) + f32(i) * f32(0.125) for i, x in enumerate(vals)]
+        s = np.sort(np.array(v, np.float32))
+        assert oracle.median4(*v) == f32(f32(0.5) * f32(s[1] + s[2]))
+    for vals in itertools.product(range(3), repeat=3):
+        v = [f32(x) for x in vals]
+        assert oracle.median3(*v) == np.sort(v)[1]
+
+
+# ---------------------------------------------------------------- linear stretch
+def test_linear_stretch_identity_and_affine(oracle):
+    x = np.random.default_rng(0).standard_normal(1000).astype(np.float32)
+    np.testing.assert_array_equal(oracle.linear_stretch(x, 1000), x)
+    a = (np.arange(17, dtype=np.float64) * 3.0 + 5.0).astype(np.float32)
+    out = oracle.linear_stretch(a, 53125)
+    xs = np.arange(53125) * (16.0 / 53124.0)
+    np.testing.assert_allclose(out, 5.0 + 3.0 * xs, rtol=0, atol=5e-4)
+    assert out[0] == a[0]
+    assert abs(out[-1] - a[-1]) < 1e-3
+
+
+# ---------------------------------------------------------------- dedispersion
+def _brute_dedisperse(fb, nsamps, nbits, nchans, dms, kc, max_delay, oracle, killmask=None):
+    per = 8 // nbits if nbits < 8 else 1
+    stride = nchans * nbits // 8
+    rows = fb.reshape(nsamps, stride)
+    x = np.zeros((nchans, nsamps), np.int64)
+    for c in range(nchans):
+        if nbits == 8:
+            x[c] = rows[:, c]
+        else:
+            x[c] = (rows[:, c // per] >> ((c % per) * nbits)) & ((1 << nbits) - 1)
+    nout = nsamps - max_delay
+    out = np.zeros((len(dms), nout), np.int64)
+    for d, dm in enumerate(dms):
+        for c in range(nchans):
+            if killmask is not None and not killmask[c]:
+                continue
+            off = oracle.delay(f32(dm), f32(kc[c]))
+            out[d] += x[c, off:off + nout]
+    return out.astype(np.float32)
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4, 8])
+def test_dedisperse_brute_force(oracle, nbits):
+    rng = np.random.default_rng(nbits)
+    nchans, nsamps = 32, 700
+    dt, f0, df = f32(256e-6), f32(1500.0), f32(-4.0)
+    kc = oracle.delay_table(nchans, dt, f0, df)
+    dms = oracle.dm_list(f32(0), f32(60), dt, f32(40), f0, df, nchans, f32(1.25))
+    max_delay = oracle.delay(dms[-1], kc[-1])
+    fb = rng.integers(0, 256, nsamps * nchans * nbits // 8, dtype=np.uint8)
+    km = np.ones(nchans, np.int32)
+    km[[3, 17]] = 0
+    got = oracle.dedisperse(fb, nsamps, nbits, nchans, dms, kc, max_delay, killmask=km,
+                            nthreads=2)
+    ref = _brute_dedisperse(fb, nsamps, nbits, nchans, dms, kc, max_delay, oracle, km)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_dedisperse_impulse_dm0_and_linearity(oracle):
+    nchans, nsamps = 16, 400
+    dt, f0, df = f32(256e-6), f32(1400.0), f32(-8.0)
+    kc = oracle.delay_table(nchans, dt, f0, df)
+    dms = oracle.dm_list(f32(0), f32(40), dt, f32(40), f0, df, nchans, f32(1.25))
+    md = oracle.delay(dms[-1], kc[-1])
+    # impulse in channel 9 at t=300 appears at t = 300 - delay(dm, 9)
+    fb = np.zeros((nsamps, nchans), np.uint8)
+    fb[300, 9] = 7
+    out = oracle.dedisperse(fb.ravel(), nsamps, 8, nchans, dms, kc, md)
+    for d, dm in enumerate(dms):
+        t = 300 - oracle.delay(dm, kc[9])
+        expect = np.zeros(nsamps - md, np.float32)
+        if 0 <= t < nsamps - md:
+            expect[t] = 7
+        np.testing.assert_array_equal(out[d], expect)
+    # trial 0 is the plain channel sum
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, 120, (nsamps, nchans), dtype=np.uint8)
+    b = rng.integers(0, 120, (nsamps, nchans), dtype=np.uint8)
+    oa = oracle.dedisperse(a.ravel(), nsamps, 8, nchans, dms, kc, md)
+    np.testing.assert_array_equal(oa[0], a[:nsamps - md].sum(axis=1).astype(np.float32))
+    ob = oracle.dedisperse(b.ravel(), nsamps, 8, nchans, dms, kc, md)
+    oab = oracle.dedisperse((a + b).ravel(), nsamps, 8, nchans, dms, kc, md)
+    np.testing.assert_array_equal(oab, oa + ob)
+
+
+# ---------------------------------------------------------------- scan / boxcar
+def test_scan_and_boxcar_exact_on_integers(oracle):
+    """Small integers keep every float sum exact, so the prefix sums and boxcar
+    windows must equal integer window sums exactly (pins offsets and strides)."""
+    rng = np.random.default_rng(2)
+    x = rng.integers(-50, 50, 5000).astype(np.float32)
+    S = oracle.scan(x)
+    np.testing.assert_array_equal(S, np.concatenate([[0], np.cumsum(x)]).astype(np.float32))
+    W = 256
+    for w in (1, 2, 4, 8, 64, 256):
+        for ts in (1, 2, 4):
+            out = oracle.boxcar(S, W, w, ts)
+            n_out = len(S) - W
+            assert len(out) == (n_out + ts - 1) // ts
+            off, ahead, behind = W // 2, (w - 1) // 2 + 1, w // 2
+            for k in range(0, len(out), 97):
+                c = off + k * ts
+                assert out[k] == x[c - behind:c + ahead].sum()
+
+
+def test_scan_sequential_rounding(oracle):
+    """Float32 prefix sums in sequential order stay within the classic
+    accumulated-rounding bound of the exact sums."""
+    x = np.random.default_rng(3).standard_normal(200000).astype(np.float32)
+    S = oracle.scan(x).astype(np.float64)
+    exact = np.concatenate([[0.0], np.cumsum(x.astype(np.float64))])
+    bound = np.arange(len(S)) * np.finfo(np.float32).eps * np.concatenate(
+        [[0.0], np.cumsum(np.abs(x.astype(np.float64)))])
+    assert np.all(np.abs(S - exact) <= bound + 1e-6)
+
+
+# ---------------------------------------------------------------- baseline / rms
+def test_baseline_removes_constant_and_ramp(oracle):
+    n, delay, r = 60000, 1000, 3906
+    N = n - delay
+    const = np.full(N, 137.0, np.float32)
+    out = oracle.remove_baseline(const, n, N, r, 1)
+    np.testing.assert_array_equal(out, np.zeros_like(out))
+    # A ramp b*i: each pyramid point is the value at the centre of its block of
+    # 5^n resampled points, but the stretch back places it at the block start,
+    # so the residual is the line b*(5^n-1)/(CR-1)*(i-(N-1)/2), zero mid-series.
+    b = 0.01
+    ramp = (np.arange(N) * b + 50).astype(np.float32)
+    out = oracle.remove_baseline(ramp, n, N, r, 1)
+    sc = int(np.float64(f32(2) * f32(n) / f32(2 * r)) + 0.5)
+    nscr = int(np.log(n // sc) / np.log(5.0))
+    cr = 5 ** nscr * sc
+    i = np.arange(N)
+    expect = b * (5 ** nscr - 1) / (cr - 1) * (i - (N - 1) / 2.0)
+    assert np.max(np.abs(out - expect)) < 0.02
+    assert abs(out[0] - expect[0]) < 0.02 and expect[0] < -19
+
+
+def test_rms_of_gaussian_and_constant(oracle):
+    rng = np.random.default_rng(4)
+    sigma = 3.0
+    x = (rng.standard_normal(3 * 78125 + 17) * sigma).astype(np.float32)
+    rms = oracle.get_rms_multibeam(x, len(x), len(x), 1)
+    assert abs(rms[0] / sigma - 1.0) < 0.05
+    c = np.full(4000, -2.5, np.float32)
+    assert oracle.get_rms_multibeam(c, 4000, 4000, 1)[0] == f32(2.5) * f32(1.4862)
+    assert oracle.block_rms(c, 4000, 4000, 1) == f32(np.float64(f32(2.5)) * 1.4862)
+
+
+# ---------------------------------------------------------------- giants
+def _brute_giants(v, thresh, merge_dist, stride, count):
+    idx = [i for i in range(len(v)) if (i % stride) < count and v[i] > thresh]
+    runs, cur = [], []
+    for i in idx:
+        if cur and i - cur[-1] > merge_dist:
+            runs.append(cur)
+            cur = []
+        cur.append(i)
+    if cur:
+        runs.append(cur)
+    out = []
+    for r in runs:
+        vals = [v[i] for i in r]
+        k = int(np.argmax(vals))  # first maximum
+        out.append((vals[k], r[k], r[0], r[-1] + 1))
+    return out
+
+
+@pytest.mark.parametrize("merge_dist,stride,count", [(3, 10**9, 10**9), (6, 997, 900), (0, 500, 500)])
+def test_find_giants_brute_force(oracle, merge_dist, stride, count):
+    rng = np.random.default_rng(merge_dist)
+    v = rng.standard_normal(20000).astype(np.float32) * 2
+    v[5000:5010] = 9.0  # plateau: ties keep the first index
+    g = oracle.find_giants(v, 5.0, merge_dist, stride, count)
+    ref = _brute_giants(v, 5.0, merge_dist, stride, count)
+    assert len(g) == len(ref)
+    for a, b in zip(g, ref):
+        assert (a["peak"], a["ind"], a["begin"], a["end"]) == b
+
+
+# ---------------------------------------------------------------- clustering / merge
+def _random_giants(rng, n, nsamps_beam, nbeams):
+    from oracle.oracle import CAND_DTYPE
+    c = np.zeros(n, CAND_DTYPE)
+    c["peak"] = rng.uniform(6, 20, n).astype(np.float32)
+    c["peak"][::7] = 10.0  # ties
+    c["ind"] = rng.integers(0, nsamps_beam * nbeams, n)
+    c["filter"] = rng.integers(0, 6, n)
+    c["dm"] = rng.integers(0, 400, n)
+    c["begin"] = c["ind"]
+    c["end"] = c["ind"] + 1
+    c["members"] = 1
+    return c
+
+
+def _coinc(a, b, tt, ft, dt, nb):
+    if a["ind"] // nb != b["ind"] // nb:
+        return False
+    tol = tt * max(1 << int(a["filter"]), 1 << int(b["filter"]))
+    return (abs(int(a["ind"]) - int(b["ind"])) <= tol and
+            abs(int(a["filter"]) - int(b["filter"])) <= ft and
+            abs(int(a["dm"]) - int(b["dm"])) <= dt)
+
+
+def test_label_clusters_is_min_reachable_earlier_index(oracle):
+    """Sequential relabelling = smallest index reachable through coincident
+    links that always step to an earlier giant (DAG reachability)."""
+    rng = np.random.default_rng(5)
+    for trial in range(4):
+        c = _random_giants(rng, 300, 5000, 2)
+        labels, count = oracle.label_clusters(c, 3, 3, 40, 5000)
+        n = len(c)
+        for i in range(n):
+            seen, stack = {i}, [i]
+            while stack:
+                k = stack.pop()
+                for j in range(k):
+                    if j not in seen and _coinc(c[k], c[j], 3, 3, 40, 5000):
+                        seen.add(j)
+                        stack.append(j)
+            assert labels[i] == min(seen)
+        assert count == int(np.sum(labels == np.arange(n)))
+
+
+def test_merge_candidates_brute_force(oracle):
+    rng = np.random.default_rng(6)
+    c = _random_giants(rng, 400, 8000, 1)
+    labels, count = oracle.label_clusters(c, 3, 3, 50, 8000)
+    groups = oracle.merge_candidates(labels, c, count)
+    roots = sorted(set(labels.tolist()))
+    assert len(groups) == len(roots)
+    for g, r in zip(groups, roots):
+        mem = [i for i in range(len(c)) if labels[i] == r]
+        best = mem[int(np.argmax([c[i]["peak"] for i in mem]))]  # first maximum
+        assert g["members"] == len(mem)
+        for f in ("peak", "ind", "begin", "end", "filter", "dm"):
+            assert g[f] == c[best][f]
+
+
+# ---------------------------------------------------------------- whole gulp
+def test_execute_recovers_injected_pulse(oracle):
+    """A single bright pulse at a trial DM comes back at that DM and sample."""
+    from heimdall_multibeam_amd import synth
+    from tests._cases import TINY, oracle_params_for
+    import dataclasses
+    w = dataclasses.replace(TINY, n_pulses=0)
+    dms = synth.trial_dms(w)
+    fb, truth = synth.make_gulp(w, 21, n_pulses=0, return_truth=True)
+    # inject one pulse by regenerating with an explicit pulse list
+    import ctypes as C
+    t0, dm_idx = 6000.0, 40
+    spec_pulses = [(t0, float(dms[dm_idx]), 4.0, 40.0)]
+    out = np.empty_like(fb)
+    S = synth._Spec()
+    S.nsamps, S.nchans, S.nbits = w.nsamps, w.nchans, 8
+    S.dt, S.f0, S.df, S.seed, S.mean, S.sigma = w.dt, w.f0, w.df, 21, 128.0, 16.0
+    P = (synth._Pulse * 1)(synth._Pulse(*spec_pulses[0]))
+    S.pulses, S.npulses = P, 1
+    assert synth._synth().hd_synth_filterbank(C.byref(S), out.ctypes.data) == 0
+    r = oracle.execute(oracle_params_for(oracle, w), out, w.nsamps, 8, 0, 1)
+    assert r.error == 0
+    best = r.groups[np.argmax(r.groups["peak"])]
+    assert abs(int(best["dm"]) - dm_idx) <= 1
+    assert abs(int(best["ind"]) - int(t0)) <= 4
+    assert best["peak"] > 25
