@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Per-gulp search throughput on one MI355X.
+
+One step = one hd_execute_device call (the whole per-gulp search: unpack,
+dedispersion over the DM trials, baseline, RMS, normalisation, prefix sums,
+boxcars 1..boxcar_max with block RMS, giant finding, clustering, merging and
+the candidate read-back) over one synthetic gulp already resident in HBM.
+
+Prints one JSON line with beam-seconds searched per second, the roofline of
+the dominant kernel (dedispersion) from HIP events on the pipeline's stream,
+and the CPU oracle timed on a bounded sample of the same gulp.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+VALU_INT_PEAK = 256 * 64 * 2.4e9  # int32 lane adds/s (256 CUs x 64 lanes x 2.4 GHz)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="full_1024ch")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-stride", type=int, default=8,
+                    help="CPU baseline: time every k-th DM trial of the same gulp")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def load_traffic(name):
+    """HBM bytes per dedispersion launch measured with rocprofv3 --pmc
+    (profiles/<kernel>_<workload>_traffic.txt: 'fetch_bytes= write_bytes=')."""
+    path = os.path.join(HERE, "profiles", "dedisperse_%s_traffic.txt" % name)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        for tok in line.split():
+            if "=" in tok:
+                k, v = tok.split("=", 1)
+                try:
+                    vals[k] = float(v)
+                except ValueError:
+                    pass
+    if "fetch_bytes" in vals and "write_bytes" in vals:
+        return vals["fetch_bytes"] + vals["write_bytes"]
+    return None
+
+
+def cpu_baseline(w, fb, stride, n_dm, beam_s):
+    from oracle import oracle as O
+    nthreads = min(16, os.cpu_count() or 1)
+    p = O.make_params(nchans=w.nchans, dt=w.dt, f0=w.f0, df=w.df, dm_min=w.dm_min,
+                      dm_max=w.dm_max, boxcar_max=w.boxcar_max, detect_thresh=w.detect_thresh,
+                      nthreads=nthreads, dm_stride=stride)
+    t0 = time.perf_counter()
+    r = O.execute(p, fb, w.nsamps, w.nbits, 0, 1, want_text=False)
+    dt = time.perf_counter() - t0
+    n_sub = len(range(0, n_dm, stride))
+    full_s = dt * n_dm / n_sub
+    return {"value": beam_s / full_s, "unit": "beam-seconds searched/s", "cores": nthreads,
+            "kind": "port",
+            "sample": "CPU oracle (C, OpenMP over trials) on the same gulp, every %d-th of %d "
+                      "DM trials (%d trials, %.1f s wall), scaled to all trials"
+                      % (stride, n_dm, n_sub, dt)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(beams or DM ranges would shard one process per GPU); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import hip, synth
+    w = synth.CONFIGS[a.workload]
+    print("generating %s gulp (seed %d)" % (w.name, a.seed), file=sys.stderr, flush=True)
+    fb = synth.make_gulp(w, a.seed)
+    params = H.default_params(nchans=w.nchans, dt=w.dt, f0=w.f0, df=w.df, dm_min=w.dm_min,
+                              dm_max=w.dm_max, boxcar_max=w.boxcar_max,
+                              detect_thresh=w.detect_thresh, use_scrunching=False)
+    pl = H.Pipeline(params)
+    pl.set_output(False)
+    n_dm = len(pl.dm_list())
+    max_delay = pl.max_delay()
+    nout = w.nsamps - max_delay
+    beam_s = (w.nsamps - w.boxcar_max - max_delay) * w.dt  # pipeline.cu:370-372
+    dbuf = hip.DeviceBuffer(fb.nbytes)
+    dbuf.upload(fb)
+    pl.preallocate(w.nsamps, 1)
+    print("n_dm=%d max_delay=%d beam_s/gulp=%.3f" % (n_dm, max_delay, beam_s), file=sys.stderr,
+          flush=True)
+    for i in range(a.warmup):
+        pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
+        print("warmup %d done" % i, file=sys.stderr, flush=True)
+    hip.synchronize()
+    stage = {k: [] for k in H.Pipeline.STAGES}
+    n_cand = 0
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
+        for k, v in pl.timings().items():
+            stage[k].append(v)
+        n_cand = len(pl.candidates()["peak"])
+    hip.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / a.steps * 1e3
+    value = beam_s * a.steps / elapsed
+    print("steps done: %.2f ms/step, %d candidates" % (ms_per_step, n_cand), file=sys.stderr,
+          flush=True)
+
+    # dominant kernel: dedispersion (HIP events around its launch)
+    t_dd = float(np.mean(stage["dedisperse"])) * 1e-3
+    b_in = w.nsamps * w.nchans * w.nbits // 8
+    plane = 4 * n_dm * nout
+    alg = b_in + plane
+    achieved = alg / t_dd / 1e9
+    adds = float(n_dm) * nout * w.nchans
+    traffic = load_traffic(w.name)
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "kernel": "k_dedisperse", "kernel_ms": round(t_dd * 1e3, 4),
+            "algorithmic_bytes": alg,
+            "int_adds_per_s": adds / t_dd, "valu_int32_frac": round(adds / t_dd / VALU_INT_PEAK, 4)}
+    stages_ms = {k: round(float(np.mean(v)), 4) for k, v in stage.items() if v}
+    out = {
+        "metric": "beam-seconds searched/s/GPU at DM 0-2000",
+        "value": round(value, 3),
+        "unit": "beam-seconds searched/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 samples, exact u32 sums, f32 search",
+        "data": "synthetic (seeded Gaussian noise N(128,16^2) + 50 dispersed pulses + "
+                "5%% tone channels + 3 zero-DM bursts)",
+        "config": {"workload": "%s: 1 beam, %d ch x %d-bit, fch1 %.4f MHz, foff %.6f MHz, "
+                               "tsamp %g us, gulp %d, DM %g-%g (%d trials, max delay %d), "
+                               "boxcars 1..%d, threshold %g sigma"
+                               % (w.name, w.nchans, w.nbits, w.f0, w.df, w.dt * 1e6, w.nsamps,
+                                  w.dm_min, w.dm_max, n_dm, max_delay, w.boxcar_max,
+                                  w.detect_thresh),
+                   "beam_seconds_per_gulp": round(beam_s, 4), "candidates": n_cand,
+                   "parallelism": "1 GPU"},
+        "stage_ms": stages_ms,
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        print("timing CPU baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, fb, a.cpu_stride, n_dm, beam_s)
+    pl.close()
+    dbuf.free()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -573,6 +573,103 @@ static void push_giants(orc_result* r, size_t* cap, const orc_cand* g, size_t n)
   r->n_giants += n;
 }
 
+/* Per-thread copies of the reference's persistent buffers: time series,
+ * prefix sums, filtered series (pipeline.cu:76-77, zero-filled by resize) and
+ * the baseline plan's scrunch buffers. */
+struct dm_state {
+  float* ts;
+  float* scanned;
+  float* filt;
+  float* beam_rms;
+  struct orc_work* work;
+  orc_cand* tmp;
+  size_t tmpcap;
+};
+static struct dm_state* dm_state_create(size_t series_stride, size_t nbeams) {
+  struct dm_state* s = (struct dm_state*)calloc(1, sizeof(struct dm_state));
+  s->ts = (float*)calloc(series_stride + 64, sizeof(float));
+  s->scanned = (float*)calloc(series_stride + 64, sizeof(float));
+  s->filt = (float*)calloc(series_stride + 64, sizeof(float));
+  s->beam_rms = (float*)calloc(nbeams, sizeof(float));
+  s->work = orc_work_create();
+  s->tmpcap = 1 << 12;
+  s->tmp = (orc_cand*)malloc(sizeof(orc_cand) * s->tmpcap);
+  return s;
+}
+static void dm_state_destroy(struct dm_state* s) {
+  free(s->ts); free(s->scanned); free(s->filt); free(s->beam_rms); free(s->tmp);
+  orc_work_destroy(s->work);
+  free(s);
+}
+struct dm_ctx {
+  const orc_params* p;
+  size_t nsamps, nbeams, nsamps_computed, max_delay, W;
+};
+
+/* One DM trial of the loop at pipeline.cu:488-790 (scrunch factor 1).
+ * Appends its giants to r; returns 1 when the giant-rate limit trips. */
+static int process_dm(const struct dm_ctx* c, struct dm_state* st, size_t dm_idx,
+                      const float* row, orc_result* r, size_t* gcap) {
+  const orc_params* p = c->p;
+  const size_t W = c->W, nbeams = c->nbeams, nsamps = c->nsamps;
+  size_t cur_dm_scrunch = 1;
+  size_t cur_nsamps = c->nsamps_computed / cur_dm_scrunch;  /* :490 */
+  float cur_dt = p->dt * (float)cur_dm_scrunch;             /* :491 */
+  size_t cur_dm_delay = c->max_delay / cur_dm_scrunch;      /* :493-497 */
+  size_t cur_beam_stride = nsamps / nbeams / cur_dm_scrunch; /* :498 */
+  size_t cur_beam_nsamps = cur_beam_stride - cur_dm_delay;  /* :499 */
+  memcpy(st->ts, row, sizeof(float) * cur_nsamps);
+  size_t nsamps_smooth = (size_t)(p->baseline_length / (2 * cur_dt)); /* :572-573 */
+  orc_remove_baseline(st->work, st->ts, cur_beam_stride, cur_beam_nsamps, nsamps_smooth, nbeams);
+  orc_get_rms_multibeam(st->ts, st->beam_rms, cur_beam_stride, cur_beam_nsamps, nbeams);
+  orc_normalise_multibeam(st->ts, st->beam_rms, cur_beam_stride, cur_nsamps);
+  size_t rel_boxcar_max = W / cur_dm_scrunch;               /* :626 */
+  size_t max_nsamps_filtered = cur_nsamps + 1 - rel_boxcar_max;
+  size_t cur_filtered_offset = rel_boxcar_max / 2;
+  orc_scan(st->ts, cur_nsamps, st->scanned);                /* :643 */
+  for (size_t fw = cur_dm_scrunch; fw <= W; fw *= 2) {      /* :657-659 */
+    size_t rel_filter_width = fw / cur_dm_scrunch;
+    size_t filter_idx = filter_index((unsigned)fw);
+    size_t rel_min_tscrunch_width = p->min_tscrunch_width / cur_dm_scrunch;
+    if (rel_min_tscrunch_width < 1) rel_min_tscrunch_width = 1;
+    size_t rel_tscrunch_width = 2 * rel_filter_width / rel_min_tscrunch_width;
+    if (rel_tscrunch_width < 1) rel_tscrunch_width = 1;
+    size_t rel_rel_filter_width = rel_filter_width / rel_tscrunch_width;
+    orc_boxcar(st->scanned, cur_nsamps + 1, rel_boxcar_max, rel_filter_width,
+               rel_tscrunch_width, st->filt);               /* :680-682 */
+    size_t cur_nsamps_filtered = ((max_nsamps_filtered - 1) / rel_tscrunch_width + 1);
+    size_t cur_scrunch = cur_dm_scrunch * rel_tscrunch_width;
+    size_t bc = cur_beam_stride - (cur_dm_delay + rel_boxcar_max); /* :692 */
+    float block_rms = orc_block_rms(st->filt, cur_beam_stride, bc, nbeams); /* :709 */
+    orc_normalise_block(st->filt, block_rms, cur_nsamps_filtered);         /* :712 */
+    size_t ng;
+    for (;;) {
+      ng = orc_find_giants(st->filt, cur_nsamps_filtered, p->detect_thresh,
+                           p->cand_sep_time * rel_rel_filter_width,
+                           cur_beam_stride, bc, st->tmp, st->tmpcap);
+      if (ng <= st->tmpcap) break;
+      st->tmpcap = ng;
+      st->tmp = (orc_cand*)realloc(st->tmp, sizeof(orc_cand) * st->tmpcap);
+    }
+    size_t rel_off = cur_filtered_offset / rel_tscrunch_width; /* :755-756 */
+    for (size_t g = 0; g < ng; ++g) {                        /* :759-775 */
+      st->tmp[g].ind = (st->tmp[g].ind + rel_off) * cur_scrunch;
+      st->tmp[g].begin = (st->tmp[g].begin + rel_off) * cur_scrunch;
+      st->tmp[g].end = (st->tmp[g].end + rel_off) * cur_scrunch;
+      st->tmp[g].filter = filter_idx;
+      st->tmp[g].dm = dm_idx;
+      st->tmp[g].members = 1;
+    }
+    push_giants(r, gcap, st->tmp, ng);
+    /* :780-787 */
+    float data_length_mins = (float)((double)((float)nsamps * p->dt) / 60.0);
+    if (p->max_giant_rate != 0 &&
+        ((float)r->n_giants / data_length_mins > p->max_giant_rate))
+      return 1;
+  }
+  return 0;
+}
+
 /* hd_execute (pipeline.cu:233-1087) without the file/TCP output, with the
  * scrunch factor fixed at 1 (adaptive dt off). */
 int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
@@ -607,88 +704,63 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
   size_t dm_first = p->dm_first;
   size_t dm_end = p->dm_count ? dm_first + p->dm_count : n_dm;
   if (dm_end > n_dm) dm_end = n_dm;
-  size_t n_sub = dm_end > dm_first ? dm_end - dm_first : 0;
+  size_t step = p->dm_stride ? p->dm_stride : 1;
+  size_t n_sub = 0;
+  for (size_t d = dm_first; d < dm_end; d += step) ++n_sub;
+  size_t* sub_idx = (size_t*)malloc(sizeof(size_t) * (n_sub ? n_sub : 1));
+  float* sub_dm = (float*)malloc(sizeof(float) * (n_sub ? n_sub : 1));
+  n_sub = 0;
+  for (size_t d = dm_first; d < dm_end; d += step) { sub_idx[n_sub] = d; sub_dm[n_sub++] = dm_list[d]; }
   float* plane = (float*)malloc(sizeof(float) * (n_sub ? n_sub : 1) * series_stride);
-  if (orc_dedisperse(fb, nsamps, nbits, nchans, p->killmask, dm_list + dm_first,
-                     n_sub, kc, max_delay, plane, series_stride, p->nthreads) != 0) {
-    r->error = 8; free(plane); free(dm_list); free(kc); return r->error;
+  if (orc_dedisperse(fb, nsamps, nbits, nchans, p->killmask, sub_dm, n_sub, kc, max_delay,
+                     plane, series_stride, p->nthreads) != 0) {
+    r->error = 8; free(plane); free(dm_list); free(kc); free(sub_idx); free(sub_dm);
+    return r->error;
   }
 
-  /* persistent device buffers of the reference: time series, filtered
-   * series (pipeline.cu:76-77, zero-filled by resize), baseline plan state */
-  float* ts = (float*)calloc(series_stride + 64, sizeof(float));
-  float* scanned = (float*)calloc(series_stride + 64, sizeof(float));
-  float* filt = (float*)calloc(series_stride + 64, sizeof(float));
-  float* beam_rms = (float*)calloc(nbeams, sizeof(float));
-  struct orc_work* work = orc_work_create();
-  size_t gcap = 0;
-  size_t tmpcap = 1 << 16;
-  orc_cand* tmp = (orc_cand*)malloc(sizeof(orc_cand) * tmpcap);
+  /* The DM loop breaks before its first trial when the delay leaves no room
+   * (pipeline.cu:501-506, 633-638); both tests are the same for every trial. */
+  size_t cur_beam_stride = nsamps / nbeams;                 /* :498 */
+  int skip = (max_delay >= cur_beam_stride) || (max_delay + W > cur_beam_stride);
   int too_many = 0;
-
-  for (size_t dm_idx = dm_first; dm_idx < dm_end; ++dm_idx) {
-    size_t cur_dm_scrunch = 1;
-    size_t cur_nsamps = nsamps_computed / cur_dm_scrunch;   /* :490 */
-    float cur_dt = p->dt * (float)cur_dm_scrunch;           /* :491 */
-    size_t cur_dm_delay = max_delay / cur_dm_scrunch;       /* :493-497 */
-    size_t cur_beam_stride = nsamps / nbeams / cur_dm_scrunch; /* :498 */
-    size_t cur_beam_nsamps = cur_beam_stride - cur_dm_delay;   /* :499 */
-    if (cur_dm_delay >= cur_beam_stride) break;             /* :501-506 */
-    if (too_many) break;                                    /* :509-511 */
-    memcpy(ts, plane + (dm_idx - dm_first) * series_stride, sizeof(float) * cur_nsamps);
-    /* :572-573 */
-    size_t nsamps_smooth = (size_t)(p->baseline_length / (2 * cur_dt));
-    orc_remove_baseline(work, ts, cur_beam_stride, cur_beam_nsamps, nsamps_smooth, nbeams);
-    orc_get_rms_multibeam(ts, beam_rms, cur_beam_stride, cur_beam_nsamps, nbeams);
-    orc_normalise_multibeam(ts, beam_rms, cur_beam_stride, cur_nsamps);
-    size_t rel_boxcar_max = W / cur_dm_scrunch;             /* :626 */
-    size_t max_nsamps_filtered = cur_nsamps + 1 - rel_boxcar_max;
-    size_t cur_filtered_offset = rel_boxcar_max / 2;
-    if (cur_dm_delay + rel_boxcar_max > cur_beam_stride) break; /* :633-638 */
-    orc_scan(ts, cur_nsamps, scanned);                      /* :643 */
-    for (size_t fw = cur_dm_scrunch; fw <= W; fw *= 2) {    /* :657-659 */
-      size_t rel_filter_width = fw / cur_dm_scrunch;
-      size_t filter_idx = filter_index((unsigned)fw);
-      size_t rel_min_tscrunch_width = p->min_tscrunch_width / cur_dm_scrunch;
-      if (rel_min_tscrunch_width < 1) rel_min_tscrunch_width = 1;
-      size_t rel_tscrunch_width = 2 * rel_filter_width / rel_min_tscrunch_width;
-      if (rel_tscrunch_width < 1) rel_tscrunch_width = 1;
-      size_t rel_rel_filter_width = rel_filter_width / rel_tscrunch_width;
-      orc_boxcar(scanned, cur_nsamps + 1, rel_boxcar_max, rel_filter_width,
-                 rel_tscrunch_width, filt);                 /* :680-682 */
-      size_t cur_nsamps_filtered = ((max_nsamps_filtered - 1) / rel_tscrunch_width + 1);
-      size_t cur_scrunch = cur_dm_scrunch * rel_tscrunch_width;
-      size_t bc = cur_beam_stride - (cur_dm_delay + rel_boxcar_max); /* :692 */
-      float block_rms = orc_block_rms(filt, cur_beam_stride, bc, nbeams); /* :709 */
-      orc_normalise_block(filt, block_rms, cur_nsamps_filtered);         /* :712 */
-      size_t ng;
-      for (;;) {
-        ng = orc_find_giants(filt, cur_nsamps_filtered, p->detect_thresh,
-                             p->cand_sep_time * rel_rel_filter_width,
-                             cur_beam_stride, bc, tmp, tmpcap);
-        if (ng <= tmpcap) break;
-        tmpcap = ng;
-        tmp = (orc_cand*)realloc(tmp, sizeof(orc_cand) * tmpcap);
-      }
-      size_t rel_off = cur_filtered_offset / rel_tscrunch_width; /* :755-756 */
-      for (size_t g = 0; g < ng; ++g) {                      /* :759-775 */
-        tmp[g].ind = (tmp[g].ind + rel_off) * cur_scrunch;
-        tmp[g].begin = (tmp[g].begin + rel_off) * cur_scrunch;
-        tmp[g].end = (tmp[g].end + rel_off) * cur_scrunch;
-        tmp[g].filter = filter_idx;
-        tmp[g].dm = dm_idx;
-        tmp[g].members = 1;
-      }
-      push_giants(r, &gcap, tmp, ng);
-      /* :780-787 */
-      float data_length_mins = (float)((double)((float)nsamps * p->dt) / 60.0);
-      if (p->max_giant_rate != 0 &&
-          ((float)r->n_giants / data_length_mins > p->max_giant_rate)) {
-        too_many = 1;
-        break;
-      }
+  struct dm_ctx ctx = {p, nsamps, nbeams, nsamps_computed, max_delay, W};
+  if (!skip && (p->max_giant_rate != 0 || p->nthreads <= 1)) {
+    /* sequential, with the giant-rate test after every width (:780-787) */
+    struct dm_state* st = dm_state_create(series_stride, nbeams);
+    size_t gcap = 0;
+    for (size_t k = 0; k < n_sub && !too_many; ++k) {
+      too_many = process_dm(&ctx, st, sub_idx[k], plane + k * series_stride, r, &gcap);
     }
+    dm_state_destroy(st);
+  } else if (!skip) {
+    /* independent trials in parallel; giants concatenated in trial order */
+    orc_result* part = (orc_result*)calloc(n_sub ? n_sub : 1, sizeof(orc_result));
+#ifdef _OPENMP
+#pragma omp parallel num_threads(p->nthreads)
+#endif
+    {
+      struct dm_state* st = dm_state_create(series_stride, nbeams);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+      for (long k = 0; k < (long)n_sub; ++k) {
+        size_t gcap = 0;
+        process_dm(&ctx, st, sub_idx[k], plane + (size_t)k * series_stride, &part[k], &gcap);
+      }
+      dm_state_destroy(st);
+    }
+    size_t total = 0;
+    for (size_t k = 0; k < n_sub; ++k) total += part[k].n_giants;
+    r->giants = (orc_cand*)malloc(sizeof(orc_cand) * (total ? total : 1));
+    for (size_t k = 0; k < n_sub; ++k) {
+      memcpy(r->giants + r->n_giants, part[k].giants, part[k].n_giants * sizeof(orc_cand));
+      r->n_giants += part[k].n_giants;
+      free(part[k].giants);
+    }
+    free(part);
   }
+  free(sub_idx);
+  free(sub_dm);
 
   if (!too_many && r->n_giants > 0) {
     uint64_t* labels = (uint64_t*)malloc(sizeof(uint64_t) * r->n_giants);
@@ -704,8 +776,6 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
     free(labels);
   }
   r->error = too_many ? 12 : 0;
-  orc_work_destroy(work);
-  free(tmp); free(ts); free(scanned); free(filt); free(beam_rms);
   free(plane); free(dm_list); free(kc);
   r->seconds = now_s() - t0;
   return r->error;

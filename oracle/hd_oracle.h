@@ -113,8 +113,9 @@ typedef struct {
   size_t min_tscrunch_width;
   size_t beam;                 /* first beam index (0-based) for output */
   const int* killmask;         /* nchans ints or NULL (all live) */
-  int    nthreads;             /* OpenMP threads for dedispersion */
+  int    nthreads;             /* OpenMP threads (dedispersion and the DM loop) */
   size_t dm_first, dm_count;   /* DM subset processed (dm_count 0 = all) */
+  size_t dm_stride;            /* process every dm_stride-th trial of the subset (0 = 1) */
 } orc_params;
 
 typedef struct {

@@ -37,7 +37,7 @@ class Params(C.Structure):
                 ("cand_sep_dm", C.c_size_t), ("max_giant_rate", C.c_float),
                 ("min_tscrunch_width", C.c_size_t), ("beam", C.c_size_t),
                 ("killmask", C.POINTER(C.c_int)), ("nthreads", C.c_int),
-                ("dm_first", C.c_size_t), ("dm_count", C.c_size_t)]
+                ("dm_first", C.c_size_t), ("dm_count", C.c_size_t), ("dm_stride", C.c_size_t)]
 
 
 class Result(C.Structure):

@@ -308,3 +308,19 @@ def test_execute_recovers_injected_pulse(oracle):
     assert abs(int(best["dm"]) - dm_idx) <= 1
     assert abs(int(best["ind"]) - int(t0)) <= 4
     assert best["peak"] > 25
+
+
+def test_parallel_trials_match_sequential(oracle):
+    """The DM loop run in parallel gives the sequential result (trials are
+    independent; giants are concatenated in trial order)."""
+    from heimdall_multibeam_amd import synth
+    from tests._cases import SMALL_128, oracle_params_for
+    fb = synth.make_gulp(SMALL_128, 8)
+    r1 = oracle.execute(oracle_params_for(oracle, SMALL_128, nthreads=1), fb, SMALL_128.nsamps,
+                        8, 0, 1)
+    r8 = oracle.execute(oracle_params_for(oracle, SMALL_128, nthreads=8), fb, SMALL_128.nsamps,
+                        8, 0, 1)
+    for f in r1.giants.dtype.names:  # field by field (struct padding is not data)
+        np.testing.assert_array_equal(r1.giants[f], r8.giants[f])
+        np.testing.assert_array_equal(r1.groups[f], r8.groups[f])
+    assert r1.text == r8.text
