@@ -87,7 +87,7 @@ struct hd_pipeline_t {
   std::vector<float> kc;
   size_t max_delay = 0;
   // device buffers
-  DevBuf fb, x, live, delays, plane, pts, rms, S, partial, wrms, winv;
+  DevBuf fb, x, live, delays, chunk_pairs, plane, pts, rms, S, partial, wrms, winv;
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
   DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
@@ -97,7 +97,7 @@ struct hd_pipeline_t {
   // delay table state for the current live-channel set
   std::vector<int> live_list;
   std::vector<int> killmask_cached;
-  size_t wlen = 0;
+  size_t wlen = 0;  // staged dwords per copy of the widest DM block
   // results of the last call
   size_t n_giants = 0;
   std::vector<float> r_peak, r_dms;
@@ -334,19 +334,35 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   for (size_t m = 0; m < n_dm; ++m)
     for (size_t l = 0; l < nl; ++l)
       d[m * nl + l] = hdplan::delay(pl->dm_list[m], pl->kc[pl->live_list[l]]);
-  // LDS window: widest (last - first) delay span of a DM block, dword aligned
+  // LDS window per DM block: widest (last - first) delay span over channels,
+  // the first trial's delay rounded down to a dword
+  // (and per chunk of channels staged together)
   const int DB = hd::dedisp_tile_dms();
-  int spread = 0;
-  for (size_t m0 = 0; m0 < n_dm; m0 += DB) {
+  const int CC = hd::dedisp_chunk_channels();
+  const size_t nblk = (n_dm + DB - 1) / DB;
+  const size_t nchunk = (nl + CC - 1) / CC;
+  std::vector<int> pairs(std::max<size_t>(nblk * nchunk, 1));
+  int hw_max = 2;
+  for (size_t b = 0; b < nblk; ++b) {
+    const size_t m0 = b * DB;
     const size_t ml = std::min(m0 + DB, n_dm) - 1;
-    for (size_t l = 0; l < nl; ++l)
-      spread = std::max(spread, d[ml * nl + l] - (d[m0 * nl + l] & ~3));
+    for (size_t k = 0; k < nchunk; ++k) {
+      int spread = 0;
+      for (size_t l = k * CC; l < std::min((k + 1) * CC, nl); ++l)
+        spread = std::max(spread, d[ml * nl + l] - (d[m0 * nl + l] & ~3));
+      const int hw = hd::dedisp_window_dwords(spread);
+      pairs[b * nchunk + k] = hw / 2;
+      hw_max = std::max(hw_max, hw);
+    }
   }
-  pl->wlen = round_up((size_t)spread + hd::dedisp_tile_time() + 8, 4);
+  pl->wlen = (size_t)hw_max;
   HD_CHECK(pl->live.ensure(std::max<size_t>(nl, 1) * sizeof(int)));
   HD_CHECK(pl->delays.ensure(d.size() * sizeof(int)));
+  HD_CHECK(pl->chunk_pairs.ensure(pairs.size() * sizeof(int)));
   HD_CHECK(hipMemcpy(pl->live.p, pl->live_list.data(), nl * sizeof(int), hipMemcpyHostToDevice));
   HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
+  HD_CHECK(hipMemcpy(pl->chunk_pairs.p, pairs.data(), pairs.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
   return HD_NO_ERROR;
 }
 
@@ -573,7 +589,7 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
 
   record(pl, 0);
   // unpack to channel-major bytes
-  const size_t xpitch = round_up(sh.ppitch + max_delay + pl->wlen + 64, 256);
+  const size_t xpitch = round_up(sh.ppitch + max_delay + 2 * pl->wlen + 256, 256);
   HD_CHECK(pl->x.ensure((size_t)std::max(n_live, 1) * xpitch));
   HD_CHECK(hd::launch_unpack(d_fb, p.nchans * nbits / 8, (int)nbits, nsamps, pl->live.as<int>(),
                              n_live, pl->x.as<uint8_t>(), xpitch, s));
@@ -582,7 +598,7 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   HD_CHECK(pl->plane.ensure((size_t)n_dm * sh.ppitch * sizeof(float)));
   hd::DedispGeom g{n_dm, n_live, sh.nout, xpitch, sh.ppitch, (int)pl->wlen};
   (void)TB;
-  HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), g,
+  HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pl->chunk_pairs.as<int>(), g,
                                  pl->plane.as<float>(), s));
   record(pl, 2);
   // baseline points, per-beam RMS, normalise
@@ -596,8 +612,11 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
                                 sh, n_dm, s));
   record(pl, 4);
   // prefix sums
-  HD_CHECK(pl->S.ensure((size_t)n_dm * sh.spitch * sizeof(float)));
-  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), sh.ppitch, sh.nout, n_dm, pl->S.as<float>(),
+  // prefix-sum rows start 3 floats into the allocation so that S[1..] is
+  // 16-byte aligned for the scan's vector stores
+  HD_CHECK(pl->S.ensure(((size_t)n_dm * sh.spitch + 8) * sizeof(float)));
+  float* const Sbase = pl->S.as<float>() + 3;
+  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), sh.ppitch, sh.nout, n_dm, Sbase,
                            sh.spitch, s));
   record(pl, 5);
   // block RMS: the ts == 1 widths together, then each decimated width in order
@@ -607,12 +626,12 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   HD_CHECK(pl->winv.ensure((size_t)n_dm * wt.nw * sizeof(float)));
   int n_fresh = 0;
   while (n_fresh < wt.nw && wt.wi[n_fresh].ts == 1) ++n_fresh;
-  HD_CHECK(hd::launch_block_rms_partials(pl->S.as<float>(), sh, wt, bs, 0, n_fresh, n_dm,
+  HD_CHECK(hd::launch_block_rms_partials(Sbase, sh, wt, bs, 0, n_fresh, n_dm,
                                          pl->partial.as<float>(), pl->winv.as<float>(), s));
   HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), bs, wt.nw, 0, n_fresh, n_dm,
                                        pl->wrms.as<float>(), pl->winv.as<float>(), s));
   for (int j = n_fresh; j < wt.nw; ++j) {
-    HD_CHECK(hd::launch_block_rms_partials(pl->S.as<float>(), sh, wt, bs, j, 1, n_dm,
+    HD_CHECK(hd::launch_block_rms_partials(Sbase, sh, wt, bs, j, 1, n_dm,
                                            pl->partial.as<float>(), pl->winv.as<float>(), s));
     HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), bs, wt.nw, j, 1, n_dm,
                                          pl->wrms.as<float>(), pl->winv.as<float>(), s));
@@ -642,7 +661,7 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
     gb.cpos = pl->cpos.as<unsigned>();
     gb.scan_tmp = pl->scan_tmp.as<unsigned>();
     HD_CHECK(hipMemsetAsync(gb.counter, 0, sizeof(unsigned), s));
-    HD_CHECK(hd::launch_threshold(pl->S.as<float>(), sh.spitch, sh, wt, pl->winv.as<float>(),
+    HD_CHECK(hd::launch_threshold(Sbase, sh.spitch, sh, wt, pl->winv.as<float>(),
                                   n_dm, p.detect_thresh, cs.bc692, gb, s));
     HD_CHECK(hipMemcpyAsync(pl->h_pinned, gb.counter, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HD_CHECK(hipStreamSynchronize(s));
@@ -860,7 +879,7 @@ extern "C" hd_error hd_preallocate(hd_pipeline pl, hd_size nsamps, hd_size nbeam
   const size_t ppitch = round_up(max_alloc + 1, (size_t)hd::dedisp_tile_time());
   HD_CHECK(pl->fb.ensure(max_alloc * pl->params.nchans));
   HD_CHECK(pl->plane.ensure(n_dm * ppitch * sizeof(float)));
-  HD_CHECK(pl->S.ensure(n_dm * round_up(max_alloc + 1, 64) * sizeof(float)));
+  HD_CHECK(pl->S.ensure((n_dm * round_up(max_alloc + 1, 64) + 8) * sizeof(float)));
   return HD_NO_ERROR;
 }
 
@@ -904,7 +923,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
   {
     DeviceGuard dg(pl->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    DevBuf* bufs[] = {&pl->fb, &pl->x, &pl->live, &pl->delays, &pl->plane, &pl->pts, &pl->rms,
+    DevBuf* bufs[] = {&pl->fb, &pl->x, &pl->live, &pl->delays, &pl->chunk_pairs, &pl->plane, &pl->pts, &pl->rms,
                       &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
                       &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->okey, &pl->oval,
                       &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,

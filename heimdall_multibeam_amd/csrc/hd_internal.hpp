@@ -77,12 +77,14 @@ struct DedispGeom {
   size_t nout;         // outputs per DM
   size_t xpitch;       // bytes per unpacked channel row
   size_t ppitch;       // floats per output row
-  int wlen;            // LDS window bytes per channel (multiple of 4)
+  int wlen;            // LDS dwords per staged copy (the largest block's need)
 };
-hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const DedispGeom& g,
-                             float* plane, hipStream_t s);
+hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
+                             const DedispGeom& g, float* plane, hipStream_t s);
 int dedisp_tile_time();     // outputs per block in time
 int dedisp_tile_dms();      // DMs per block
+int dedisp_window_dwords(int spread);  // staged dwords per copy for a delay span
+int dedisp_chunk_channels();           // channels staged per step
 
 hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
                                   float* pts, hipStream_t s);

@@ -8,26 +8,15 @@
 // sample in channel-major rows x[live][t] (row pitch xpitch, zero padded), so
 // every channel row is a contiguous byte stream in HBM.
 //
-// Dedispersion tile: a 256-thread block owns 8 consecutive DM trials x 512
-// output samples.  For each chunk of 16 channels it stages, per channel, the
-// byte window [t0 + d_min, t0 + d_max + 512) into LDS (d_min/d_max = delays
-// of the block's first/last trial; delays are monotone in DM).  Each thread
-// then sums 16 consecutive outputs of one trial: 5 aligned LDS dwords,
-// byte-aligned with v_alignbyte, accumulated as packed 16-bit lanes
-// (even/odd bytes) and widened to 32 bits every <= 256 channels.
+// Dedispersion stages, per block of DM trials and channel, the byte window
+// [t0 + d_min, t0 + d_max + tile) into LDS (delays are monotone in DM), so
+// each input byte is read from L2 once per block and many times from LDS.
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
 
 namespace hd {
 
 namespace {
-
-constexpr int DD_DB = 8;                 // DM trials per block
-constexpr int DD_TPD = 32;               // threads per trial
-constexpr int DD_R = 16;                 // outputs per thread
-constexpr int DD_TB = DD_TPD * DD_R;     // outputs per block (time)
-constexpr int DD_CC = 16;                // channels staged per step
-constexpr int DD_BS = DD_DB * DD_TPD;    // 256 threads
 
 __device__ __forceinline__ unsigned unpack_sample(const uint8_t* row, int c, int nbits) {
   if (nbits == 8) return row[c];
@@ -63,97 +52,173 @@ __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ fb,
   }
 }
 
-__global__ __launch_bounds__(DD_BS) void k_dedisperse(const uint8_t* __restrict__ x,
-                                                      const int* __restrict__ delays,
-                                                      DedispGeom g, float* __restrict__ out) {
-  extern __shared__ uint32_t win[];          // DD_CC rows of wlen bytes
-  __shared__ int sdly[DD_DB][DD_CC];
+// ---------------------------------------------------------------------------
+// Dedispersion, 16-bit staging.  A 512-thread block owns 16 consecutive DM
+// trials x 1024 output samples; wave w sums trials w and w+8.  Per channel
+// the block's byte window [t0 + base, ...) (base = first trial's delay,
+// dword aligned) is staged twice as 16-bit sample pairs: E[k] = (x[2k],
+// x[2k+1]) and O[k] = (x[2k+1], x[2k+2]).  A trial's delay offset o is the
+// same for every lane of the wave, so it reads E (o even) or O (o odd) at
+// dword (o>>1) + lane + 64q: consecutive lanes hit consecutive banks, and one
+// 32-bit add accumulates two samples (16-bit lanes never carry for <= 257
+// channels; they are widened to 32 bits every <= 256 channels).
+constexpr int D2_DB = 16;                 // trials per block
+constexpr int D2_Q = 8;                   // sample pairs per lane per trial
+constexpr int D2_TB = 64 * 2 * D2_Q;      // 1024 outputs per block
+constexpr int D2_CC = 4;                  // channels staged per step
+constexpr int D2_BS = 512;
+constexpr int D2_WAVES = D2_BS / 64;     // 8: wave w sums trials w, w+8
+constexpr int D2_PF = 6;                  // staging items prefetched per thread
+
+// chunk_pairs[block][chunk]: staged E/O dword pairs each channel of a
+// 4-channel chunk needs for this block of trials (host-computed from the
+// delay spans, so top-of-band chunks stage short windows).
+__global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restrict__ x,
+                                                       const int* __restrict__ delays,
+                                                       const int* __restrict__ chunk_pairs,
+                                                       DedispGeom g, int hw_max,
+                                                       float* __restrict__ out) {
+  extern __shared__ uint32_t lds16[];  // [D2_CC][2][hw_max]
+  __shared__ int sdly[D2_DB][D2_CC];
   const int tid = threadIdx.x;
-  const int dml = tid / DD_TPD;
-  const int tl = (tid % DD_TPD) * DD_R;
-  const size_t t0 = (size_t)blockIdx.x * DD_TB;
-  const int m0 = blockIdx.y * DD_DB;
-  const int wd = g.wlen >> 2;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int mb = blockIdx.x;
+  const int m0 = mb * D2_DB;
+  const size_t t0 = (size_t)blockIdx.y * D2_TB;
   const int* drow0 = delays + (size_t)m0 * g.n_live;
 
-  uint32_t acc[DD_R];
+  uint32_t pa[2][D2_Q];
+  uint32_t ua[2][D2_Q][2];
 #pragma unroll
-  for (int i = 0; i < DD_R; ++i) acc[i] = 0;
-  uint32_t ae[DD_R / 4], ao[DD_R / 4];
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-  for (int i = 0; i < DD_R / 4; ++i) { ae[i] = 0; ao[i] = 0; }
+    for (int q = 0; q < D2_Q; ++q) {
+      pa[a][q] = 0;
+      ua[a][q][0] = 0;
+      ua[a][q][1] = 0;
+    }
   int pending = 0;
 
-  for (int c0 = 0; c0 < g.n_live; c0 += DD_CC) {
-    const int ncc = min(DD_CC, g.n_live - c0);
-    __syncthreads();
-    if (tid < DD_DB * DD_CC) {
-      const int d = tid / DD_CC, cc = tid % DD_CC;
-      if (cc < ncc) {
-        const int mm = min(m0 + d, g.n_dm - 1);
-        const int base = drow0[c0 + cc] & ~3;
-        sdly[d][cc] = delays[(size_t)mm * g.n_live + c0 + cc] - base;
+  // Staging of a channel chunk: item (cc, j) turns source dwords j, j+1 of
+  // channel c0+cc into E/O dwords 2j, 2j+1.  The global loads of chunk k+1
+  // are issued into registers before chunk k is summed, so their latency
+  // hides behind the LDS work.
+  const int nchunk = (g.n_live + D2_CC - 1) / D2_CC;
+  const int* cpairs = chunk_pairs + (size_t)mb * nchunk;
+  // D2_BS/D2_CC threads per staged channel; thread jj handles pairs jj + TPC*r
+  constexpr int TPC = D2_BS / D2_CC;
+  const int scc = tid / TPC;
+  const int sjj = tid % TPC;
+  uint32_t pf_lo[D2_PF], pf_hi[D2_PF];
+  auto prefetch = [&](int k) {
+    if (k >= nchunk) return;
+    const int c = k * D2_CC + scc;
+    if (c < g.n_live) {
+      const int pairs = cpairs[k];
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(
+          x + (size_t)c * g.xpitch + t0 + (drow0[c] & ~3));
+#pragma unroll
+      for (int r = 0; r < D2_PF; ++r) {
+        const int j = sjj + r * TPC;
+        if (j < pairs) {
+          pf_lo[r] = src[j];
+          pf_hi[r] = src[j + 1];
+        }
       }
     }
-    for (int idx = tid; idx < ncc * wd; idx += DD_BS) {
-      const int cc = idx / wd;
-      const int k = idx - cc * wd;
-      const int base = drow0[c0 + cc] & ~3;
-      const uint32_t* src =
-          reinterpret_cast<const uint32_t*>(x + (size_t)(c0 + cc) * g.xpitch + t0 + base);
-      win[cc * wd + k] = src[k];
+  };
+  // bytes b0..b3 of lo and b4 of hi -> E pair (b0,b1)(b2,b3), O pair (b1,b2)(b3,b4)
+  auto put = [&](int j, uint32_t lo, uint32_t hi) {
+    uint32_t* E = lds16 + (size_t)(scc * 2) * hw_max;
+    uint32_t* O = E + hw_max;
+    uint2 e, o;
+    e.x = __builtin_amdgcn_perm(hi, lo, 0x0C010C00u);
+    e.y = __builtin_amdgcn_perm(hi, lo, 0x0C030C02u);
+    o.x = __builtin_amdgcn_perm(hi, lo, 0x0C020C01u);
+    o.y = __builtin_amdgcn_perm(hi, lo, 0x0C040C03u);
+    *reinterpret_cast<uint2*>(E + 2 * j) = e;
+    *reinterpret_cast<uint2*>(O + 2 * j) = o;
+  };
+  prefetch(0);
+  int pf_delay = 0;
+  auto prefetch_delay = [&](int k) {
+    if (k < nchunk && tid < D2_DB * D2_CC) {
+      const int c = min(k * D2_CC + tid % D2_CC, g.n_live - 1);
+      const int mm = min(m0 + tid / D2_CC, g.n_dm - 1);
+      pf_delay = delays[(size_t)mm * g.n_live + c] - (drow0[c] & ~3);
+    }
+  };
+  prefetch_delay(0);
+  for (int k = 0; k < nchunk; ++k) {
+    const int c0 = k * D2_CC;
+    const int ncc = min(D2_CC, g.n_live - c0);
+    const int pairs = cpairs[k];
+    __syncthreads();
+    if (tid < D2_DB * D2_CC) sdly[tid / D2_CC][tid % D2_CC] = pf_delay;
+    if (scc < ncc) {
+#pragma unroll
+      for (int r = 0; r < D2_PF; ++r) {
+        const int j = sjj + r * TPC;
+        if (j < pairs) put(j, pf_lo[r], pf_hi[r]);
+      }
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(
+          x + (size_t)(c0 + scc) * g.xpitch + t0 + (drow0[c0 + scc] & ~3));
+      for (int j = sjj + D2_PF * TPC; j < pairs; j += TPC) put(j, src[j], src[j + 1]);
     }
     __syncthreads();
+    prefetch(k + 1);
+    prefetch_delay(k + 1);
     for (int cc = 0; cc < ncc; ++cc) {
-      const int off = sdly[dml][cc] + tl;
-      const uint32_t* p = win + cc * wd + (off >> 2);
-      const uint32_t sh = (uint32_t)(off & 3);
-      uint32_t w[DD_R / 4 + 1];
 #pragma unroll
-      for (int i = 0; i <= DD_R / 4; ++i) w[i] = p[i];
+      for (int a = 0; a < 2; ++a) {
+        const int o = sdly[wave + D2_WAVES * a][cc];
+        const uint32_t* src = lds16 + (cc * 2 + (o & 1)) * hw_max + (o >> 1) + lane;
 #pragma unroll
-      for (int i = 0; i < DD_R / 4; ++i) {
-        const uint32_t a = __builtin_amdgcn_alignbyte(w[i + 1], w[i], sh);
-        ae[i] += a & 0x00FF00FFu;
-        ao[i] += (a >> 8) & 0x00FF00FFu;
+        for (int q = 0; q < D2_Q; ++q) pa[a][q] += src[64 * q];
       }
     }
     pending += ncc;
-    if (pending > 256 - DD_CC || c0 + DD_CC >= g.n_live) {
+    if (pending > 256 - D2_CC || c0 + D2_CC >= g.n_live) {
 #pragma unroll
-      for (int i = 0; i < DD_R / 4; ++i) {
-        acc[4 * i + 0] += ae[i] & 0xFFFFu;
-        acc[4 * i + 2] += ae[i] >> 16;
-        acc[4 * i + 1] += ao[i] & 0xFFFFu;
-        acc[4 * i + 3] += ao[i] >> 16;
-        ae[i] = 0;
-        ao[i] = 0;
-      }
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int q = 0; q < D2_Q; ++q) {
+          ua[a][q][0] += pa[a][q] & 0xFFFFu;
+          ua[a][q][1] += pa[a][q] >> 16;
+          pa[a][q] = 0;
+        }
       pending = 0;
     }
   }
-  const int m = m0 + dml;
-  if (m >= g.n_dm) return;
-  float* orow = out + (size_t)m * g.ppitch;
-  const size_t tb = t0 + tl;
 #pragma unroll
-  for (int q = 0; q < DD_R / 4; ++q) {
-    const size_t t = tb + 4 * q;
-    if (t + 3 < g.ppitch) {
-      float4 v;
-      v.x = (t + 0 < g.nout) ? (float)acc[4 * q + 0] : 0.f;
-      v.y = (t + 1 < g.nout) ? (float)acc[4 * q + 1] : 0.f;
-      v.z = (t + 2 < g.nout) ? (float)acc[4 * q + 2] : 0.f;
-      v.w = (t + 3 < g.nout) ? (float)acc[4 * q + 3] : 0.f;
-      *reinterpret_cast<float4*>(orow + t) = v;
+  for (int a = 0; a < 2; ++a) {
+    const int m = m0 + wave + D2_WAVES * a;
+    if (m >= g.n_dm) continue;
+    float* orow = out + (size_t)m * g.ppitch;
+#pragma unroll
+    for (int q = 0; q < D2_Q; ++q) {
+      const size_t t = t0 + 2 * lane + 128 * q;
+      if (t + 1 < g.ppitch) {
+        float2 v;
+        v.x = (t < g.nout) ? (float)ua[a][q][0] : 0.f;
+        v.y = (t + 1 < g.nout) ? (float)ua[a][q][1] : 0.f;
+        *reinterpret_cast<float2*>(orow + t) = v;
+      }
     }
   }
 }
 
 }  // namespace
 
-int dedisp_tile_time() { return DD_TB; }
-int dedisp_tile_dms() { return DD_DB; }
+int dedisp_tile_time() { return D2_TB; }
+int dedisp_tile_dms() { return D2_DB; }
+int dedisp_chunk_channels() { return D2_CC; }
+int dedisp_window_dwords(int spread) {
+  // E/O copies: dword (o>>1) + 511 with o <= spread + 3, rounded to 64
+  const int need = (spread + 3) / 2 + 512 + 1;
+  return (need + 63) / 64 * 64;
+}
 
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
@@ -164,12 +229,19 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
   return hipGetLastError();
 }
 
-hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const DedispGeom& g,
-                             float* plane, hipStream_t s) {
-  dim3 grid((unsigned)(g.ppitch / DD_TB), (unsigned)((g.n_dm + DD_DB - 1) / DD_DB));
-  const size_t lds = (size_t)DD_CC * (size_t)g.wlen;
-  if (lds > 65536) return hipErrorInvalidValue;  // delay span of a DM block too wide
-  hipLaunchKernelGGL(k_dedisperse, grid, dim3(DD_BS), lds, s, x, d_delays, g, plane);
+hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
+                             const DedispGeom& g, float* plane, hipStream_t s) {
+  // DM blocks fastest: blocks resident together share one time tile's input
+  dim3 grid((unsigned)((g.n_dm + D2_DB - 1) / D2_DB), (unsigned)(g.ppitch / D2_TB));
+  const size_t lds = (size_t)D2_CC * 2 * (size_t)g.wlen * sizeof(uint32_t);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;  // delay span of a DM block too wide
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dedisperse16),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_dedisperse16, grid, dim3(D2_BS), lds, s, x, d_delays, d_chunk_pairs, g,
+                     g.wlen, plane);
   return hipGetLastError();
 }
 

@@ -168,56 +168,88 @@ __device__ unsigned block_exclusive_scan(unsigned v, unsigned* total) {
 }
 
 constexpr int TH_PER = 16;
-constexpr int TH_CHUNK = FB_BS * TH_PER;  // 4096 outputs per block
+constexpr int TH_CHUNK = FB_BS * TH_PER;  // 4096 prefix-sum positions per block
 
+// One block per (chunk of TH_CHUNK positions i*ts, DM): the chunk's prefix
+// sums (plus the widest boxcar's halo) are staged in LDS once and every width
+// is thresholded from them.  Output i of a width belongs to the chunk holding
+// i*ts, so each (DM, width, chunk) key covers a time-ordered run of outputs.
 __global__ __launch_bounds__(FB_BS) void k_threshold(
     const float* __restrict__ Sall, size_t spitch, size_t stride, size_t bc692, WidthTable wt,
-    const float* __restrict__ winv_all, float thresh, GiantBuffers gb) {
+    unsigned amax, unsigned bmax, size_t s_len, const float* __restrict__ winv_all,
+    float thresh, GiantBuffers gb) {
+  extern __shared__ float T[];  // TH_CHUNK + amax + bmax
   const size_t c = blockIdx.x;
-  const int j = blockIdx.y;
-  const int m = blockIdx.z;
-  const WidthInfo wi = wt.wi[j];
+  const int m = blockIdx.y;
   const float* S = Sall + (size_t)m * spitch;
-  const float inv = winv_all[(size_t)m * wt.nw + j];
-  const size_t ib = c * TH_CHUNK + (size_t)threadIdx.x * TH_PER;
-  float vals[TH_PER];
-  unsigned mask = 0;
-#pragma unroll
-  for (int q = 0; q < TH_PER; ++q) {
-    const size_t i = ib + q;
-    vals[q] = 0.f;
-    if (i < wi.n_w) {
-      const float v = raw_val(S, wi, wt.off, i) * inv;  // get_rms.cu:194-197
-      vals[q] = v;
-      const size_t samp = (size_t)(unsigned)i % stride;  // find_giants.cu:55-58
-      if (samp < bc692 && v > thresh) mask |= 1u << q;
-    }
-  }
-  const unsigned cnt = __popc(mask);
-  unsigned total;
-  const unsigned rank = block_exclusive_scan<FB_BS>(cnt, &total);
-  __shared__ unsigned sbase;
-  const size_t key = ((size_t)m * wt.nw + j) * gb.nchunks_per_w + c;
-  if (threadIdx.x == 0) {
-    unsigned base = 0;
-    if (total) base = atomicAdd(gb.counter, total);
-    sbase = base;
-    gb.ccount[key] = total;
-    gb.coff[key] = base;
+  const size_t p0 = c * TH_CHUNK;                 // first position of the chunk
+  const size_t tbase = p0 + wt.off - bmax;        // S index held in T[0]
+  const size_t tlen = TH_CHUNK + amax + bmax;
+  for (size_t t = threadIdx.x; t < tlen; t += FB_BS) {
+    const size_t si = tbase + t;
+    T[t] = si < s_len ? S[si] : 0.f;
   }
   __syncthreads();
-  if (!mask) return;
-  const size_t base = sbase;
-  if (base + total > gb.scap) return;  // overflow: detected by the host
-  unsigned r = rank;
-  const uint64_t dmw = (uint64_t)m * wt.nw + j;
-#pragma unroll
-  for (int q = 0; q < TH_PER; ++q) {
-    if (mask & (1u << q)) {
-      gb.skey[base + r] = (dmw << 32) | (uint64_t)(ib + q);
-      gb.sval[base + r] = vals[q];
-      ++r;
+  __shared__ unsigned sbase;
+  for (int j = 0; j < wt.nw; ++j) {
+    const WidthInfo wi = wt.wi[j];
+    const float inv = winv_all[(size_t)m * wt.nw + j];
+    // outputs i with i*ts in [p0, p0 + TH_CHUNK)
+    const size_t i0 = (p0 + wi.ts - 1) / wi.ts;
+    size_t i1 = (p0 + TH_CHUNK + wi.ts - 1) / wi.ts;
+    if (i1 > wi.n_w) i1 = wi.n_w;
+    const size_t key = ((size_t)m * wt.nw + j) * gb.nchunks_per_w + c;
+    if (i0 >= i1) {
+      if (threadIdx.x == 0) gb.ccount[key] = 0;
+      continue;
     }
+    const size_t nout = i1 - i0;
+    const size_t per = (nout + FB_BS - 1) / FB_BS;  // <= TH_PER
+    const size_t ib = i0 + (size_t)threadIdx.x * per;
+    float vals[TH_PER];
+    unsigned mask = 0;
+#pragma unroll
+    for (int q = 0; q < TH_PER; ++q) {
+      vals[q] = 0.f;
+      const size_t i = ib + q;
+      if (q < (int)per && i < i1) {
+        const size_t r = i * wi.ts - p0 + bmax;  // T index of position i*ts
+        const float a = T[r + wi.ahead];
+        const float b = T[r - wi.behind];
+        const float v = (a - b) * inv;               // get_rms.cu:194-197
+        vals[q] = v;
+        const size_t samp = (size_t)(unsigned)i % stride;  // find_giants.cu:55-58
+        if (samp < bc692 && v > thresh) mask |= 1u << q;
+      }
+    }
+    const int any = __syncthreads_or(mask != 0);
+    if (!any) {
+      if (threadIdx.x == 0) gb.ccount[key] = 0;
+      continue;
+    }
+    const unsigned cnt = __popc(mask);
+    unsigned total;
+    const unsigned rank = block_exclusive_scan<FB_BS>(cnt, &total);
+    if (threadIdx.x == 0) {
+      sbase = atomicAdd(gb.counter, total);
+      gb.ccount[key] = total;
+      gb.coff[key] = sbase;
+    }
+    __syncthreads();
+    const size_t base = sbase;
+    if (mask && base + total <= gb.scap) {  // overflow: detected by the host
+      unsigned rr = rank;
+      const uint64_t dmw = (uint64_t)m * wt.nw + j;
+#pragma unroll
+      for (int q = 0; q < TH_PER; ++q) {
+        if (mask & (1u << q)) {
+          gb.skey[base + rr] = (dmw << 32) | (uint64_t)(ib + q);
+          gb.sval[base + rr] = vals[q];
+          ++rr;
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -354,9 +386,16 @@ hipError_t launch_threshold(const float* S, size_t spitch, const SeriesShape& sh
                             const WidthTable& wt, const float* winv, int n_dm, float thresh,
                             size_t bc692, GiantBuffers& gb, hipStream_t s) {
   gb.chunk = TH_CHUNK;
-  dim3 grid((unsigned)gb.nchunks_per_w, wt.nw, n_dm);
-  hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), 0, s, S, spitch, sh.stride, bc692, wt, winv,
-                     thresh, gb);
+  unsigned amax = 0, bmax = 0;
+  for (int j = 0; j < wt.nw; ++j) {
+    amax = wt.wi[j].ahead > amax ? wt.wi[j].ahead : amax;
+    bmax = wt.wi[j].behind > bmax ? wt.wi[j].behind : bmax;
+  }
+  const size_t lds = ((size_t)TH_CHUNK + amax + bmax) * sizeof(float);
+  if (lds > 65536) return hipErrorInvalidValue;  // boxcar_max beyond 12288
+  dim3 grid((unsigned)gb.nchunks_per_w, n_dm);
+  hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), lds, s, S, spitch, sh.stride, bc692, wt,
+                     amax, bmax, sh.nout + 1, winv, thresh, gb);
   return hipGetLastError();
 }
 

@@ -141,35 +141,106 @@ __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
 }
 
 // Sequential exclusive scan per series: S[0] = 0, S[k] = S[k-1] + z[k-1].
-__global__ __launch_bounds__(64) void k_scan(const float* __restrict__ z, size_t zpitch,
-                                             size_t count, int n_series,
-                                             float* __restrict__ S, size_t spitch) {
-  __shared__ float tile[64][65];
-  const int lane = threadIdx.x;
-  const int r0 = blockIdx.x * 64;
-  const int myrow = r0 + lane;
+//
+// A block owns SC_ROWS series.  Wave 0 runs the dependent adds (one lane per
+// series) over a SC_T-sample tile held in LDS while waves 1..8 store the
+// previous tile's prefix sums and load the next tile (double buffer), so the
+// add chain never waits on HBM latency.
+constexpr int SC_ROWS = 16;
+constexpr int SC_T = 1024;
+constexpr int SC_PAD = 4;
+constexpr int SC_MOVERS = 512;                  // waves 1..8
+constexpr int SC_THREADS = 64 + SC_MOVERS;
+constexpr int SC_VEC = SC_ROWS * SC_T / 4;       // float4 per tile
+constexpr int SC_PER = SC_VEC / SC_MOVERS;       // float4 per mover thread
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z, size_t zpitch,
+                                                     size_t count, int n_series,
+                                                     float* __restrict__ S, size_t spitch,
+                                                     int s_vec_ok) {
+  __shared__ float buf[2][SC_ROWS][SC_T + SC_PAD];
+  const int tid = threadIdx.x;
+  const int r0 = blockIdx.x * SC_ROWS;
+  const size_t ntiles = (count + SC_T - 1) / SC_T;
+  const bool z_vec = (zpitch % 4 == 0) && ((reinterpret_cast<uintptr_t>(z) & 15) == 0);
+
+  auto load_tile = [&](size_t tile, int b) {
+    const int mt = tid - 64;
+#pragma unroll
+    for (int q = 0; q < SC_PER; ++q) {
+      const int idx = mt + SC_MOVERS * q;
+      const int row = idx / (SC_T / 4);
+      const int c4 = (idx % (SC_T / 4)) * 4;
+      const size_t t = tile * SC_T + c4;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (r0 + row < n_series) {
+        const float* src = z + (size_t)(r0 + row) * zpitch + t;
+        if (z_vec && t + 3 < count) {
+          v = *reinterpret_cast<const float4*>(src);
+        } else {
+          if (t + 0 < count) v.x = src[0];
+          if (t + 1 < count) v.y = src[1];
+          if (t + 2 < count) v.z = src[2];
+          if (t + 3 < count) v.w = src[3];
+        }
+      }
+      *reinterpret_cast<float4*>(&buf[b][row][c4]) = v;
+    }
+  };
+  auto store_tile = [&](size_t tile, int b) {
+    const int mt = tid - 64;
+#pragma unroll
+    for (int q = 0; q < SC_PER; ++q) {
+      const int idx = mt + SC_MOVERS * q;
+      const int row = idx / (SC_T / 4);
+      const int c4 = (idx % (SC_T / 4)) * 4;
+      const size_t t = tile * SC_T + c4;
+      if (r0 + row >= n_series) continue;
+      const float4 v = *reinterpret_cast<const float4*>(&buf[b][row][c4]);
+      float* dst = S + (size_t)(r0 + row) * spitch + t + 1;
+      if (s_vec_ok && t + 3 < count) {
+        *reinterpret_cast<float4*>(dst) = v;
+      } else {
+        if (t + 0 < count) dst[0] = v.x;
+        if (t + 1 < count) dst[1] = v.y;
+        if (t + 2 < count) dst[2] = v.z;
+        if (t + 3 < count) dst[3] = v.w;
+      }
+    }
+  };
+
+  if (tid < SC_ROWS && r0 + tid < n_series) S[(size_t)(r0 + tid) * spitch] = 0.f;
+  if (tid >= 64 && ntiles > 0) load_tile(0, 0);
+  __syncthreads();
   float run = 0.f;
-  if (myrow < n_series) S[(size_t)myrow * spitch] = 0.f;
-  for (size_t t0 = 0; t0 < count; t0 += 64) {
-    const size_t t = t0 + lane;
-#pragma unroll 8
-    for (int r = 0; r < 64; ++r) {
-      const int row = r0 + r;
-      float v = 0.f;
-      if (row < n_series && t < count) v = z[(size_t)row * zpitch + t];
-      tile[r][lane] = v;
-    }
-    __syncthreads();
-    const int ncol = (int)((count - t0) < 64 ? (count - t0) : 64);
-    for (int c = 0; c < ncol; ++c) {
-      run = run + tile[lane][c];
-      tile[lane][c] = run;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int r = 0; r < 64; ++r) {
-      const int row = r0 + r;
-      if (row < n_series && t < count) S[(size_t)row * spitch + t + 1] = tile[r][lane];
+  for (size_t p = 0; p <= ntiles; ++p) {
+    const int b = (int)(p & 1);
+    if (tid < 64) {
+      if (p < ntiles && tid < SC_ROWS) {
+        const size_t rem = count - p * SC_T;
+        const int ncol = rem < (size_t)SC_T ? (int)rem : SC_T;
+        float* rowp = &buf[b][tid][0];
+        int c = 0;
+        for (; c + 4 <= ncol; c += 4) {
+          float4 v = *reinterpret_cast<float4*>(rowp + c);
+          run = run + v.x;
+          v.x = run;
+          run = run + v.y;
+          v.y = run;
+          run = run + v.z;
+          v.z = run;
+          run = run + v.w;
+          v.w = run;
+          *reinterpret_cast<float4*>(rowp + c) = v;
+        }
+        for (; c < ncol; ++c) {
+          run = run + rowp[c];
+          rowp[c] = run;
+        }
+      }
+    } else {
+      if (p >= 1) store_tile(p - 1, b ^ 1);
+      if (p + 1 < ntiles) load_tile(p + 1, b ^ 1);
     }
     __syncthreads();
   }
@@ -207,8 +278,9 @@ hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
 
 hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series, float* S,
                        size_t spitch, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan, dim3((n_series + 63) / 64), dim3(64), 0, s, z, zpitch, count,
-                     n_series, S, spitch);
+  const int s_vec_ok = (spitch % 4 == 0) && (((reinterpret_cast<uintptr_t>(S) + 4) & 15) == 0);
+  hipLaunchKernelGGL(k_scan, dim3((n_series + SC_ROWS - 1) / SC_ROWS), dim3(SC_THREADS), 0, s, z,
+                     zpitch, count, n_series, S, spitch, s_vec_ok);
   return hipGetLastError();
 }
 
