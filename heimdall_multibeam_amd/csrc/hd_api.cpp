@@ -89,7 +89,7 @@ struct hd_pipeline_t {
   // device buffers
   DevBuf fb, x, live, delays, chunk_pairs, plane, pts, rms, S, partial, wrms, winv;
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
-  DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm;
+  DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm, g_best;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
   DevBuf o_peak, o_ind, o_begin, o_end, o_filter, o_dm, o_members;
   unsigned* h_pinned = nullptr;  // small pinned staging word(s)
@@ -702,7 +702,9 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
     hd::GiantOut go{pl->g_peak.as<float>(), pl->g_ind.as<uint64_t>(), pl->g_begin.as<uint64_t>(),
                     pl->g_end.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
                     pl->g_dm.as<uint32_t>()};
-    HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, go, s));
+    HD_CHECK(pl->g_best.ensure(std::max<size_t>(n_giants, 1) * sizeof(uint64_t)));
+    HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, (unsigned)n_giants, go,
+                                       pl->g_best.as<uint64_t>(), s));
   }
   pl->n_giants = n_giants;
   record(pl, 7);
@@ -927,7 +929,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
                       &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->okey, &pl->oval,
                       &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,
-                      &pl->g_begin, &pl->g_end, &pl->g_filter, &pl->g_dm, &pl->c_keys,
+                      &pl->g_begin, &pl->g_end, &pl->g_filter, &pl->g_dm, &pl->g_best, &pl->c_keys,
                       &pl->c_lo, &pl->c_hi, &pl->c_labels, &pl->c_flag, &pl->c_best,
                       &pl->c_members, &pl->c_rflag, &pl->c_rpos, &pl->c_tmp, &pl->o_peak,
                       &pl->o_ind, &pl->o_begin, &pl->o_end, &pl->o_filter, &pl->o_dm,

@@ -138,7 +138,7 @@ struct GiantOut {
   uint32_t* dm;
 };
 hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
-                                 GiantOut go, hipStream_t s);
+                                 unsigned ngiants, GiantOut go, uint64_t* best, hipStream_t s);
 
 // clustering / merge
 struct ClusterBuffers {

@@ -111,6 +111,8 @@ __global__ void k_relax(const uint64_t* __restrict__ ind, const uint32_t* __rest
   }
 }
 
+__global__ void k_clear_flag(unsigned* flag) { *flag = 0u; }
+
 __global__ void k_merge_init(unsigned n, const int32_t* __restrict__ labels, uint64_t* best,
                              unsigned* members, unsigned* root_flag) {
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,17 +179,19 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
                      cb.lo, cb.hi, cb.labels);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  for (int it = 0; it < 100000; ++it) {
-    e = hipMemsetAsync(cb.flag, 0, sizeof(unsigned), s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_relax, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter, dm, n,
-                       cb.sort_keys, cb.lo, cb.hi, time_tol, filter_tol, dm_tol, nsamps_beam,
-                       cb.labels, cb.flag);
+  // relaxation passes run in batches between host checks of the change flag
+  constexpr int kBatch = 4;
+  for (int it = 0; it < 100000; it += kBatch) {
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, cb.flag);
+    for (int b = 0; b < kBatch; ++b)
+      hipLaunchKernelGGL(k_relax, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter, dm, n,
+                         cb.sort_keys, cb.lo, cb.hi, time_tol, filter_tol, dm_tol, nsamps_beam,
+                         cb.labels, cb.flag);
     e = hipMemcpyAsync(h_flag_pinned, cb.flag, sizeof(unsigned), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    *iterations = it + 1;
+    *iterations = it + kBatch;
     if (*h_flag_pinned == 0) break;
   }
   return hipGetLastError();

@@ -178,7 +178,9 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     const float* __restrict__ Sall, size_t spitch, size_t stride, size_t bc692, WidthTable wt,
     unsigned amax, unsigned bmax, size_t s_len, const float* __restrict__ winv_all,
     float thresh, GiantBuffers gb) {
-  extern __shared__ float T[];  // TH_CHUNK + amax + bmax
+  // T holds prefix sums with one pad word after every 16, so the 16-apart
+  // positions that a wave's lanes read fall in different LDS banks.
+  extern __shared__ float T[];  // (TH_CHUNK + amax + bmax) * 17/16
   const size_t c = blockIdx.x;
   const int m = blockIdx.y;
   const float* S = Sall + (size_t)m * spitch;
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const size_t tlen = TH_CHUNK + amax + bmax;
   for (size_t t = threadIdx.x; t < tlen; t += FB_BS) {
     const size_t si = tbase + t;
-    T[t] = si < s_len ? S[si] : 0.f;
+    T[t + (t >> 4)] = si < s_len ? S[si] : 0.f;
   }
   __syncthreads();
   __shared__ unsigned sbase;
@@ -214,8 +216,9 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       const size_t i = ib + q;
       if (q < (int)per && i < i1) {
         const size_t r = i * wi.ts - p0 + bmax;  // T index of position i*ts
-        const float a = T[r + wi.ahead];
-        const float b = T[r - wi.behind];
+        const size_t ra = r + wi.ahead, rb = r - wi.behind;
+        const float a = T[ra + (ra >> 4)];
+        const float b = T[rb + (rb >> 4)];
         const float v = (a - b) * inv;               // get_rms.cu:194-197
         vals[q] = v;
         const size_t samp = (size_t)(unsigned)i % stride;  // find_giants.cu:55-58
@@ -317,35 +320,46 @@ __global__ void k_segment_heads(GiantBuffers gb, WidthTable wt, unsigned nsamp) 
   gb.head[k] = h;
 }
 
-__global__ void k_segment_reduce(GiantBuffers gb, WidthTable wt, unsigned nsamp, GiantOut go) {
+// Segment reduction in one pass over the samples: every sample offers
+// (value, earliest position) to its giant with a 64-bit atomic max, which
+// keeps the first maximum (maximum_first); the head sample writes the
+// begin, the last sample of a run the end.
+__global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp, GiantOut go,
+                                  uint64_t* best) {
   const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nsamp || !gb.head[k]) return;
-  const unsigned g = gb.segid[k];
+  if (k >= nsamp) return;
+  const unsigned h = gb.head[k];
+  const unsigned g = gb.segid[k] + h - 1u;
   const uint64_t key = gb.okey[k];
   const uint64_t dmw = key >> 32;
   const unsigned j = (unsigned)(dmw % (uint64_t)wt.nw);
-  const unsigned m = (unsigned)(dmw / (uint64_t)wt.nw);
-  float peak = gb.oval[k];
-  uint64_t pind = key & 0xffffffffull;
-  const uint64_t first = pind;
-  uint64_t last = pind;
-  for (unsigned kk = k + 1; kk < nsamp && !gb.head[kk]; ++kk) {
-    const float v = gb.oval[kk];
-    const uint64_t i = gb.okey[kk] & 0xffffffffull;
-    if (!(peak >= v)) {  // maximum_first keeps the earlier of equal peaks
-      peak = v;
-      pind = i;
-    }
-    last = i;
-  }
   const WidthInfo wi = wt.wi[j];
   const uint64_t ro = wi.rel_off, sc = wi.ts;
-  go.peak[g] = peak;
-  go.ind[g] = (pind + ro) * sc;
-  go.begin[g] = (first + ro) * sc;
-  go.end[g] = (last + 1 + ro) * sc;
-  go.filter[g] = wi.filter_idx;
-  go.dm[g] = m;
+  const uint64_t i = key & 0xffffffffull;
+  const uint64_t bk = ((uint64_t)float_key(gb.oval[k]) << 32) | (uint64_t)(0xffffffffu - k);
+  atomicMax((unsigned long long*)&best[g], (unsigned long long)bk);
+  if (h) {
+    go.begin[g] = (i + ro) * sc;
+    go.filter[g] = wi.filter_idx;
+    go.dm[g] = (uint32_t)(dmw / (uint64_t)wt.nw);
+  }
+  if (k + 1 == nsamp || gb.head[k + 1]) go.end[g] = (i + 1 + ro) * sc;
+}
+
+__global__ void k_segment_finish(GiantBuffers gb, WidthTable wt, unsigned ngiants, GiantOut go,
+                                 const uint64_t* best) {
+  const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ngiants) return;
+  const unsigned k = 0xffffffffu - (unsigned)(best[g] & 0xffffffffull);
+  const uint64_t key = gb.okey[k];
+  const WidthInfo wi = wt.wi[(unsigned)((key >> 32) % (uint64_t)wt.nw)];
+  go.peak[g] = gb.oval[k];
+  go.ind[g] = ((key & 0xffffffffull) + wi.rel_off) * (uint64_t)wi.ts;
+}
+
+__global__ void k_zero_u64(uint64_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 }  // namespace
@@ -391,8 +405,9 @@ hipError_t launch_threshold(const float* S, size_t spitch, const SeriesShape& sh
     amax = wt.wi[j].ahead > amax ? wt.wi[j].ahead : amax;
     bmax = wt.wi[j].behind > bmax ? wt.wi[j].behind : bmax;
   }
-  const size_t lds = ((size_t)TH_CHUNK + amax + bmax) * sizeof(float);
-  if (lds > 65536) return hipErrorInvalidValue;  // boxcar_max beyond 12288
+  const size_t tlen = (size_t)TH_CHUNK + amax + bmax;
+  const size_t lds = (tlen + tlen / 16 + 1) * sizeof(float);
+  if (lds > 65536) return hipErrorInvalidValue;  // boxcar_max beyond ~11000
   dim3 grid((unsigned)gb.nchunks_per_w, n_dm);
   hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), lds, s, S, spitch, sh.stride, bc692, wt,
                      amax, bmax, sh.nout + 1, winv, thresh, gb);
@@ -438,10 +453,14 @@ hipError_t launch_segment_heads(const GiantBuffers& gb, const WidthTable& wt, un
 }
 
 hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
-                                 GiantOut go, hipStream_t s) {
-  if (!nsamp) return hipSuccess;
-  hipLaunchKernelGGL(k_segment_reduce, dim3((nsamp + 255) / 256), dim3(256), 0, s, gb, wt, nsamp,
-                     go);
+                                 unsigned ngiants, GiantOut go, uint64_t* best, hipStream_t s) {
+  if (!nsamp || !ngiants) return hipSuccess;
+  hipLaunchKernelGGL(k_zero_u64, dim3((ngiants + 255) / 256), dim3(256), 0, s, best,
+                     (size_t)ngiants);
+  hipLaunchKernelGGL(k_segment_scatter, dim3((nsamp + 255) / 256), dim3(256), 0, s, gb, wt,
+                     nsamp, go, best);
+  hipLaunchKernelGGL(k_segment_finish, dim3((ngiants + 255) / 256), dim3(256), 0, s, gb, wt,
+                     ngiants, go, best);
   return hipGetLastError();
 }
 

@@ -421,7 +421,9 @@ extern "C" hd_error hd_find_giants(const hd_float* d_data, hd_size count, hd_siz
   ST_CHECK(tmp.get(&go.end, ng));
   ST_CHECK(tmp.get(&go.filter, ng));
   ST_CHECK(tmp.get(&go.dm, ng));
-  ST_CHECK(launch_segment_reduce(gb, wt, nsamp, go, s));
+  uint64_t* best;
+  ST_CHECK(tmp.get(&best, ng));
+  ST_CHECK(launch_segment_reduce(gb, wt, nsamp, (unsigned)ng, go, best, s));
   const size_t k = std::min<size_t>(ng, out->capacity);
   if (k) {
     if (out->peaks) ST_CHECK(hipMemcpyAsync(out->peaks, go.peak, k * 4, hipMemcpyDeviceToDevice, s));
