@@ -32,9 +32,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="full_1024ch")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-stride", type=int, default=8,
+    ap.add_argument("--cpu-stride", type=int, default=4,
                     help="CPU baseline: time every k-th DM trial of the same gulp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="before the gulps, run one device-to-device copy of this many MiB "
+                         "(the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -100,6 +103,14 @@ def main():
     dbuf = hip.DeviceBuffer(fb.nbytes)
     dbuf.upload(fb)
     pl.preallocate(w.nsamps, 1)
+    if a.calibrate_copy_mib > 0:
+        nb = a.calibrate_copy_mib << 20
+        src, dst = hip.DeviceBuffer(nb), hip.DeviceBuffer(nb)
+        hip.copy_device(dst.address, src.address, nb)
+        hip.synchronize()
+        src.free()
+        dst.free()
+        print("calibration copy: %d bytes" % nb, file=sys.stderr, flush=True)
     print("n_dm=%d max_delay=%d beam_s/gulp=%.3f" % (n_dm, max_delay, beam_s), file=sys.stderr,
           flush=True)
     for i in range(a.warmup):
@@ -144,7 +155,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u8 samples, exact u32 sums, f32 search",
         "data": "synthetic (seeded Gaussian noise N(128,16^2) + 50 dispersed pulses + "
-                "5%% tone channels + 3 zero-DM bursts)",
+                "5% tone channels + 3 zero-DM bursts)",
         "config": {"workload": "%s: 1 beam, %d ch x %d-bit, fch1 %.4f MHz, foff %.6f MHz, "
                                "tsamp %g us, gulp %d, DM %g-%g (%d trials, max delay %d), "
                                "boxcars 1..%d, threshold %g sigma"

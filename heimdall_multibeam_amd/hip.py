@@ -70,5 +70,10 @@ class DeviceBuffer:
         return self.ptr.value
 
 
+def copy_device(dst, src, nbytes):
+    """Device-to-device copy (one runtime copy kernel of nbytes)."""
+    check(hip().hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, D2D), "hipMemcpy D2D")
+
+
 def synchronize():
     check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
