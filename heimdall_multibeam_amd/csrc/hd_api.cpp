@@ -601,23 +601,21 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pl->chunk_pairs.as<int>(), g,
                                  pl->plane.as<float>(), s));
   record(pl, 2);
-  // baseline points, per-beam RMS, normalise
+  // baseline points and per-beam RMS (normalisation is fused into the scan)
   HD_CHECK(pl->pts.ensure(((size_t)n_dm * nbeams * std::max<size_t>(sh.sc, 1) + 64) * sizeof(float)));
   HD_CHECK(pl->rms.ensure((size_t)n_dm * nbeams * sizeof(float)));
   HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh, n_dm, pl->pts.as<float>(), s));
   record(pl, 3);
   HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh, n_dm,
                                pl->rms.as<float>(), s));
-  HD_CHECK(hd::launch_normalise(pl->plane.as<float>(), pl->pts.as<float>(), pl->rms.as<float>(),
-                                sh, n_dm, s));
   record(pl, 4);
-  // prefix sums
+  // normalised prefix sums
   // prefix-sum rows start 3 floats into the allocation so that S[1..] is
   // 16-byte aligned for the scan's vector stores
   HD_CHECK(pl->S.ensure(((size_t)n_dm * sh.spitch + 8) * sizeof(float)));
   float* const Sbase = pl->S.as<float>() + 3;
   HD_CHECK(hd::launch_scan(pl->plane.as<float>(), sh.ppitch, sh.nout, n_dm, Sbase,
-                           sh.spitch, s));
+                           sh.spitch, s, pl->pts.as<float>(), pl->rms.as<float>(), &sh));
   record(pl, 5);
   // block RMS: the ts == 1 widths together, then each decimated width in order
   const BlockRmsShape& bs = cs.bs;

@@ -92,8 +92,11 @@ hipError_t launch_beam_rms(const float* plane, const float* pts, const SeriesSha
                            int n_dm, float* rms, hipStream_t s);
 hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
                             const SeriesShape& sh, int n_dm, hipStream_t s);
+// Prefix sums of n_series rows; with rms (and pts, sh) the rows are first
+// normalised as launch_normalise would.
 hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series, float* S,
-                       size_t spitch, hipStream_t s);
+                       size_t spitch, hipStream_t s, const float* pts = nullptr,
+                       const float* rms = nullptr, const SeriesShape* sh = nullptr);
 
 hipError_t launch_block_rms_partials(const float* S, const SeriesShape& sh, const WidthTable& wt,
                                      const BlockRmsShape& bs, int first_w, int n_w_group,

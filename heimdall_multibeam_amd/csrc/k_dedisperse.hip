@@ -11,6 +11,8 @@
 // Dedispersion stages, per block of DM trials and channel, the byte window
 // [t0 + d_min, t0 + d_max + tile) into LDS (delays are monotone in DM), so
 // each input byte is read from L2 once per block and many times from LDS.
+#include <cstdlib>
+
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
 
@@ -76,16 +78,27 @@ constexpr int D2_PF = 6;                  // staging items prefetched per thread
 __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restrict__ x,
                                                        const int* __restrict__ delays,
                                                        const int* __restrict__ chunk_pairs,
-                                                       DedispGeom g, int hw_max,
+                                                       DedispGeom g, int hw_max, int xcd_remap,
                                                        float* __restrict__ out) {
   extern __shared__ uint32_t lds16[];  // [D2_CC][2][hw_max]
   __shared__ int sdly[D2_DB][D2_CC];
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int mb = blockIdx.x;
+  // Workgroups are dispatched round-robin over the 8 XCDs in linear order.
+  // With the remap, XCD x takes the contiguous run [start_x, start_x + n_x)
+  // of the (trial-block fastest) tile order, so the trial blocks it runs
+  // together are neighbours whose input windows overlap in its own L2.
+  const unsigned nbx = gridDim.x;
+  unsigned lin = blockIdx.y * nbx + blockIdx.x;
+  if (xcd_remap) {
+    const unsigned nblk = nbx * gridDim.y;
+    const unsigned q = nblk / 8, r = nblk % 8, x = lin % 8;
+    lin = x * q + min(x, r) + lin / 8;
+  }
+  const int mb = (int)(lin % nbx);
   const int m0 = mb * D2_DB;
-  const size_t t0 = (size_t)blockIdx.y * D2_TB;
+  const size_t t0 = (size_t)(lin / nbx) * D2_TB;
   const int* drow0 = delays + (size_t)m0 * g.n_live;
 
   uint32_t pa[2][D2_Q];
@@ -240,8 +253,12 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
+  static const int remap = [] {
+    const char* e = getenv("HD_DEDISP_XCD_REMAP");
+    return e ? atoi(e) : 1;
+  }();
   hipLaunchKernelGGL(k_dedisperse16, grid, dim3(D2_BS), lds, s, x, d_delays, d_chunk_pairs, g,
-                     g.wlen, plane);
+                     g.wlen, remap, plane);
   return hipGetLastError();
 }
 

@@ -9,11 +9,11 @@
 //   beam rms   GetRMSPlanMB::exec_multibeam per-beam form (get_rms.cu:66-100):
 //              median-of-5 pyramid of |x - baseline| per (beam, DM) block.
 //   normalise  normalise_multibeam (get_rms.cu:202-212): z = (x-baseline)/rms,
-//              written in place over the DM plane.
+//              in place (stage entry point), or fused into the scan's loads.
 //   scan       MatchedFilterPlan::prep (matched_filter.cu:21-32): the exclusive
 //              prefix sum in the reference's sequential order, one lane per
-//              series, 64x64 tiles transposed through LDS so that HBM reads
-//              and writes stay coalesced.
+//              series over row tiles staged through LDS so that HBM reads and
+//              writes stay coalesced.
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
 
@@ -145,50 +145,143 @@ __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
 // A block owns SC_ROWS series.  Wave 0 runs the dependent adds (one lane per
 // series) over a SC_T-sample tile held in LDS while waves 1..8 store the
 // previous tile's prefix sums and load the next tile (double buffer), so the
-// add chain never waits on HBM latency.
-constexpr int SC_ROWS = 16;
+// add chain never waits on HBM latency.  The scan lane works through its row
+// in batches of 32 values held in registers, loading the next batch before
+// it adds the current one, so LDS latency stays off the add chain too.
+//
+// When rms is given the loader normalises on the fly, z = (x - baseline)/rms
+// exactly as k_normalise does, so the pipeline never writes the normalised
+// plane back to HBM.
+constexpr int SC_ROWS = 8;
 constexpr int SC_T = 1024;
 constexpr int SC_PAD = 4;
-constexpr int SC_MOVERS = 512;                  // waves 1..8
+constexpr int SC_MOVERS = 256;                  // waves 1..4
 constexpr int SC_THREADS = 64 + SC_MOVERS;
 constexpr int SC_VEC = SC_ROWS * SC_T / 4;       // float4 per tile
 constexpr int SC_PER = SC_VEC / SC_MOVERS;       // float4 per mover thread
+constexpr int SC_B4 = 8;                         // float4 per scan batch
 
+__device__ __forceinline__ void scan4(float4& v, float& run) {
+  run = run + v.x;
+  v.x = run;
+  run = run + v.y;
+  v.y = run;
+  run = run + v.z;
+  v.z = run;
+  run = run + v.w;
+  v.w = run;
+}
+
+constexpr int SC_PMAX = 4096;                    // staged baseline points per block
+constexpr int SC_RMAX = 1024;                    // staged beam RMS values per block
+
+// STAGED: the block's baseline points and beam RMS values live in LDS (the
+// normalising loader reads them for every sample); otherwise they are read
+// from global memory.
+template <bool STAGED>
 __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z, size_t zpitch,
                                                      size_t count, int n_series,
                                                      float* __restrict__ S, size_t spitch,
-                                                     int s_vec_ok) {
+                                                     int s_vec_ok, const float* __restrict__ pts,
+                                                     const float* __restrict__ rms,
+                                                     SeriesShape sh) {
   __shared__ float buf[2][SC_ROWS][SC_T + SC_PAD];
+  __shared__ float sP[STAGED ? SC_PMAX + 1 : 1];
+  __shared__ float sR[STAGED ? SC_RMAX : 1];
   const int tid = threadIdx.x;
   const int r0 = blockIdx.x * SC_ROWS;
   const size_t ntiles = (count + SC_T - 1) / SC_T;
   const bool z_vec = (zpitch % 4 == 0) && ((reinterpret_cast<uintptr_t>(z) & 15) == 0);
+  const size_t prow = (size_t)sh.nbeams * sh.sc;  // baseline points per series
+  if (STAGED) {
+    // rows r0.. of pts (plus the float after them, which the stretch may
+    // touch at the very end, as it would in global memory) and of rms
+    const int nrow = min(SC_ROWS, n_series - r0);
+    const size_t np = (size_t)nrow * prow + 1;
+    for (size_t i = tid; i < np; i += SC_THREADS) sP[i] = pts[(size_t)r0 * prow + i];
+    for (size_t i = tid; i < (size_t)nrow * sh.nbeams; i += SC_THREADS)
+      sR[i] = rms[(size_t)r0 * sh.nbeams + i];
+  }
 
+  auto norm = [&](int row, size_t t, float x) -> float {
+    const float* P = STAGED ? sP + (size_t)row * prow : pts + (size_t)(r0 + row) * prow;
+    const float* R = STAGED ? sR + (size_t)row * sh.nbeams : rms + (size_t)(r0 + row) * sh.nbeams;
+    if (sh.nbeams == 1) {
+      // one beam: every position is in beam 0, stretched over bc samples
+      const float bl = (sh.sc == 0) ? 0.f : Stretch{sh.step_bl_last}(P, (unsigned)t);
+      return (x - bl) / R[0];
+    }
+    const float y = x - baseline_at(P, sh, t);
+    return y / R[(unsigned)((unsigned)t / sh.stride)];
+  };
   auto load_tile = [&](size_t tile, int b) {
     const int mt = tid - 64;
+    const bool full = z_vec && (tile + 1) * SC_T <= count;
+    float4 v[SC_PER];
+    // all global loads first, so that they are in flight together (rows past
+    // the last series load the last series and are zeroed afterwards)
+    if (full) {
+#pragma unroll
+      for (int q = 0; q < SC_PER; ++q) {
+        const int idx = mt + SC_MOVERS * q;
+        const int row = idx / (SC_T / 4);
+        const int c4 = (idx % (SC_T / 4)) * 4;
+        const int rr = min(r0 + row, n_series - 1);
+        v[q] = *reinterpret_cast<const float4*>(z + (size_t)rr * zpitch + tile * SC_T + c4);
+      }
+#pragma unroll
+      for (int q = 0; q < SC_PER; ++q) {
+        const int row = (mt + SC_MOVERS * q) / (SC_T / 4);
+        if (r0 + row >= n_series) v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < SC_PER; ++q) {
+        const int idx = mt + SC_MOVERS * q;
+        const int row = idx / (SC_T / 4);
+        const int c4 = (idx % (SC_T / 4)) * 4;
+        const size_t t = tile * SC_T + c4;
+        v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (r0 + row < n_series) {
+          const float* src = z + (size_t)(r0 + row) * zpitch + t;
+          if (t + 0 < count) v[q].x = src[0];
+          if (t + 1 < count) v[q].y = src[1];
+          if (t + 2 < count) v[q].z = src[2];
+          if (t + 3 < count) v[q].w = src[3];
+        }
+      }
+    }
+    if (rms) {
+#pragma unroll
+      for (int q = 0; q < SC_PER; ++q) {
+        const int idx = mt + SC_MOVERS * q;
+        const int row = idx / (SC_T / 4);
+        const int c4 = (idx % (SC_T / 4)) * 4;
+        const size_t t = tile * SC_T + c4;
+        if (r0 + row < n_series) {
+          if (t + 0 < count) v[q].x = norm(row, t + 0, v[q].x);
+          if (t + 1 < count) v[q].y = norm(row, t + 1, v[q].y);
+          if (t + 2 < count) v[q].z = norm(row, t + 2, v[q].z);
+          if (t + 3 < count) v[q].w = norm(row, t + 3, v[q].w);
+        }
+      }
+    }
 #pragma unroll
     for (int q = 0; q < SC_PER; ++q) {
       const int idx = mt + SC_MOVERS * q;
       const int row = idx / (SC_T / 4);
       const int c4 = (idx % (SC_T / 4)) * 4;
-      const size_t t = tile * SC_T + c4;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (r0 + row < n_series) {
-        const float* src = z + (size_t)(r0 + row) * zpitch + t;
-        if (z_vec && t + 3 < count) {
-          v = *reinterpret_cast<const float4*>(src);
-        } else {
-          if (t + 0 < count) v.x = src[0];
-          if (t + 1 < count) v.y = src[1];
-          if (t + 2 < count) v.z = src[2];
-          if (t + 3 < count) v.w = src[3];
-        }
-      }
-      *reinterpret_cast<float4*>(&buf[b][row][c4]) = v;
+      *reinterpret_cast<float4*>(&buf[b][row][c4]) = v[q];
     }
   };
   auto store_tile = [&](size_t tile, int b) {
     const int mt = tid - 64;
+    float4 v[SC_PER];
+#pragma unroll
+    for (int q = 0; q < SC_PER; ++q) {
+      const int idx = mt + SC_MOVERS * q;
+      v[q] = *reinterpret_cast<const float4*>(&buf[b][idx / (SC_T / 4)][(idx % (SC_T / 4)) * 4]);
+    }
 #pragma unroll
     for (int q = 0; q < SC_PER; ++q) {
       const int idx = mt + SC_MOVERS * q;
@@ -196,20 +289,20 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
       const int c4 = (idx % (SC_T / 4)) * 4;
       const size_t t = tile * SC_T + c4;
       if (r0 + row >= n_series) continue;
-      const float4 v = *reinterpret_cast<const float4*>(&buf[b][row][c4]);
       float* dst = S + (size_t)(r0 + row) * spitch + t + 1;
       if (s_vec_ok && t + 3 < count) {
-        *reinterpret_cast<float4*>(dst) = v;
+        *reinterpret_cast<float4*>(dst) = v[q];
       } else {
-        if (t + 0 < count) dst[0] = v.x;
-        if (t + 1 < count) dst[1] = v.y;
-        if (t + 2 < count) dst[2] = v.z;
-        if (t + 3 < count) dst[3] = v.w;
+        if (t + 0 < count) dst[0] = v[q].x;
+        if (t + 1 < count) dst[1] = v[q].y;
+        if (t + 2 < count) dst[2] = v[q].z;
+        if (t + 3 < count) dst[3] = v[q].w;
       }
     }
   };
 
   if (tid < SC_ROWS && r0 + tid < n_series) S[(size_t)(r0 + tid) * spitch] = 0.f;
+  if (STAGED) __syncthreads();
   if (tid >= 64 && ntiles > 0) load_tile(0, 0);
   __syncthreads();
   float run = 0.f;
@@ -220,18 +313,35 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
         const size_t rem = count - p * SC_T;
         const int ncol = rem < (size_t)SC_T ? (int)rem : SC_T;
         float* rowp = &buf[b][tid][0];
+        float4* row4 = reinterpret_cast<float4*>(rowp);
         int c = 0;
+        if (ncol == SC_T) {
+          // two register batches in flight: load B, add A, store A, load A', add B ...
+          float4 va[SC_B4], vb[SC_B4];
+#pragma unroll
+          for (int i = 0; i < SC_B4; ++i) va[i] = row4[i];
+          for (int c4 = 0; c4 < SC_T / 4; c4 += 2 * SC_B4) {
+#pragma unroll
+            for (int i = 0; i < SC_B4; ++i) vb[i] = row4[c4 + SC_B4 + i];
+#pragma unroll
+            for (int i = 0; i < SC_B4; ++i) scan4(va[i], run);
+#pragma unroll
+            for (int i = 0; i < SC_B4; ++i) row4[c4 + i] = va[i];
+            if (c4 + 2 * SC_B4 < SC_T / 4) {
+#pragma unroll
+              for (int i = 0; i < SC_B4; ++i) va[i] = row4[c4 + 2 * SC_B4 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < SC_B4; ++i) scan4(vb[i], run);
+#pragma unroll
+            for (int i = 0; i < SC_B4; ++i) row4[c4 + SC_B4 + i] = vb[i];
+          }
+          c = SC_T;
+        }
         for (; c + 4 <= ncol; c += 4) {
-          float4 v = *reinterpret_cast<float4*>(rowp + c);
-          run = run + v.x;
-          v.x = run;
-          run = run + v.y;
-          v.y = run;
-          run = run + v.z;
-          v.z = run;
-          run = run + v.w;
-          v.w = run;
-          *reinterpret_cast<float4*>(rowp + c) = v;
+          float4 v = row4[c / 4];
+          scan4(v, run);
+          row4[c / 4] = v;
         }
         for (; c < ncol; ++c) {
           run = run + rowp[c];
@@ -277,10 +387,20 @@ hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
 }
 
 hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series, float* S,
-                       size_t spitch, hipStream_t s) {
+                       size_t spitch, hipStream_t s, const float* pts, const float* rms,
+                       const SeriesShape* sh) {
   const int s_vec_ok = (spitch % 4 == 0) && (((reinterpret_cast<uintptr_t>(S) + 4) & 15) == 0);
-  hipLaunchKernelGGL(k_scan, dim3((n_series + SC_ROWS - 1) / SC_ROWS), dim3(SC_THREADS), 0, s, z,
-                     zpitch, count, n_series, S, spitch, s_vec_ok);
+  SeriesShape none{};
+  if (rms && !sh) return hipErrorInvalidValue;
+  const dim3 grid((n_series + SC_ROWS - 1) / SC_ROWS);
+  const bool staged = rms && (size_t)SC_ROWS * sh->nbeams * sh->sc + 1 <= (size_t)SC_PMAX &&
+                      (size_t)SC_ROWS * sh->nbeams <= (size_t)SC_RMAX;
+  if (staged)
+    hipLaunchKernelGGL(k_scan<true>, grid, dim3(SC_THREADS), 0, s, z, zpitch, count, n_series, S,
+                       spitch, s_vec_ok, pts, rms, *sh);
+  else
+    hipLaunchKernelGGL(k_scan<false>, grid, dim3(SC_THREADS), 0, s, z, zpitch, count, n_series,
+                       S, spitch, s_vec_ok, pts, rms, rms ? *sh : none);
   return hipGetLastError();
 }
 
