@@ -87,7 +87,7 @@ struct hd_pipeline_t {
   std::vector<float> kc;
   size_t max_delay = 0;
   // device buffers
-  DevBuf fb, x, live, delays, chunk_pairs, plane, pts, rms, S, partial, wrms, winv;
+  DevBuf fb, x, live, delays, chunk_pairs, plane, pts, rms, brms, S, partial, wrms, winv;
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
   DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm, g_best;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
@@ -606,8 +606,12 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   HD_CHECK(pl->rms.ensure((size_t)n_dm * nbeams * sizeof(float)));
   HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh, n_dm, pl->pts.as<float>(), s));
   record(pl, 3);
-  HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh, n_dm,
-                               pl->rms.as<float>(), s));
+  {
+    const size_t words = hd::beam_rms_scratch_words(sh, n_dm);
+    HD_CHECK(pl->brms.ensure(std::max<size_t>(words, 1) * sizeof(float)));
+    HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh, n_dm,
+                                 pl->rms.as<float>(), s, pl->brms.as<float>(), words));
+  }
   record(pl, 4);
   // normalised prefix sums
   // prefix-sum rows start 3 floats into the allocation so that S[1..] is
@@ -923,7 +927,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
   {
     DeviceGuard dg(pl->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    DevBuf* bufs[] = {&pl->fb, &pl->x, &pl->live, &pl->delays, &pl->chunk_pairs, &pl->plane, &pl->pts, &pl->rms,
+    DevBuf* bufs[] = {&pl->fb, &pl->x, &pl->live, &pl->delays, &pl->chunk_pairs, &pl->plane, &pl->pts, &pl->rms, &pl->brms,
                       &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
                       &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->okey, &pl->oval,
                       &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,

@@ -88,8 +88,12 @@ int dedisp_chunk_channels();           // channels staged per step
 
 hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
                                   float* pts, hipStream_t s);
+// Per-beam RMS of every series; with scratch (beam_rms_scratch_words floats)
+// large series take the two-phase kernels.
+size_t beam_rms_scratch_words(const SeriesShape& sh, int n_dm);
 hipError_t launch_beam_rms(const float* plane, const float* pts, const SeriesShape& sh,
-                           int n_dm, float* rms, hipStream_t s);
+                           int n_dm, float* rms, hipStream_t s, float* scratch = nullptr,
+                           size_t scratch_words = 0);
 hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
                             const SeriesShape& sh, int n_dm, hipStream_t s);
 // Prefix sums of n_series rows; with rms (and pts, sh) the rows are first

@@ -74,12 +74,18 @@ __device__ float lds_finish_f(float* A, float* B, size_t n) {
 }
 
 // One block per (chunk, DM): level-L0 medians of |F_w| for widths
-// [first_w, first_w + n_group).  `tile` selects the LDS-staged path, valid
-// when every width of the group has ts == 1.
+// [first_w, first_w + n_group).  `tile_ok` selects the LDS-staged path for a
+// group of ts == 1 widths.  A single decimated width (ts > 1) stages the
+// prefix sums that the chunk's values come from: those of its own decimated
+// boxcar where the chunk is fresh, or of the one earlier width whose
+// normalised output the reference's buffer still holds there.  Either way
+// the staged entries are every ts-th prefix sum, E[x] = S[off - behind +
+// ts*(pos0 + x)], so that F[pos] = E[pos - pos0 + w/ts] - E[pos - pos0]
+// (w = ahead + behind is a multiple of ts); up to tile_words of them.
 __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     const float* __restrict__ Sall, size_t spitch, WidthTable wt, BlockRmsShape bs, int first_w,
-    int n_group, int tile_ok, unsigned amax, unsigned bmax, const float* __restrict__ winv_all,
-    float* __restrict__ partial) {
+    int n_group, int tile_ok, unsigned amax, unsigned bmax, unsigned tile_words,
+    const float* __restrict__ winv_all, float* __restrict__ partial) {
   extern __shared__ float lds[];
   const size_t c = blockIdx.x;
   const int m = blockIdx.y;
@@ -88,10 +94,31 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
   const size_t k0 = c * bs.chunk;
   const size_t pos0 = reblock_pos((unsigned)k0, bs.bc, bs.delta);
   const size_t posl = reblock_pos((unsigned)(k0 + bs.chunk - 1), bs.bc, bs.delta);
-  const bool use_tile = tile_ok && (posl - pos0 == bs.chunk - 1);
-  const size_t tlen = bs.chunk + amax + bmax;
+  const bool contiguous = (posl - pos0 == bs.chunk - 1);
+  const bool use_tile = tile_ok && contiguous;
+  // single decimated width: which width's boxcar this chunk reads
+  int src_j = -1;
+  if (!tile_ok && n_group == 1 && contiguous && tile_words > 0) {
+    const WidthInfo wj = wt.wi[first_w];
+    if (posl < wj.n_w) {
+      src_j = first_w;
+    } else if (pos0 >= wj.n_w) {
+      int a = -1, b = -1;
+      for (int jj = first_w - 1; jj >= 0 && a < 0; --jj)
+        if (pos0 < wt.wi[jj].n_w) a = jj;
+      for (int jj = first_w - 1; jj >= 0 && b < 0; --jj)
+        if (posl < wt.wi[jj].n_w) b = jj;
+      if (a == b) src_j = a;
+    }
+    if (src_j >= 0) {
+      const WidthInfo ws = wt.wi[src_j];
+      const size_t span = bs.chunk + (ws.ahead + ws.behind) / ws.ts;
+      if (span > tile_words || (ws.ahead + ws.behind) % ws.ts != 0) src_j = -1;
+    }
+  }
+  const size_t tlen = use_tile ? bs.chunk + amax + bmax : (src_j >= 0 ? tile_words : 0);
   float* T = lds;
-  float* A = lds + (use_tile ? tlen : 0);
+  float* A = lds + tlen;
   const size_t n1 = bs.chunk / 5;
   float* B = A + (n1 ? n1 : 1);
   if (use_tile) {
@@ -99,29 +126,52 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     for (size_t t = threadIdx.x; t < tlen; t += FB_BS) T[t] = src[t];
     __syncthreads();
   }
+  unsigned wts = 0;
+  float sscale = 1.f;
+  if (src_j >= 0) {
+    const WidthInfo ws = wt.wi[src_j];
+    sscale = winv[src_j];
+    wts = (ws.ahead + ws.behind) / ws.ts;
+    const size_t span = bs.chunk + wts;
+    const float* src = S + wt.off - ws.behind + (size_t)ws.ts * pos0;
+    for (size_t t = threadIdx.x; t < span; t += FB_BS) T[t] = src[t * ws.ts];
+    __syncthreads();
+  }
+  // level-L0 median of the chunk's leaves for one width (each mode below
+  // passes its own branch-free leaf function)
+  auto chunk_value = [&](auto leaf) -> float {
+    if (bs.L0 == 0) return leaf(k0);
+    for (size_t q = threadIdx.x; q < n1; q += FB_BS) {
+      const size_t b = k0 + 5 * q;
+      A[q] = med5(leaf(b), leaf(b + 1), leaf(b + 2), leaf(b + 3), leaf(b + 4));
+    }
+    return lds_pyramid<FB_BS>(A, B, (int)n1, bs.L0 - 1)[0];
+  };
   for (int g = 0; g < n_group; ++g) {
     const int j = first_w + g;
     const WidthInfo wi = wt.wi[j];
-    auto leaf = [&](size_t k) -> float {
-      if (use_tile) {
-        const size_t r = k - k0;  // pos - pos0
-        const float a = T[r + bmax + wi.ahead];
-        const float b = T[r + bmax - wi.behind];
-        return fabsf(a - b);
-      }
-      const size_t pos = reblock_pos((unsigned)k, bs.bc, bs.delta);
-      return fabsf(filtered_at(S, wt, winv, j, pos));
-    };
     float value;
-    if (bs.L0 == 0) {
-      value = leaf(k0);
+    if (use_tile) {
+      const unsigned oa = bmax + wi.ahead, ob = bmax - wi.behind;
+      value = chunk_value([&](size_t k) -> float {
+        const unsigned r = (unsigned)(k - k0);  // pos - pos0
+        return fabsf(T[r + oa] - T[r + ob]);
+      });
+    } else if (src_j == j) {
+      value = chunk_value([&](size_t k) -> float {
+        const unsigned r = (unsigned)(k - k0);
+        return fabsf(T[r + wts] - T[r]);
+      });
+    } else if (src_j >= 0) {
+      value = chunk_value([&](size_t k) -> float {
+        const unsigned r = (unsigned)(k - k0);
+        return fabsf((T[r + wts] - T[r]) * sscale);
+      });
     } else {
-      for (size_t q = threadIdx.x; q < n1; q += FB_BS) {
-        const size_t b = k0 + 5 * q;
-        A[q] = med5(leaf(b), leaf(b + 1), leaf(b + 2), leaf(b + 3), leaf(b + 4));
-      }
-      float* res = lds_pyramid<FB_BS>(A, B, (int)n1, bs.L0 - 1);
-      value = res[0];
+      value = chunk_value([&](size_t k) -> float {
+        const size_t pos = reblock_pos((unsigned)k, bs.bc, bs.delta);
+        return fabsf(filtered_at(S, wt, winv, j, pos));
+      });
     }
     if (threadIdx.x == 0) partial[((size_t)m * wt.nw + j) * bs.np + c] = value;
     __syncthreads();
@@ -206,23 +256,31 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       continue;
     }
     const size_t nout = i1 - i0;
-    const size_t per = (nout + FB_BS - 1) / FB_BS;  // <= TH_PER
+    const unsigned per = (unsigned)((nout + FB_BS - 1) / FB_BS);  // <= TH_PER
     const size_t ib = i0 + (size_t)threadIdx.x * per;
+    // find_giants.cu:55-58 keeps i with (i mod stride) < bc692.  When the
+    // chunk's outputs all lie in one beam period that is a single bound.
+    const size_t beam0 = (size_t)(unsigned)i0 / stride;
+    const bool one_period = (size_t)(unsigned)(i1 - 1) / stride == beam0 &&
+                            i1 - 1 <= 0xffffffffull;
+    const size_t lim = beam0 * stride + bc692;
+    // T indices (32-bit) of position ib*ts: ahead and behind reads
+    const unsigned r0 = (unsigned)(ib * wi.ts - p0 + bmax);
+    const unsigned ra0 = r0 + wi.ahead, rb0 = r0 - wi.behind;
     float vals[TH_PER];
     unsigned mask = 0;
 #pragma unroll
     for (int q = 0; q < TH_PER; ++q) {
       vals[q] = 0.f;
       const size_t i = ib + q;
-      if (q < (int)per && i < i1) {
-        const size_t r = i * wi.ts - p0 + bmax;  // T index of position i*ts
-        const size_t ra = r + wi.ahead, rb = r - wi.behind;
+      if ((unsigned)q < per && i < i1) {
+        const unsigned ra = ra0 + (unsigned)q * wi.ts, rb = rb0 + (unsigned)q * wi.ts;
         const float a = T[ra + (ra >> 4)];
         const float b = T[rb + (rb >> 4)];
         const float v = (a - b) * inv;               // get_rms.cu:194-197
         vals[q] = v;
-        const size_t samp = (size_t)(unsigned)i % stride;  // find_giants.cu:55-58
-        if (samp < bc692 && v > thresh) mask |= 1u << q;
+        const bool in_beam = one_period ? (i < lim) : ((size_t)(unsigned)i % stride < bc692);
+        if (in_beam && v > thresh) mask |= 1u << q;
       }
     }
     const int any = __syncthreads_or(mask != 0);
@@ -378,10 +436,23 @@ hipError_t launch_block_rms_partials(const float* S, const SeriesShape& sh, cons
   }
   const size_t tlen = bs.chunk + amax + bmax;
   const size_t n1 = bs.chunk / 5;
-  if ((tlen + n1 + n1 / 5 + 8) * sizeof(float) > 65536) tile_ok = 0;  // keep LDS <= 64 KiB
-  size_t lds = ((tile_ok ? tlen : 0) + (n1 ? n1 : 1) + n1 / 5 + 8) * sizeof(float);
+  const size_t rest = (n1 ? n1 : 1) + n1 / 5 + 8;
+  constexpr size_t kMaxLds = 65536 / sizeof(float);
+  if (tlen + rest > kMaxLds) tile_ok = 0;  // keep LDS <= 64 KiB
+  // single decimated width: room for the widest range it or an earlier width reads
+  size_t tile_words = 0;
+  if (!tile_ok && n_group == 1 && bs.chunk > 0) {
+    for (int jj = 0; jj <= first_w; ++jj) {
+      const WidthInfo& w = wt.wi[jj];
+      const size_t span = bs.chunk + (w.ahead + w.behind) / w.ts;
+      if (span > tile_words) tile_words = span;
+    }
+    if (tile_words + rest > kMaxLds) tile_words = kMaxLds > rest ? kMaxLds - rest : 0;
+  }
+  size_t lds = ((tile_ok ? tlen : tile_words) + rest) * sizeof(float);
   hipLaunchKernelGGL(k_block_rms_partials, dim3((unsigned)bs.np, n_dm), dim3(FB_BS), lds, s, S,
-                     sh.spitch, wt, bs, first_w, n_group, tile_ok, amax, bmax, winv, partial);
+                     sh.spitch, wt, bs, first_w, n_group, tile_ok, amax, bmax,
+                     (unsigned)tile_words, winv, partial);
   return hipGetLastError();
 }
 

@@ -122,6 +122,66 @@ __global__ __launch_bounds__(NB_BS) void k_beam_rms(const float* __restrict__ pl
   if (threadIdx.x == 0) rms[(size_t)m * sh.nbeams + b] = med * 1.4862f;  // get_rms.cu:94-97
 }
 
+// Two-phase per-beam RMS for large series: blocks of 5^L0 consecutive leaves
+// (staged in LDS with coalesced reads) reduce to one median each; a finishing
+// wave per (beam, DM) runs the remaining levels and the small-count rule.
+// Exact because the first L0 levels of the reference pyramid only combine
+// leaves within aligned 5^L0 groups (bc >= 5^L0 keeps every such level in
+// the groups-of-five regime).
+__device__ __forceinline__ float leaf_abs_dev(const float* d, const float* P,
+                                              const SeriesShape& sh, size_t k) {
+  if (sh.nbeams == 1 && sh.bc <= (1u << 24)) {
+    // one beam with exact float positions: the reblocked index is k itself
+    const float bl = (sh.sc == 0) ? 0.f : Stretch{sh.step_bl_last}(P, (unsigned)k);
+    return fabsf(d[k] - bl);
+  }
+  const size_t pos = reblock_pos((unsigned)k, sh.bc, sh.rms_delta);
+  return fabsf(d[pos] - baseline_at(P, sh, pos));
+}
+
+__global__ __launch_bounds__(NB_BS) void k_beam_rms_partials(const float* __restrict__ plane,
+                                                             const float* __restrict__ pts,
+                                                             SeriesShape sh, int L0, size_t np,
+                                                             int stage_pts,
+                                                             float* __restrict__ partial) {
+  extern __shared__ float lds[];
+  const size_t c = blockIdx.x;
+  const int m = blockIdx.y;
+  const unsigned b = blockIdx.z;
+  const size_t chunk = pow5(L0);
+  const size_t prow = (size_t)sh.nbeams * sh.sc;
+  const float* d = plane + (size_t)m * sh.ppitch;
+  float* A = lds;
+  float* B = lds + chunk;
+  // the row's baseline points (and the float after them) staged in LDS
+  float* sP = B + chunk / 5 + 8;
+  const float* P = pts + (size_t)m * prow;
+  if (stage_pts) {
+    for (size_t i = threadIdx.x; i <= prow; i += NB_BS) sP[i] = P[i];
+    __syncthreads();
+    P = sP;
+  }
+  const size_t k0 = (size_t)b * sh.bc + c * chunk;
+#pragma unroll 4
+  for (size_t i = threadIdx.x; i < chunk; i += NB_BS) A[i] = leaf_abs_dev(d, P, sh, k0 + i);
+  float* res = lds_pyramid<NB_BS>(A, B, (int)chunk, L0);
+  if (threadIdx.x == 0) partial[((size_t)m * sh.nbeams + b) * np + c] = res[0];
+}
+
+__global__ __launch_bounds__(64) void k_beam_rms_finish(const float* __restrict__ partial,
+                                                        size_t np, unsigned nbeams,
+                                                        float* __restrict__ rms) {
+  extern __shared__ float lds[];
+  const unsigned b = blockIdx.x;
+  const int m = blockIdx.y;
+  const float* p = partial + ((size_t)m * nbeams + b) * np;
+  float* A = lds;
+  float* B = lds + np;
+  for (size_t i = threadIdx.x; i < np; i += 64) A[i] = p[i];
+  const float med = lds_finish<64>(A, B, np);
+  if (threadIdx.x == 0) rms[(size_t)m * nbeams + b] = med * 1.4862f;  // get_rms.cu:94-97
+}
+
 // z = (x - baseline) / rms[pos / stride], in place (remove_baseline.cu:99-102,
 // get_rms.cu:183-184).  Position nout (one past the series) is left as is.
 __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
@@ -368,8 +428,30 @@ hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int
   return hipGetLastError();
 }
 
+size_t beam_rms_scratch_words(const SeriesShape& sh, int n_dm) {
+  int L0 = 0;
+  while (L0 < 5 && sh.bc >= pow5_host(L0 + 1)) ++L0;
+  return (size_t)n_dm * sh.nbeams * (sh.bc / pow5_host(L0));
+}
+
 hipError_t launch_beam_rms(const float* plane, const float* pts, const SeriesShape& sh,
-                           int n_dm, float* rms, hipStream_t s) {
+                           int n_dm, float* rms, hipStream_t s, float* scratch,
+                           size_t scratch_words) {
+  int L0 = 0;
+  while (L0 < 5 && sh.bc >= pow5_host(L0 + 1)) ++L0;
+  const size_t np = sh.bc / pow5_host(L0);
+  if (scratch && L0 >= 3 && (size_t)n_dm * sh.nbeams * np <= scratch_words &&
+      (np + np / 5 + 8) * sizeof(float) <= 65536) {
+    const size_t chunk = pow5_host(L0);
+    const size_t prow = (size_t)sh.nbeams * sh.sc;
+    const int stage_pts = (pts && sh.sc > 0 && prow + 1 <= 4096) ? 1 : 0;
+    const size_t lds = (chunk + chunk / 5 + 8 + (stage_pts ? prow + 1 : 0)) * sizeof(float);
+    hipLaunchKernelGGL(k_beam_rms_partials, dim3((unsigned)np, n_dm, sh.nbeams), dim3(NB_BS),
+                       lds, s, plane, pts, sh, L0, np, stage_pts, scratch);
+    hipLaunchKernelGGL(k_beam_rms_finish, dim3(sh.nbeams, n_dm), dim3(64),
+                       (np + np / 5 + 8) * sizeof(float), s, scratch, np, sh.nbeams, rms);
+    return hipGetLastError();
+  }
   const size_t nl = sh.bc / pow5_host(sh.rms_direct);
   const size_t lds = (nl + nl / 5 + 8) * sizeof(float);
   hipLaunchKernelGGL(k_beam_rms, dim3(sh.nbeams, n_dm), dim3(NB_BS), lds, s, plane, pts, sh,
