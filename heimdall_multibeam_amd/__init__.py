@@ -107,6 +107,12 @@ def lib():
     L.hd_get_last_timings.restype = sz
     L.hd_set_output_enabled.argtypes = [vp, C.c_int]
     L.hd_set_output_enabled.restype = None
+    L.hd_set_cleaning.argtypes = [vp, C.c_int]
+    L.hd_set_cleaning.restype = None
+    L.hd_clean_filterbank_rfi.argtypes = [sz, C.c_float, C.c_float, vp, sz, sz, vp,
+                                          C.POINTER(C.c_int), C.c_float, C.c_float, C.c_float,
+                                          C.c_float, sz, sz, sz, vp]
+    L.hd_clean_filterbank_rfi.restype = C.c_int
     L.hd_generate_dm_list.argtypes = [fp, sz, C.c_float, C.c_float, C.c_float, C.c_float,
                                       C.c_float, C.c_float, sz, C.c_float]
     L.hd_generate_dm_list.restype = sz
@@ -180,6 +186,10 @@ class Pipeline:
     def set_output(self, enabled):
         lib().hd_set_output_enabled(self.handle, 1 if enabled else 0)
 
+    def set_cleaning(self, enabled):
+        """Zero-DM RFI cleaning before each search (the reference's CLEANING build)."""
+        lib().hd_set_cleaning(self.handle, 1 if enabled else 0)
+
     def execute(self, filterbank, nsamps, nbits, first_idx=0, nbeams=1, device=False,
                 allow=(HD_NO_ERROR,)):
         """hd_execute on a host array (or a device pointer with device=True).
@@ -241,12 +251,21 @@ class Pipeline:
         return buf.value.decode()
 
     STAGES = ("unpack", "dedisperse", "baseline", "rms", "scan", "block_rms", "giants",
-              "cluster", "total")
+              "cluster", "total", "clean")
 
     def timings(self):
         out = (C.c_double * len(self.STAGES))()
         n = lib().hd_get_last_timings(self.handle, out, len(self.STAGES))
         return dict(zip(self.STAGES[:n], list(out)[:n]))
+
+
+def clean_filterbank_rfi(d_in, d_out, nsamps, nbits, nchans, f0, df, dm, dt, baseline_length,
+                         rfi_tol, boxcar_max, nbeams, stream=None):
+    """hd_clean_filterbank_rfi over device buffers (addresses); raises HdError."""
+    rc = lib().hd_clean_filterbank_rfi(nchans, f0, df, d_in, nsamps, nbits, d_out, None, dm, dt,
+                                       baseline_length, rfi_tol, 0, boxcar_max, nbeams, stream)
+    if rc != HD_NO_ERROR:
+        raise HdError(rc, "hd_clean_filterbank_rfi")
 
 
 def generate_dm_list(dm_min, dm_max, dt, pulse_width_us, f0, df, nchans, tol):

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libheimdall_mi355x.so")
 SYNTH = os.path.join(HERE, "libhd_synth.so")
 
 HIP_SOURCES = ["hd_api.cpp", "k_dedisperse.hip", "k_normalise.hip", "k_filter.hip",
-               "k_cluster.hip", "k_stages.hip"]
+               "k_cluster.hip", "k_stages.hip", "k_clean.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

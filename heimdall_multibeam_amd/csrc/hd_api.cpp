@@ -71,7 +71,7 @@ unsigned log2_pow2(unsigned v) {  // get_filter_index (pipeline.cu:126-136)
 }
 
 enum Stage { ST_UNPACK, ST_DEDISP, ST_BASELINE, ST_RMS, ST_SCAN, ST_BLOCKRMS, ST_GIANTS,
-             ST_CLUSTER, ST_TOTAL, ST_COUNT };
+             ST_CLUSTER, ST_TOTAL, ST_CLEAN, ST_COUNT };
 
 }  // namespace
 
@@ -82,6 +82,7 @@ struct hd_pipeline_t {
   int device = 0;
   hipStream_t stream = nullptr;
   bool output_enabled = true;
+  bool cleaning = false;  // zero-DM RFI cleaning first (the reference's CLEANING build)
   // plan
   std::vector<float> dm_list;
   std::vector<float> kc;
@@ -92,6 +93,7 @@ struct hd_pipeline_t {
   DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm, g_best;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
   DevBuf o_peak, o_ind, o_begin, o_end, o_filter, o_dm, o_members;
+  DevBuf clean_out, clean_work;
   unsigned* h_pinned = nullptr;  // small pinned staging word(s)
   size_t sample_cap = 1u << 22;
   // delay table state for the current live-channel set
@@ -291,6 +293,48 @@ extern "C" hd_size hd_generate_dm_list(hd_float* out, hd_size cap, hd_float dm_m
 }
 
 // ---------------------------------------------------------------------------
+namespace hd {
+
+// Baseline and per-beam RMS geometry of a series whose nbeams, stride, nout
+// and bc are set (pipeline.cu:572-573, remove_baseline.cu:36-46,
+// get_rms.cu:66-100).
+int series_shape(float cur_dt, float baseline_length, SeriesShape& sh) {
+  const size_t smooth = (size_t)(baseline_length / (2 * cur_dt));
+  const float oversample = 2;
+  sh.sc = (size_t)((double)(oversample * (float)sh.stride / (float)(2 * smooth)) + 0.5);
+  if (sh.sc) {
+    sh.nscr = (int)(size_t)(std::log((double)(sh.stride / sh.sc)) / std::log(5.));
+    sh.count_round = (size_t)(std::pow(5., (double)sh.nscr) * (double)sh.sc);
+    if (sh.count_round < 2) return HD_INVALID_STRIDE;
+    int Ld = 0;
+    while (Ld < std::min(sh.nscr, 3) && sh.count_round / pow5_host(Ld) > 12288) ++Ld;
+    if (sh.count_round / pow5_host(Ld) > 12288) return HD_INVALID_STRIDE;
+    sh.bl_direct = Ld;
+    sh.step_rs_beam = (float)(sh.stride - 1) / (float)sh.count_round;
+    sh.step_rs_last = (float)(sh.bc - 1) / (float)(sh.count_round - 1);
+    sh.step_bl_beam = (float)(sh.sc - 1) / (float)sh.stride;
+    sh.step_bl_last = (float)(sh.sc - 1) / (float)(sh.bc - 1);
+    // The last beam's stretch back could read one point past the pyramid
+    // (median_filter.cu:462 with j+1 == sample_count), i.e. a stale scrunch
+    // buffer entry.  That case is refused rather than guessed.
+    for (size_t i = sh.bc; i-- > 0;) {
+      const float x = (float)(unsigned)i * sh.step_bl_last;
+      const unsigned j = (unsigned)x;
+      const float fr = x - (float)j;
+      if ((size_t)j + 1 < sh.sc) break;
+      if (fr > 1e-5f) return HD_INVALID_STRIDE;
+    }
+  }
+  int Ld = 0;
+  while (sh.bc / pow5_host(Ld) > 12288 && sh.bc >= pow5_host(Ld + 1) && Ld < 3) ++Ld;
+  if (sh.bc / pow5_host(Ld) > 12288) return HD_INVALID_STRIDE;
+  sh.rms_direct = Ld;
+  sh.rms_delta = (float)(sh.stride - sh.bc);
+  return HD_NO_ERROR;
+}
+
+}  // namespace hd
+
 namespace {
 
 hd_error gpu_err(hipError_t e) {
@@ -390,40 +434,9 @@ hd_error make_shapes(hd_pipeline pl, size_t nsamps, size_t nbeams, CallShape& cs
   const int TB = hd::dedisp_tile_time();
   sh.ppitch = round_up(sh.nout + 1, (size_t)TB);
   sh.spitch = round_up(sh.nout + 1, 64);
-  // baseline (pipeline.cu:572-573 then remove_baseline.cu:36-46)
-  const float cur_dt = p.dt * 1.0f;
-  const size_t smooth = (size_t)(p.baseline_length / (2 * cur_dt));
-  const float oversample = 2;
-  sh.sc = (size_t)((double)(oversample * (float)sh.stride / (float)(2 * smooth)) + 0.5);
-  if (sh.sc) {
-    sh.nscr = (int)(size_t)(std::log((double)(sh.stride / sh.sc)) / std::log(5.));
-    sh.count_round = (size_t)(std::pow(5., (double)sh.nscr) * (double)sh.sc);
-    if (sh.count_round < 2) return HD_INVALID_STRIDE;
-    int Ld = 0;
-    while (Ld < std::min(sh.nscr, 3) && sh.count_round / hd::pow5_host(Ld) > 12288) ++Ld;
-    if (sh.count_round / hd::pow5_host(Ld) > 12288) return HD_INVALID_STRIDE;
-    sh.bl_direct = Ld;
-    sh.step_rs_beam = (float)(sh.stride - 1) / (float)sh.count_round;
-    sh.step_rs_last = (float)(sh.bc - 1) / (float)(sh.count_round - 1);
-    sh.step_bl_beam = (float)(sh.sc - 1) / (float)sh.stride;
-    sh.step_bl_last = (float)(sh.sc - 1) / (float)(sh.bc - 1);
-    // The last beam's stretch back could read one point past the pyramid
-    // (median_filter.cu:462 with j+1 == sample_count), i.e. a stale scrunch
-    // buffer entry.  That case is refused rather than guessed.
-    for (size_t i = sh.bc; i-- > 0;) {
-      const float x = (float)(unsigned)i * sh.step_bl_last;
-      const unsigned j = (unsigned)x;
-      const float fr = x - (float)j;
-      if ((size_t)j + 1 < sh.sc) break;
-      if (fr > 1e-5f) return HD_INVALID_STRIDE;
-    }
-  }
   {
-    int Ld = 0;
-    while (sh.bc / hd::pow5_host(Ld) > 12288 && sh.bc >= hd::pow5_host(Ld + 1) && Ld < 3) ++Ld;
-    if (sh.bc / hd::pow5_host(Ld) > 12288) return HD_INVALID_STRIDE;
-    sh.rms_direct = Ld;
-    sh.rms_delta = (float)(sh.stride - sh.bc);
+    const int e = hd::series_shape(p.dt * 1.0f, p.baseline_length, sh);
+    if (e != HD_NO_ERROR) return (hd_error)e;
   }
   // widths (pipeline.cu:657-692)
   WidthTable& wt = cs.wt;
@@ -587,6 +600,19 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   const int n_live = (int)pl->live_list.size();
   const int TB = hd::dedisp_tile_time();
 
+  record(pl, 9);
+  if (pl->cleaning) {
+    // pipeline.cu:263-299: clean at DM 0 with boxcars up to 64, then search
+    // the cleaned copy (:447-451)
+    hd::CleanPlan cp;
+    const int ce = hd::clean_plan(nsamps, nbits, p.nchans, nbeams, 0.f, p.dt, p.f0, p.df,
+                                  p.baseline_length, p.rfi_tol, 64, cp);
+    if (ce != HD_NO_ERROR) return (hd_error)ce;
+    HD_CHECK(pl->clean_out.ensure(nsamps * p.nchans * nbits / 8));
+    HD_CHECK(pl->clean_work.ensure(cp.work_bytes));
+    HD_CHECK(hd::run_clean(cp, d_fb, pl->clean_out.as<uint8_t>(), pl->clean_work.p, s));
+    d_fb = pl->clean_out.as<hd_byte>();
+  }
   record(pl, 0);
   // unpack to channel-major bytes
   const size_t xpitch = round_up(sh.ppitch + max_delay + 2 * pl->wlen + 256, 256);
@@ -792,7 +818,7 @@ hd_error finish_call(hd_pipeline pl, hd_error err, size_t nsamps, size_t nbeams,
   if (err != HD_NO_ERROR) return err;
   (void)hipEventSynchronize(pl->ev[8]);
   static const int stage_ev[ST_COUNT][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5},
-                                            {5, 6}, {6, 7}, {7, 8}, {0, 8}};
+                                            {5, 6}, {6, 7}, {7, 8}, {9, 8}, {9, 0}};
   for (int k = 0; k < ST_COUNT; ++k) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, pl->ev[stage_ev[k][0]], pl->ev[stage_ev[k][1]]) == hipSuccess)
@@ -806,7 +832,8 @@ hd_error finish_call(hd_pipeline pl, hd_error err, size_t nsamps, size_t nbeams,
   if (e != HD_NO_ERROR) return e;
   if (pl->params.verbosity >= 1 && pl->output_enabled) {
     const double* t = pl->timings;
-    std::cout << "Unpack time:             " << t[ST_UNPACK] * 1e-3 << "\n"
+    std::cout << "0-DM cleaning time:      " << t[ST_CLEAN] * 1e-3 << "\n"
+              << "Unpack time:             " << t[ST_UNPACK] * 1e-3 << "\n"
               << "Dedispersion time:       " << t[ST_DEDISP] * 1e-3 << "\n"
               << "Baselining time:         " << t[ST_BASELINE] * 1e-3 << "\n"
               << "Normalisation time:      " << t[ST_RMS] * 1e-3 << "\n"
@@ -927,7 +954,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
   {
     DeviceGuard dg(pl->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    DevBuf* bufs[] = {&pl->fb, &pl->x, &pl->live, &pl->delays, &pl->chunk_pairs, &pl->plane, &pl->pts, &pl->rms, &pl->brms,
+    DevBuf* bufs[] = {&pl->clean_out, &pl->clean_work, &pl->fb, &pl->x, &pl->live, &pl->delays, &pl->chunk_pairs, &pl->plane, &pl->pts, &pl->rms, &pl->brms,
                       &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
                       &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->okey, &pl->oval,
                       &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,
@@ -1023,4 +1050,7 @@ extern "C" hd_size hd_get_last_timings(hd_pipeline pl, double* ms, hd_size cap) 
 }
 extern "C" void hd_set_output_enabled(hd_pipeline pl, int enabled) {
   if (pl) pl->output_enabled = enabled != 0;
+}
+extern "C" void hd_set_cleaning(hd_pipeline pl, int enabled) {
+  if (pl) pl->cleaning = enabled != 0;
 }

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace hd {
 
 // Geometry of one gulp's DM series (one hd_execute call, scrunch 1).
@@ -66,6 +68,28 @@ struct BlockRmsShape {
   size_t chunk;        // 5^L0
   size_t np;           // n / 5^L0 partial values
 };
+
+// Fills the baseline and per-beam RMS fields of sh (nbeams, stride, nout and
+// bc set by the caller); returns an hd_error code.
+int series_shape(float cur_dt, float baseline_length, SeriesShape& sh);
+
+// Zero-DM RFI cleaning (clean_filterbank_rfi.cu:256-528): geometry and the
+// byte offsets of its device workspace.
+struct CleanPlan {
+  size_t nsamps, nbits, nchans, stride_w;  // stride_w: 32-bit words per time sample
+  float dm, rfi_tol;
+  size_t boxcar_max, max_delay, nsmooth;
+  SeriesShape sh;                          // the cleaning series (nout = nsamps - max_delay)
+  std::vector<int> delays;                 // per-channel delays at dm
+  size_t brms_words;
+  size_t off_series, off_S, off_pts, off_rms, off_brms, off_mask, off_delays, work_bytes;
+};
+int clean_plan(size_t nsamps, size_t nbits, size_t nchans, size_t nbeams, float dm, float dt,
+               float f0, float df, float baseline_length, float rfi_tol, size_t boxcar_max,
+               CleanPlan& cp);
+// Writes the first nsamps - max_delay time samples of d_out (d_out != d_in).
+hipError_t run_clean(const CleanPlan& cp, const uint8_t* d_in, uint8_t* d_out, void* d_work,
+                     hipStream_t s);
 
 // ---- launchers (return hipError_t of the launch) ----
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,

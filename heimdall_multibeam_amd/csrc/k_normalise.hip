@@ -16,6 +16,7 @@
 //              writes stay coalesced.
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
+#include "hd_series.hpp"
 
 namespace hd {
 
@@ -51,20 +52,6 @@ __device__ float lds_finish(float* A, float* B, size_t n) {
   if (threadIdx.x == 0) result = (n >= 1) ? med_tail(A, (int)n) : 0.f;
   __syncthreads();
   return result;
-}
-
-// The stretched baseline at series position pos (remove_baseline.cu:88-90):
-// beams 0..nb-2 through linear_stretch_beam over their full stride, the last
-// beam through linear_stretch over bc samples.
-__device__ __forceinline__ float baseline_at(const float* P, const SeriesShape& sh, size_t pos) {
-  if (sh.sc == 0) return 0.f;
-  const size_t split = (size_t)(sh.nbeams - 1) * sh.stride;
-  if (pos < split) {
-    StretchBeam f{sh.step_bl_beam, (unsigned)sh.stride};
-    return f(P, (unsigned)pos);
-  }
-  Stretch f{sh.step_bl_last};
-  return f(P + (size_t)(sh.nbeams - 1) * sh.sc, (unsigned)(pos - split));
 }
 
 __global__ __launch_bounds__(NB_BS) void k_baseline_points(const float* __restrict__ plane,
@@ -194,9 +181,7 @@ __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
   const float* R = rms + (size_t)m * sh.nbeams;
   for (size_t pos = (size_t)blockIdx.x * 256 + threadIdx.x; pos < sh.nout;
        pos += (size_t)gridDim.x * 256) {
-    const float y = d[pos] - baseline_at(P, sh, pos);
-    const unsigned beam = (unsigned)((unsigned)pos / sh.stride);
-    d[pos] = y / R[beam];
+    d[pos] = normalised_at(d[pos], P, R, sh, pos);
   }
 }
 
@@ -266,13 +251,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
   auto norm = [&](int row, size_t t, float x) -> float {
     const float* P = STAGED ? sP + (size_t)row * prow : pts + (size_t)(r0 + row) * prow;
     const float* R = STAGED ? sR + (size_t)row * sh.nbeams : rms + (size_t)(r0 + row) * sh.nbeams;
-    if (sh.nbeams == 1) {
-      // one beam: every position is in beam 0, stretched over bc samples
-      const float bl = (sh.sc == 0) ? 0.f : Stretch{sh.step_bl_last}(P, (unsigned)t);
-      return (x - bl) / R[0];
-    }
-    const float y = x - baseline_at(P, sh, t);
-    return y / R[(unsigned)((unsigned)t / sh.stride)];
+    return normalised_at(x, P, R, sh, t);
   };
   auto load_tile = [&](size_t tile, int b) {
     const int mt = tid - 64;

@@ -524,3 +524,27 @@ extern "C" hd_error hd_merge_candidates(hd_size count, const hd_size* d_labels,
   ST_CHECK(hipStreamSynchronize(s));
   return HD_NO_ERROR;
 }
+
+extern "C" hd_error hd_clean_filterbank_rfi(hd_size nchans, hd_float f0, hd_float df,
+                                            const hd_byte* d_in, hd_size nsamps, hd_size nbits,
+                                            hd_byte* d_out, int* h_killmask, hd_float dm,
+                                            hd_float dt, hd_float baseline_length,
+                                            hd_float rfi_tol, hd_size rfi_min_beams,
+                                            hd_size boxcar_max, hd_size nbeams, void* stream) {
+  (void)h_killmask;     // not used by the reference either
+  (void)rfi_min_beams;
+  if (!d_in || !d_out) return HD_INVALID_POINTER;
+  if (d_in == d_out) return HD_INVALID_POINTER;  // cleaning is out of place (pipeline.cu:253)
+  CleanPlan cp;
+  const int e = clean_plan(nsamps, nbits, nchans, nbeams, dm, dt, f0, df, baseline_length,
+                           rfi_tol, boxcar_max, cp);
+  if (e != HD_NO_ERROR) return (hd_error)e;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch tmp(s);
+  uint8_t* work;
+  ST_CHECK(tmp.get(&work, cp.work_bytes));
+  ST_CHECK(run_clean(cp, d_in, d_out, work, s));
+  // the delay table is copied from cp's host memory, which must outlive the copy
+  if (cp.max_delay > 0) ST_CHECK(hipStreamSynchronize(s));
+  return HD_NO_ERROR;
+}

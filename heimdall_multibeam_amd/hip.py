@@ -54,6 +54,14 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         check(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
 
+    def download(self, nbytes=None):
+        """Bytes of the buffer as a uint8 numpy array."""
+        import numpy as np
+        n = self.nbytes if nbytes is None else nbytes
+        out = np.empty(n, np.uint8)
+        check(hip().hipMemcpy(out.ctypes.data, self.ptr, n, D2H), "hipMemcpy D2H")
+        return out
+
     def free(self):
         if self.ptr:
             hip().hipFree(self.ptr)

@@ -45,12 +45,19 @@ hd_size hd_get_dm_list(hd_pipeline pl, hd_float* out, hd_size cap);
 
 /* Stage timings (milliseconds, HIP events) of the last call, in the order
  * unpack, dedisperse, baseline, rms, scan, block_rms, giants, cluster,
- * total.  Returns the number of values written (at most cap). */
+ * total, clean (total includes clean).  Returns the number of values
+ * written (at most cap). */
 hd_size hd_get_last_timings(hd_pipeline pl, double* ms, hd_size cap);
 
 /* When enabled, hd_execute* skips the .cand file / socket output (used by
  * benchmarks and tests that read candidates through the accessors). */
 void hd_set_output_enabled(hd_pipeline pl, int enabled);
+
+/* Runs the zero-DM RFI cleaning before every search (off by default).  The
+ * reference selects it at compile time (-DCLEANING, Pipeline/pipeline.cu:
+ * 255-299, 447-451): clean_filterbank_rfi at DM 0 with boxcars up to 64 and
+ * params.rfi_tol, then dedispersion of the cleaned copy. */
+void hd_set_cleaning(hd_pipeline pl, int enabled);
 
 #ifdef __cplusplus
 }

@@ -77,7 +77,8 @@ hd_error hd_find_giants(const hd_float* d_data, hd_size count, hd_size nbeams,
                         hd_size beam_count, hd_giants* out, void* stream);
 
 /* replaces label_candidate_clusters (Pipeline/label_candidate_clusters.cu:203-394),
- * with the reference's single-thread result (see DESIGN notes in the source). */
+ * with the reference's single-thread result: each giant takes the smallest index
+ * reachable along coincident links to earlier giants. */
 hd_error hd_label_candidate_clusters(hd_size count, const struct ConstRawCandidates* d_cands,
                                      hd_size time_tol, hd_size filter_tol, hd_size dm_tol,
                                      hd_size nsamps_beam, hd_size* d_labels,
@@ -87,6 +88,20 @@ hd_error hd_label_candidate_clusters(hd_size count, const struct ConstRawCandida
 hd_error hd_merge_candidates(hd_size count, const hd_size* d_labels,
                              const struct ConstRawCandidates* d_cands,
                              const struct RawCandidates* d_groups, void* stream);
+
+/* replaces clean_filterbank_rfi (Pipeline/clean_filterbank_rfi.cu:256-528,
+ * hd/clean_filterbank_rfi.h:14-26).  The reference takes the channel count,
+ * f0 and df from its dedisp plan; here they are arguments.  d_in and d_out
+ * are device filterbanks of nsamps x nchans*nbits/8 bytes (that row size
+ * must be a multiple of 4); the first nsamps - max_delay(dm) time samples of
+ * d_out receive the cleaned data.  h_killmask and rfi_min_beams are unused,
+ * as in the reference. */
+hd_error hd_clean_filterbank_rfi(hd_size nchans, hd_float f0, hd_float df,
+                                 const hd_byte* d_in, hd_size nsamps, hd_size nbits,
+                                 hd_byte* d_out, int* h_killmask, hd_float dm, hd_float dt,
+                                 hd_float baseline_length, hd_float rfi_tol,
+                                 hd_size rfi_min_beams, hd_size boxcar_max, hd_size nbeams,
+                                 void* stream);
 
 /* replaces apply_manual_killmasks (Pipeline/clean_filterbank_rfi.cu:530-545):
  * zeroes killmask[j] for j in [start, end) of each range (end exclusive). */

@@ -548,6 +548,146 @@ void orc_default_params(orc_params* p) {
   p->max_giant_rate = 0;
   p->min_tscrunch_width = 4096;
   p->nthreads = 1;
+  p->rfi_tol = 5.0f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Zero-DM RFI cleaning (clean_filterbank_rfi.cu). */
+
+/* hash() at clean_filterbank_rfi.cu:33-43, the published 32-bit integer hash
+ * of Robert Jenkins (six add/xor-shift rounds). */
+uint32_t orc_hash32(uint32_t a) {
+  a = (a + 0x7ed55d16u) + (a << 12);
+  a = (a ^ 0xc761c23cu) ^ (a >> 19);
+  a = (a + 0x165667b1u) + (a << 5);
+  a = (a + 0xd3a2646cu) ^ (a << 9);
+  a = (a + 0xfd7046c5u) + (a << 3);
+  a = (a ^ 0xb55a4f09u) ^ (a >> 16);
+  return a;
+}
+
+/* minstd_rand (Park-Miller: multiplier 48271, modulus 2^31-1, increment 0);
+ * a seed that is 0 mod m becomes 1 (thrust linear_congruential_engine::seed). */
+#define ORC_MINSTD_M 2147483647u
+uint32_t orc_minstd_seed(uint32_t s) {
+  uint32_t x = s % ORC_MINSTD_M;
+  return x == 0 ? 1u : x;
+}
+uint32_t orc_minstd_next(uint32_t* x) {
+  *x = (uint32_t)(((uint64_t)*x * 48271u) % ORC_MINSTD_M);
+  return *x;
+}
+
+/* uniform_int_distribution<unsigned>(lo, hi)(rng): a uniform_real_distribution
+ * <double>(lo, hi + 1) value truncated to unsigned; the real draw maps the
+ * engine output u (min 1, max m-1) to (u - 1) / (1 + (m - 2)). */
+uint32_t orc_uniform_draw(uint32_t* x, uint32_t lo, uint32_t hi) {
+  double r = (double)(uint32_t)(orc_minstd_next(x) - 1u);
+  r /= (1.0 + (double)(ORC_MINSTD_M - 1u - 1u));
+  const double a = (double)lo;
+  const double b = (double)hi + 1.0;
+  const double v = (r * (b - a)) + a;
+  return (uint32_t)v;
+}
+
+/* zap_fb_rfi_functor (clean_filterbank_rfi.cu:51-108) over 32-bit words. */
+int orc_zap_filterbank_rfi(const int* mask, const uint8_t* in, size_t nsamps, size_t nbits,
+                           size_t nchans, size_t max_resample_dist, uint8_t* out) {
+  const size_t stride_bytes = nchans * nbits / 8;
+  if (stride_bytes % 4 != 0 || nbits == 0 || nbits > 16) return -1;
+  if (nsamps == 0 || nsamps - 1 < max_resample_dist) return -2;
+  if (nsamps * (stride_bytes / 4) > 0xffffffffull) return -1;  /* 32-bit word index */
+  const uint32_t stride = (uint32_t)(stride_bytes / 4);
+  const uint32_t n = (uint32_t)nsamps, maxd = (uint32_t)max_resample_dist;
+  const uint32_t bitmask = (1u << nbits) - 1u;
+  const uint32_t* win = (const uint32_t*)in;
+  uint32_t* wout = (uint32_t*)out;
+  const uint32_t nwords = n * stride;
+  for (uint32_t i = 0; i < nwords; ++i) {
+    const uint32_t t = i / stride, c = i % stride;
+    uint32_t result;
+    if (mask[t]) {
+      uint32_t x = orc_minstd_seed(orc_hash32(i));
+      result = 0;
+      for (uint32_t k = 0; k < 32; k += (uint32_t)nbits) {
+        const uint32_t min_t = t > maxd ? t - maxd : 0;
+        const uint32_t max_t = t < n - 1 - maxd ? t + maxd : n - 1;
+        uint32_t new_t = orc_uniform_draw(&x, min_t, max_t);
+        int attempts = 0;
+        while (mask[new_t] && ++attempts < 10 + 1) new_t = orc_uniform_draw(&x, min_t, max_t);
+        const uint32_t val = (win[(size_t)new_t * stride + c] >> k) & bitmask;
+        result |= val << k;
+      }
+    } else {
+      result = win[i];
+    }
+    wout[i] = result;
+  }
+  return 0;
+}
+
+int orc_clean_filterbank_rfi(const uint8_t* in, size_t nsamps, size_t nbits, size_t nchans,
+                             float dm, float dt, float f0, float df, float baseline_length,
+                             float rfi_tol, size_t boxcar_max, size_t nbeams, uint8_t* out,
+                             int* mask_out) {
+  if (nbeams == 0 || boxcar_max == 0) return -1;
+  /* single-DM plan (:361-377): adaptive dt off, all channels */
+  float* kc = (float*)malloc(sizeof(float) * (nchans ? nchans : 1));
+  orc_delay_table(kc, nchans, dt, f0, df);
+  size_t max_delay = 0;
+  for (size_t c = 0; c < nchans; ++c) {
+    int dd = orc_delay(dm, kc[c]);
+    if (dd > 0 && (size_t)dd > max_delay) max_delay = (size_t)dd;
+  }
+  if ((nchans * nbits / 8) % 4 != 0) { free(kc); return -1; }
+  if (max_delay >= nsamps) { free(kc); return -2; }
+  const size_t beam_stride = nsamps / nbeams;              /* :379 */
+  const size_t N = nsamps - max_delay;                     /* :380 nsamps_computed */
+  if (max_delay >= beam_stride || N + 1 < boxcar_max) { free(kc); return -2; }
+  const size_t beam_nsamps = beam_stride - max_delay;      /* :381 */
+  /* (:383-397) dedisperse at dm */
+  float* series = (float*)calloc(N + 64, sizeof(float));
+  if (orc_dedisperse(in, nsamps, nbits, nchans, NULL, &dm, 1, kc, max_delay, series, N, 1) != 0) {
+    free(series); free(kc); return -1;
+  }
+  free(kc);
+  /* (:404-439) baseline, per-beam RMS, normalise */
+  const size_t nsamps_smooth = (size_t)(baseline_length / (2 * dt)); /* :405 */
+  orc_remove_baseline(NULL, series, beam_stride, beam_nsamps, nsamps_smooth, nbeams);
+  float* beam_rms = (float*)calloc(nbeams, sizeof(float));
+  orc_get_rms_multibeam(series, beam_rms, beam_stride, beam_nsamps, nbeams);
+  orc_normalise_multibeam(series, beam_rms, beam_stride, N);
+  free(beam_rms);
+  /* (:448-459) sigma cut */
+  int* mask = (int*)calloc(N + 1, sizeof(int));
+  for (size_t i = 0; i < N; ++i) mask[i] = fabsf(series[i]) > rfi_tol;
+  /* (:461-518) boxcars of the series; each width's cut is widened by a
+   * boxcar over the cut itself and ORed in */
+  const size_t L = N + 1 - boxcar_max;
+  const size_t off = boxcar_max / 2;
+  float* S = (float*)calloc(N + 1, sizeof(float));
+  float* filt = (float*)calloc(L ? L : 1, sizeof(float));
+  int* fm = (int*)calloc(N + 1, sizeof(int));
+  int* Sm = (int*)calloc(N + 1, sizeof(int));
+  orc_scan(series, N, S);
+  for (size_t w = 1; w <= boxcar_max; w *= 2) {
+    orc_boxcar(S, N + 1, boxcar_max, w, 1, filt);
+    const float norm = (float)(1.0 / sqrtf((float)w));   /* :477-478 */
+    for (size_t k = 0; k < L; ++k) filt[k] = filt[k] * norm;
+    for (size_t k = 0; k < L; ++k) fm[off + k] = fabsf(filt[k]) > rfi_tol; /* :487-489 */
+    /* MatchedFilterPlan<int> over the cut (:494-498), written in place */
+    Sm[0] = 0;
+    for (size_t k = 1; k <= N; ++k) Sm[k] = Sm[k - 1] + fm[k - 1];
+    const size_t ahead = (w - 1) / 2 + 1, behind = w / 2;
+    for (size_t k = 0; k < L; ++k) fm[off + k] = Sm[off + ahead + k] - Sm[off - behind + k];
+    for (size_t i = 0; i < N; ++i) mask[i] = mask[i] || fm[i];  /* :501-504 */
+  }
+  free(S); free(filt); free(fm); free(Sm); free(series);
+  if (mask_out) memcpy(mask_out, mask, sizeof(int) * N);
+  /* (:509-518) zap the first N samples */
+  const int rc = orc_zap_filterbank_rfi(mask, in, N, nbits, nchans, nsamps_smooth / 4, out);
+  free(mask);
+  return rc;
 }
 
 static unsigned filter_index(unsigned v) { /* get_filter_index, pipeline.cu:126-136 */
@@ -712,11 +852,28 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
   n_sub = 0;
   for (size_t d = dm_first; d < dm_end; d += step) { sub_idx[n_sub] = d; sub_dm[n_sub++] = dm_list[d]; }
   float* plane = (float*)malloc(sizeof(float) * (n_sub ? n_sub : 1) * series_stride);
+  /* CLEANING build (pipeline.cu:263-299): zero-DM cleaning, boxcars to 64 */
+  uint8_t* cleaned = NULL;
+  if (p->cleaning) {
+    const size_t nbytes = nsamps * nchans * nbits / 8;
+    cleaned = (uint8_t*)malloc(nbytes ? nbytes : 1);
+    memcpy(cleaned, fb, nbytes);
+    int rc = orc_clean_filterbank_rfi(fb, nsamps, nbits, nchans, 0.f, p->dt, p->f0, p->df,
+                                      p->baseline_length, p->rfi_tol, 64, nbeams, cleaned, NULL);
+    if (rc != 0) {
+      r->error = rc == -1 ? 7 : 11;
+      free(cleaned); free(plane); free(dm_list); free(kc); free(sub_idx); free(sub_dm);
+      return r->error;
+    }
+    fb = cleaned;
+  }
   if (orc_dedisperse(fb, nsamps, nbits, nchans, p->killmask, sub_dm, n_sub, kc, max_delay,
                      plane, series_stride, p->nthreads) != 0) {
     r->error = 8; free(plane); free(dm_list); free(kc); free(sub_idx); free(sub_dm);
+    free(cleaned);
     return r->error;
   }
+  free(cleaned);
 
   /* The DM loop breaks before its first trial when the delay leaves no room
    * (pipeline.cu:501-506, 633-638); both tests are the same for every trial. */

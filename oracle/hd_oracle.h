@@ -100,6 +100,33 @@ size_t orc_label_clusters(size_t n, const orc_cand* c, size_t time_tol,
 void orc_merge_candidates(size_t n, const uint64_t* labels, const orc_cand* c,
                           orc_cand* groups);
 
+/* ---- zero-DM RFI cleaning (clean_filterbank_rfi.cu:256-528) ---- */
+/* hash() (clean_filterbank_rfi.cu:33-43): Robert Jenkins' 32-bit integer hash. */
+uint32_t orc_hash32(uint32_t a);
+/* thrust::default_random_engine (minstd_rand: x <- 48271 x mod 2^31-1) seeded
+ * with s, as in linear_congruential_engine::seed. */
+uint32_t orc_minstd_seed(uint32_t s);
+/* Advances the engine state and returns the new output. */
+uint32_t orc_minstd_next(uint32_t* x);
+/* One draw of thrust::uniform_int_distribution<unsigned>(lo, hi) from the
+ * engine state *x (double-precision uniform-real path of
+ * thrust/random/detail/uniform_{int,real}_distribution.inl). */
+uint32_t orc_uniform_draw(uint32_t* x, uint32_t lo, uint32_t hi);
+/* zap_filterbank_rfi (clean_filterbank_rfi.cu:51-108,197-234): 32-bit words of
+ * the first nsamps time samples; masked samples are replaced channel by
+ * channel from random unmasked nearby samples. */
+int orc_zap_filterbank_rfi(const int* mask, const uint8_t* in, size_t nsamps, size_t nbits,
+                           size_t nchans, size_t max_resample_dist, uint8_t* out);
+/* clean_filterbank_rfi: series at one DM (dm 0 in the pipeline), baseline,
+ * per-beam RMS, sigma cut and boxcars 1..boxcar_max, then the zap.  Writes
+ * the first nsamps - max_delay(dm) samples of out; mask_out (may be NULL)
+ * receives that many mask entries.  Returns 0, or -1 for shapes the
+ * reference cannot process (word stride, too few samples). */
+int orc_clean_filterbank_rfi(const uint8_t* in, size_t nsamps, size_t nbits, size_t nchans,
+                             float dm, float dt, float f0, float df, float baseline_length,
+                             float rfi_tol, size_t boxcar_max, size_t nbeams, uint8_t* out,
+                             int* mask_out);
+
 /* ---- whole-gulp restatement of hd_execute (pipeline.cu:233-1087) ---- */
 typedef struct {
   size_t nchans;
@@ -116,6 +143,8 @@ typedef struct {
   int    nthreads;             /* OpenMP threads (dedispersion and the DM loop) */
   size_t dm_first, dm_count;   /* DM subset processed (dm_count 0 = all) */
   size_t dm_stride;            /* process every dm_stride-th trial of the subset (0 = 1) */
+  int    cleaning;             /* run clean_filterbank_rfi first (the CLEANING build) */
+  float  rfi_tol;              /* sigma cut of the cleaning (default 5) */
 } orc_params;
 
 typedef struct {

@@ -37,7 +37,8 @@ class Params(C.Structure):
                 ("cand_sep_dm", C.c_size_t), ("max_giant_rate", C.c_float),
                 ("min_tscrunch_width", C.c_size_t), ("beam", C.c_size_t),
                 ("killmask", C.POINTER(C.c_int)), ("nthreads", C.c_int),
-                ("dm_first", C.c_size_t), ("dm_count", C.c_size_t), ("dm_stride", C.c_size_t)]
+                ("dm_first", C.c_size_t), ("dm_count", C.c_size_t), ("dm_stride", C.c_size_t),
+                ("cleaning", C.c_int), ("rfi_tol", C.c_float)]
 
 
 class Result(C.Structure):
@@ -103,6 +104,20 @@ def lib():
         L.orc_merge_candidates.restype = None
         L.orc_merge_candidates.argtypes = [sz, C.POINTER(C.c_uint64), C.POINTER(Cand),
                                            C.POINTER(Cand)]
+        L.orc_hash32.restype = C.c_uint32
+        L.orc_hash32.argtypes = [C.c_uint32]
+        L.orc_minstd_seed.restype = C.c_uint32
+        L.orc_minstd_seed.argtypes = [C.c_uint32]
+        L.orc_minstd_next.restype = C.c_uint32
+        L.orc_minstd_next.argtypes = [C.POINTER(C.c_uint32)]
+        L.orc_uniform_draw.restype = C.c_uint32
+        L.orc_uniform_draw.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]
+        L.orc_zap_filterbank_rfi.restype = C.c_int
+        L.orc_zap_filterbank_rfi.argtypes = [C.c_void_p, C.c_void_p, sz, sz, sz, sz, C.c_void_p]
+        L.orc_clean_filterbank_rfi.restype = C.c_int
+        L.orc_clean_filterbank_rfi.argtypes = [C.c_void_p, sz, sz, sz, C.c_float, C.c_float,
+                                               C.c_float, C.c_float, C.c_float, C.c_float, sz,
+                                               sz, C.c_void_p, C.c_void_p]
         L.orc_default_params.restype = None
         L.orc_default_params.argtypes = [C.POINTER(Params)]
         L.orc_execute.restype = C.c_int
@@ -238,6 +253,50 @@ def merge_candidates(labels, cands, n_groups):
                                cands.ctypes.data_as(C.POINTER(Cand)),
                                groups.ctypes.data_as(C.POINTER(Cand)))
     return groups[:n_groups]
+
+
+def hash32(a):
+    return lib().orc_hash32(a & 0xffffffff)
+
+
+def minstd_seed(s):
+    return lib().orc_minstd_seed(s & 0xffffffff)
+
+
+def minstd_outputs(state, n):
+    """The next n outputs of the minstd engine from `state`."""
+    x = C.c_uint32(state)
+    return [lib().orc_minstd_next(C.byref(x)) for _ in range(n)]
+
+
+def uniform_draws(seed_state, lo, hi, n):
+    """n successive uniform_int_distribution(lo, hi) draws from engine state
+    seed_state; returns (draws, final state)."""
+    x = C.c_uint32(seed_state)
+    out = [lib().orc_uniform_draw(C.byref(x), lo, hi) for _ in range(n)]
+    return out, x.value
+
+
+def zap_filterbank_rfi(mask, fb, nsamps, nbits, nchans, max_resample_dist):
+    mask = np.ascontiguousarray(mask, np.int32)
+    fb = np.ascontiguousarray(fb, np.uint8)
+    out = np.zeros_like(fb)
+    rc = lib().orc_zap_filterbank_rfi(mask.ctypes.data, fb.ctypes.data, nsamps, nbits, nchans,
+                                      max_resample_dist, out.ctypes.data)
+    return rc, out
+
+
+def clean_filterbank_rfi(fb, nsamps, nbits, nchans, dm, dt, f0, df, baseline_length, rfi_tol,
+                         boxcar_max, nbeams):
+    """Returns (rc, cleaned filterbank, mask).  Samples past nsamps -
+    max_delay(dm) keep the input's bytes."""
+    fb = np.ascontiguousarray(fb, np.uint8)
+    out = fb.copy()
+    mask = np.zeros(nsamps, np.int32)
+    rc = lib().orc_clean_filterbank_rfi(fb.ctypes.data, nsamps, nbits, nchans, dm, dt, f0, df,
+                                        baseline_length, rfi_tol, boxcar_max, nbeams,
+                                        out.ctypes.data, mask.ctypes.data)
+    return rc, out, mask
 
 
 def default_params():

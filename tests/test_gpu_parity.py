@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_both(O, w, seed, nbeams=1, nbits=8, first_idx=0, gpu_kw=None, orc_kw=None,
-              fb=None, nsamps=None):
+              fb=None, nsamps=None, cleaning=False):
     import heimdall_multibeam_amd as H
     from heimdall_multibeam_amd import synth
     gpu_kw = gpu_kw or {}
@@ -22,9 +22,11 @@ def _run_both(O, w, seed, nbeams=1, nbits=8, first_idx=0, gpu_kw=None, orc_kw=No
         beams = [synth.make_gulp(w, seed + b) for b in range(nbeams)]
         fb = np.concatenate(beams)
         nsamps = w.nsamps * nbeams
-    print("case", w.name, "seed", seed, "nbeams", nbeams, "nbits", nbits, flush=True)
+    print("case", w.name, "seed", seed, "nbeams", nbeams, "nbits", nbits, "cleaning", cleaning,
+          flush=True)
     pl = H.Pipeline(params_for(w, **gpu_kw))
     pl.set_output(False)
+    pl.set_cleaning(cleaning)
     allow = (H.HD_NO_ERROR, H.HD_TOO_MANY_EVENTS)
     err, processed = pl.execute(fb, nsamps, nbits, first_idx, nbeams, allow=allow)
     g = pl.giants()
@@ -33,7 +35,7 @@ def _run_both(O, w, seed, nbeams=1, nbits=8, first_idx=0, gpu_kw=None, orc_kw=No
     dms = pl.dm_list()
     md = pl.max_delay()
     pl.close()
-    p = oracle_params_for(O, w, **orc_kw)
+    p = oracle_params_for(O, w, cleaning=1 if cleaning else 0, **orc_kw)
     r = O.execute(p, fb, nsamps, nbits, first_idx, nbeams)
     return dict(err=err, processed=processed, giants=g, cands=c, text=text, dms=dms,
                 max_delay=md), r
