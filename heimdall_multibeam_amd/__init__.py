@@ -61,6 +61,12 @@ class HdParams(C.Structure):
     ]
 
 
+class HdGulpDesc(C.Structure):
+    """hd_gulp_desc (include/hd/mi355x.h): one gulp of hd_execute_batch."""
+    _fields_ = [("filterbank", C.c_void_p), ("nsamps", C.c_size_t), ("nbeams", C.c_size_t),
+                ("first_idx", C.c_size_t), ("beam", C.c_size_t)]
+
+
 _lib = None
 
 
@@ -107,6 +113,11 @@ def lib():
     L.hd_get_last_timings.restype = sz
     L.hd_set_output_enabled.argtypes = [vp, C.c_int]
     L.hd_set_output_enabled.restype = None
+    L.hd_execute_batch.argtypes = [vp, C.POINTER(HdGulpDesc), sz, sz, C.c_int,
+                                   C.POINTER(C.c_size_t), C.POINTER(C.c_int)]
+    L.hd_execute_batch.restype = C.c_int
+    L.hd_get_last_candidate_gulps.argtypes = [vp, C.POINTER(C.c_size_t), sz]
+    L.hd_get_last_candidate_gulps.restype = sz
     L.hd_set_cleaning.argtypes = [vp, C.c_int]
     L.hd_set_cleaning.restype = None
     L.hd_clean_filterbank_rfi.argtypes = [sz, C.c_float, C.c_float, vp, sz, sz, vp,
@@ -206,6 +217,39 @@ class Pipeline:
         if err not in allow:
             raise HdError(err, "hd_execute")
         return err, processed.value
+
+    def execute_batch(self, gulps, nbits, device=False, allow=(HD_NO_ERROR,)):
+        """hd_execute_batch.  gulps: sequence of (filterbank, nsamps, nbeams,
+        first_idx, beam), filterbank a host array (or a device address with
+        device=True).  Returns (error, processed list, per-gulp errors)."""
+        n = len(gulps)
+        descs = (HdGulpDesc * max(n, 1))()
+        keep = []
+        for k, (fb, ns, nb, fi, beam) in enumerate(gulps):
+            if device:
+                addr = int(fb)
+            else:
+                a = np.ascontiguousarray(fb, dtype=np.uint8)
+                keep.append(a)
+                addr = a.ctypes.data
+            descs[k] = HdGulpDesc(addr, ns, nb, fi, beam)
+        processed = (C.c_size_t * max(n, 1))()
+        errs = (C.c_int * max(n, 1))()
+        err = lib().hd_execute_batch(self.handle, descs, n, nbits, 1 if device else 0, processed,
+                                     errs)
+        if err not in allow:
+            raise HdError(err, "hd_execute_batch")
+        return err, list(processed)[:n], list(errs)[:n]
+
+    def candidate_gulps(self):
+        """Gulp index (in the last batch) of each candidate."""
+        L = lib()
+        n = L.hd_get_last_candidate_gulps(self.handle, None, 0)
+        out = np.zeros(n, np.uint64)
+        if n:
+            L.hd_get_last_candidate_gulps(self.handle,
+                                          out.ctypes.data_as(C.POINTER(C.c_size_t)), n)
+        return out
 
     def dm_list(self):
         n = lib().hd_get_dm_count(self.handle)

@@ -73,6 +73,17 @@ unsigned log2_pow2(unsigned v) {  // get_filter_index (pipeline.cu:126-136)
 enum Stage { ST_UNPACK, ST_DEDISP, ST_BASELINE, ST_RMS, ST_SCAN, ST_BLOCKRMS, ST_GIANTS,
              ST_CLUSTER, ST_TOTAL, ST_CLEAN, ST_COUNT };
 
+// One gulp's output.  Candidates [lo, hi) of the result arrays belong to it;
+// `beam` is the params.beam of the call that searched it.
+struct GulpOut {
+  size_t nsamps = 0, nbeams = 1, first_idx = 0, beam = 0;
+  size_t processed = 0;
+  bool too_many = false;
+  hd_error err = HD_NO_ERROR;
+  size_t lo = 0, hi = 0;
+};
+
+
 }  // namespace
 
 struct hd_pipeline_t {
@@ -94,6 +105,10 @@ struct hd_pipeline_t {
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
   DevBuf o_peak, o_ind, o_begin, o_end, o_filter, o_dm, o_members;
   DevBuf clean_out, clean_work;
+  DevBuf gdev, g_group, o_group;        // group table, group of each giant / candidate
+  std::vector<hd::GroupDev> hgd;        // host copy of the group table
+  std::vector<GulpOut> gulps;           // per-gulp status and candidate ranges
+  std::vector<uint32_t> r_group;        // active-group index of each candidate
   unsigned* h_pinned = nullptr;  // small pinned staging word(s)
   size_t sample_cap = 1u << 22;
   // delay table state for the current live-channel set
@@ -477,12 +492,13 @@ hd_error make_shapes(hd_pipeline pl, size_t nsamps, size_t nbeams, CallShape& cs
 void record(hd_pipeline pl, int st) { (void)hipEventRecord(pl->ev[st], pl->stream); }
 
 // ostream float formatting as the reference's candidate writer uses it.
-void write_cand_lines(hd_pipeline pl, std::ostream& os, size_t nsamps, size_t nbeams,
-                      size_t first_idx, size_t processed) {
+void write_cand_lines(hd_pipeline pl, std::ostream& os, const GulpOut& go) {
   const hd_params& p = pl->params;
-  const unsigned nsamps_beam = (unsigned)(nsamps / nbeams);      // pipeline.cu:903
-  const unsigned nsamps_valid = (unsigned)(processed / nbeams);   // :905
-  for (size_t i = 0; i < pl->r_peak.size(); ++i) {
+  const size_t first_idx = go.first_idx;
+  const unsigned nsamps_beam = (unsigned)(go.nsamps / go.nbeams);      // pipeline.cu:903
+  const unsigned nsamps_valid = (unsigned)(go.processed / go.nbeams);   // :905
+  const size_t pbeam = go.beam;
+  for (size_t i = go.lo; i < go.hi; ++i) {
     const hd_size ind = pl->r_ind[i];
     const hd_size beam_idx = (hd_size)std::floor((float)ind / (float)nsamps_beam);
     const hd_size beam_start = beam_idx * nsamps_beam + (p.boxcar_max / 2);
@@ -494,17 +510,20 @@ void write_cand_lines(hd_pipeline pl, std::ostream& os, size_t nsamps, size_t nb
     os << snr << "\t" << samp_idx << "\t" << p.dt * samp_idx << "\t" << pl->r_filter[i] << "\t"
        << pl->r_dm[i] << "\t" << pl->r_dms[i] << "\t" << pl->r_members[i] << "\t"
        << first_idx + (pl->r_begin[i] % nsamps_beam) << "\t"
-       << first_idx + (pl->r_end[i] % nsamps_beam) << "\t" << (p.beam + beam_idx + 1) << "\t"
+       << first_idx + (pl->r_end[i] % nsamps_beam) << "\t" << (pbeam + beam_idx + 1) << "\t"
        << "\n";
   }
 }
 
-hd_error emit_output(hd_pipeline pl, size_t nsamps, size_t nbeams, size_t first_idx,
-                     size_t processed) {
+hd_error emit_output(hd_pipeline pl, const GulpOut& go) {
   const hd_params& p = pl->params;
+  const size_t nsamps = go.nsamps, nbeams = go.nbeams, first_idx = go.first_idx;
+  const size_t processed = go.processed;
+  const size_t pbeam = go.beam;
   std::ostringstream lines;
-  write_cand_lines(pl, lines, nsamps, nbeams, first_idx, processed);
-  pl->cand_text = lines.str();
+  write_cand_lines(pl, lines, go);
+  const std::string text = lines.str();
+  pl->cand_text += text;
   if (!pl->output_enabled) return HD_NO_ERROR;
   // Output name (pipeline.cu:893-901).  The reference divides by
   // spectra_per_second unconditionally; a zero rate is taken as offset 0.
@@ -513,13 +532,13 @@ hd_error emit_output(hd_pipeline pl, size_t nsamps, size_t nbeams, size_t first_
   time_t now = p.utc_start + off;
   std::strftime(buffer, 64, "%Y-%m-%d-%H:%M:%S", std::gmtime(&now));
   std::ostringstream ss;
-  ss << std::setw(2) << std::setfill('0') << (p.beam + 1);
+  ss << std::setw(2) << std::setfill('0') << (pbeam + 1);
   if (!pl->coinc_host.empty() && p.coincidencer_port != -1) {
     // TCP branch (pipeline.cu:907-995): header line then one line per event
     const unsigned nsamps_beam = (unsigned)(nsamps / nbeams);
     const unsigned nsamps_valid = (unsigned)(processed / nbeams);
     unsigned n_events = 0;
-    for (size_t i = 0; i < pl->r_ind.size(); ++i) {
+    for (size_t i = go.lo; i < go.hi; ++i) {
       const hd_size ind = pl->r_ind[i];
       const hd_size beam_idx = (hd_size)std::floor((float)ind / (float)nsamps_beam);
       const hd_size beam_start = beam_idx * nsamps_beam + (p.boxcar_max / 2);
@@ -529,10 +548,10 @@ hd_error emit_output(hd_pipeline pl, size_t nsamps, size_t nbeams, size_t first_
     char ubuf[64];
     time_t utc = p.utc_start;
     std::strftime(ubuf, 64, "%Y-%m-%d-%H:%M:%S", std::gmtime(&utc));
-    msg << ubuf << " " << buffer << " " << first_idx << " " << p.beam << " " << nbeams << " "
+    msg << ubuf << " " << buffer << " " << first_idx << " " << pbeam << " " << nbeams << " "
         << n_events << std::endl;
     std::ostringstream ev;
-    for (size_t i = 0; i < pl->r_ind.size(); ++i) {
+    for (size_t i = go.lo; i < go.hi; ++i) {
       const hd_size ind = pl->r_ind[i];
       const hd_size beam_idx = (hd_size)std::floor((float)ind / (float)nsamps_beam);
       const hd_size beam_start = beam_idx * nsamps_beam + (p.boxcar_max / 2);
@@ -542,7 +561,7 @@ hd_error emit_output(hd_pipeline pl, size_t nsamps, size_t nbeams, size_t first_
       ev << snr << "\t" << samp_idx << "\t" << samp_idx * p.dt << "\t" << pl->r_filter[i] << "\t"
          << pl->r_dm[i] << "\t" << pl->r_dms[i] << "\t" << pl->r_members[i] << "\t"
          << first_idx + (pl->r_begin[i] % nsamps_beam) << "\t"
-         << first_idx + (pl->r_end[i] % nsamps_beam) << "\t" << (p.beam + beam_idx + 1)
+         << first_idx + (pl->r_end[i] % nsamps_beam) << "\t" << (pbeam + beam_idx + 1)
          << std::endl;
     }
     addrinfo hints{}, *res = nullptr;
@@ -567,109 +586,184 @@ hd_error emit_output(hd_pipeline pl, size_t nsamps, size_t nbeams, size_t first_
   }
   const std::string filename = pl->output_dir + "/" + std::string(buffer) + "_" + ss.str() + ".cand";
   std::ofstream cand_file(filename.c_str(), std::ios::out);
-  if (cand_file.good()) cand_file << pl->cand_text;
+  if (cand_file.good()) cand_file << text;
   else std::cout << "Skipping dump due to bad file open on " << filename << std::endl;
   cand_file.close();
   return HD_NO_ERROR;
 }
 
-// The device part of one gulp.  `d_fb` is the filterbank in device memory.
-hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbits,
-                  size_t first_idx, size_t nbeams, size_t* processed, bool* too_many) {
+// Input of one gulp: what a separate hd_execute call would get, with the
+// filterbank already in device memory.
+struct GulpIn {
+  const hd_byte* d_fb;
+  size_t nsamps, nbeams, first_idx, beam;
+};
+
+// The device part of a call: every gulp of `in` is searched with one launch
+// per stage (the gulps are the groups of the kernels' group dimension).
+// Per-gulp results land in pl->gulps; a returned error concerns the call.
+hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) {
   const hd_params& p = pl->params;
   hipStream_t s = pl->stream;
-  *too_many = false;
   pl->n_giants = 0;
   pl->r_peak.clear(); pl->r_ind.clear(); pl->r_begin.clear(); pl->r_end.clear();
   pl->r_filter.clear(); pl->r_dm.clear(); pl->r_members.clear(); pl->r_dms.clear();
+  pl->r_group.clear();
+  pl->cand_text.clear();
   for (double& t : pl->timings) t = 0;
   const size_t W = p.boxcar_max;
   const size_t max_delay = pl->max_delay;
-  // pipeline.cu:370-372
-  *processed = (nsamps < nbeams * (W + max_delay)) ? 0 : nsamps - nbeams * (W + max_delay);
-  if (max_delay >= nsamps) return HD_TOO_FEW_NSAMPS;
-  CallShape cs;
-  hd_error err = make_shapes(pl, nsamps, nbeams, cs);
+  pl->gulps.assign(in.size(), GulpOut{});
+  // shapes of every gulp (pipeline.cu:366-372, 488-499, 633-638)
+  std::vector<CallShape> shapes(in.size());
+  std::vector<int> active;
+  for (size_t k = 0; k < in.size(); ++k) {
+    GulpOut& go = pl->gulps[k];
+    go.nsamps = in[k].nsamps;
+    go.nbeams = in[k].nbeams;
+    go.first_idx = in[k].first_idx;
+    go.beam = in[k].beam;
+    const size_t ns = in[k].nsamps, nb = in[k].nbeams;
+    go.processed = (ns < nb * (W + max_delay)) ? 0 : ns - nb * (W + max_delay);
+    if (max_delay >= ns) {
+      go.err = HD_TOO_FEW_NSAMPS;
+      continue;
+    }
+    go.err = make_shapes(pl, ns, nb, shapes[k]);
+    if (go.err == HD_NO_ERROR && !shapes[k].skip) active.push_back((int)k);
+  }
+  hd_error err = build_live_and_delays(pl);
   if (err != HD_NO_ERROR) return err;
-  err = build_live_and_delays(pl);
-  if (err != HD_NO_ERROR) return err;
-  if (cs.skip) return HD_NO_ERROR;
-  const SeriesShape& sh = cs.sh;
-  const WidthTable& wt = cs.wt;
+  if (active.empty()) return HD_NO_ERROR;
+  const int G = (int)active.size();
+  const WidthTable& wt = shapes[active[0]].wt;  // shared but for the fresh counts
   const int n_dm = (int)pl->dm_list.size();
   const int n_live = (int)pl->live_list.size();
-  const int TB = hd::dedisp_tile_time();
+  // common row pitches; every group keeps its own lengths
+  size_t ppitch = 0, spitch = 0;
+  unsigned nbmax = 0;
+  size_t prow = 0;
+  for (int k : active) {
+    ppitch = std::max(ppitch, shapes[k].sh.ppitch);
+    spitch = std::max(spitch, shapes[k].sh.spitch);
+    nbmax = std::max(nbmax, shapes[k].sh.nbeams);
+    prow = std::max(prow, (size_t)shapes[k].sh.nbeams * shapes[k].sh.sc);
+  }
+  const uint64_t span = (uint64_t)p.cand_sep_time << wt.wi[wt.nw - 1].filter_idx;
+  pl->hgd.assign(G, hd::GroupDev{});
+  uint64_t t_off = 0;
+  for (int g = 0; g < G; ++g) {
+    const CallShape& cs = shapes[active[g]];
+    hd::GroupDev& d = pl->hgd[g];
+    d.sh = cs.sh;
+    d.sh.ppitch = ppitch;
+    d.sh.spitch = spitch;
+    d.bs = cs.bs;
+    d.bc692 = cs.bc692;
+    d.fb = in[active[g]].d_fb;
+    d.nsamps = in[active[g]].nsamps;
+    for (int j = 0; j < cs.wt.nw; ++j) d.n_w[j] = cs.wt.wi[j].n_w;
+    d.t_off = t_off;
+    d.nsamps_beam = d.nsamps / cs.sh.nbeams;
+    t_off += d.nsamps + 2 * span + 64;  // groups never fall within one coincidence window
+  }
+  if (t_off >= (1ull << 32)) return HD_INVALID_STRIDE;  // clustering coordinate is 32-bit
 
   record(pl, 9);
   if (pl->cleaning) {
     // pipeline.cu:263-299: clean at DM 0 with boxcars up to 64, then search
     // the cleaned copy (:447-451)
-    hd::CleanPlan cp;
-    const int ce = hd::clean_plan(nsamps, nbits, p.nchans, nbeams, 0.f, p.dt, p.f0, p.df,
-                                  p.baseline_length, p.rfi_tol, 64, cp);
-    if (ce != HD_NO_ERROR) return (hd_error)ce;
-    HD_CHECK(pl->clean_out.ensure(nsamps * p.nchans * nbits / 8));
-    HD_CHECK(pl->clean_work.ensure(cp.work_bytes));
-    HD_CHECK(hd::run_clean(cp, d_fb, pl->clean_out.as<uint8_t>(), pl->clean_work.p, s));
-    d_fb = pl->clean_out.as<hd_byte>();
+    const size_t row_bytes = p.nchans * nbits / 8;
+    size_t total = 0, work = 0;
+    std::vector<hd::CleanPlan> cps(G);
+    for (int g = 0; g < G; ++g) {
+      const int ce = hd::clean_plan(pl->hgd[g].nsamps, nbits, p.nchans, pl->hgd[g].sh.nbeams, 0.f,
+                                    p.dt, p.f0, p.df, p.baseline_length, p.rfi_tol, 64, cps[g]);
+      if (ce != HD_NO_ERROR) return (hd_error)ce;
+      total += pl->hgd[g].nsamps * row_bytes;
+      work = std::max(work, cps[g].work_bytes);
+    }
+    HD_CHECK(pl->clean_out.ensure(total));
+    HD_CHECK(pl->clean_work.ensure(work));
+    size_t off = 0;
+    for (int g = 0; g < G; ++g) {
+      hd_byte* dst = pl->clean_out.as<hd_byte>() + off;
+      HD_CHECK(hd::run_clean(cps[g], pl->hgd[g].fb, dst, pl->clean_work.p, s));
+      pl->hgd[g].fb = dst;
+      off += pl->hgd[g].nsamps * row_bytes;
+    }
   }
+  HD_CHECK(pl->gdev.ensure((size_t)G * sizeof(hd::GroupDev)));
+  HD_CHECK(hipMemcpyAsync(pl->gdev.p, pl->hgd.data(), (size_t)G * sizeof(hd::GroupDev),
+                          hipMemcpyHostToDevice, s));
+  hd::GroupArgs ga;
+  ga.gd = pl->gdev.as<hd::GroupDev>();
+  ga.host = pl->hgd.data();
+  ga.G = G;
+  ga.nbmax = nbmax;
+  ga.prow = std::max<size_t>(prow, 1);
+  const SeriesShape& sh0 = pl->hgd[0].sh;
+  const size_t rows = (size_t)G * n_dm;
+
   record(pl, 0);
   // unpack to channel-major bytes
-  const size_t xpitch = round_up(sh.ppitch + max_delay + 2 * pl->wlen + 256, 256);
-  HD_CHECK(pl->x.ensure((size_t)std::max(n_live, 1) * xpitch));
-  HD_CHECK(hd::launch_unpack(d_fb, p.nchans * nbits / 8, (int)nbits, nsamps, pl->live.as<int>(),
-                             n_live, pl->x.as<uint8_t>(), xpitch, s));
+  const size_t xpitch = round_up(ppitch + max_delay + 2 * pl->wlen + 256, 256);
+  HD_CHECK(pl->x.ensure((size_t)G * std::max(n_live, 1) * xpitch));
+  HD_CHECK(hd::launch_unpack(nullptr, p.nchans * nbits / 8, (int)nbits, 0, pl->live.as<int>(),
+                             n_live, pl->x.as<uint8_t>(), xpitch, s, ga));
   record(pl, 1);
   // dedisperse
-  HD_CHECK(pl->plane.ensure((size_t)n_dm * sh.ppitch * sizeof(float)));
-  hd::DedispGeom g{n_dm, n_live, sh.nout, xpitch, sh.ppitch, (int)pl->wlen};
-  (void)TB;
-  HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pl->chunk_pairs.as<int>(), g,
-                                 pl->plane.as<float>(), s));
+  HD_CHECK(pl->plane.ensure(rows * ppitch * sizeof(float)));
+  hd::DedispGeom dg{n_dm, n_live, sh0.nout, xpitch, ppitch, (int)pl->wlen};
+  HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
+                                 pl->chunk_pairs.as<int>(), dg, pl->plane.as<float>(), s, ga));
   record(pl, 2);
   // baseline points and per-beam RMS (normalisation is fused into the scan)
-  HD_CHECK(pl->pts.ensure(((size_t)n_dm * nbeams * std::max<size_t>(sh.sc, 1) + 64) * sizeof(float)));
-  HD_CHECK(pl->rms.ensure((size_t)n_dm * nbeams * sizeof(float)));
-  HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh, n_dm, pl->pts.as<float>(), s));
+  HD_CHECK(pl->pts.ensure((rows * ga.prow + 64) * sizeof(float)));
+  HD_CHECK(pl->rms.ensure(rows * nbmax * sizeof(float)));
+  HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh0, n_dm, pl->pts.as<float>(), s,
+                                      ga));
   record(pl, 3);
   {
-    const size_t words = hd::beam_rms_scratch_words(sh, n_dm);
+    const size_t words = hd::beam_rms_scratch_words(sh0, n_dm, ga);
     HD_CHECK(pl->brms.ensure(std::max<size_t>(words, 1) * sizeof(float)));
-    HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh, n_dm,
-                                 pl->rms.as<float>(), s, pl->brms.as<float>(), words));
+    HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh0, n_dm,
+                                 pl->rms.as<float>(), s, pl->brms.as<float>(), words, ga));
   }
   record(pl, 4);
-  // normalised prefix sums
-  // prefix-sum rows start 3 floats into the allocation so that S[1..] is
-  // 16-byte aligned for the scan's vector stores
-  HD_CHECK(pl->S.ensure(((size_t)n_dm * sh.spitch + 8) * sizeof(float)));
+  // normalised prefix sums; rows start 3 floats into the allocation so that
+  // S[1..] is 16-byte aligned for the scan's vector stores
+  HD_CHECK(pl->S.ensure((rows * spitch + 8) * sizeof(float)));
   float* const Sbase = pl->S.as<float>() + 3;
-  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), sh.ppitch, sh.nout, n_dm, Sbase,
-                           sh.spitch, s, pl->pts.as<float>(), pl->rms.as<float>(), &sh));
+  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), ppitch, 0, n_dm, Sbase, spitch, s,
+                           pl->pts.as<float>(), pl->rms.as<float>(), nullptr, ga));
   record(pl, 5);
   // block RMS: the ts == 1 widths together, then each decimated width in order
-  const BlockRmsShape& bs = cs.bs;
-  HD_CHECK(pl->partial.ensure((size_t)n_dm * wt.nw * std::max<size_t>(bs.np, 1) * sizeof(float)));
-  HD_CHECK(pl->wrms.ensure((size_t)n_dm * wt.nw * sizeof(float)));
-  HD_CHECK(pl->winv.ensure((size_t)n_dm * wt.nw * sizeof(float)));
+  const size_t np_max = hd::block_rms_np_max(ga);
+  HD_CHECK(pl->partial.ensure(rows * wt.nw * std::max<size_t>(np_max, 1) * sizeof(float)));
+  HD_CHECK(pl->wrms.ensure(rows * wt.nw * sizeof(float)));
+  HD_CHECK(pl->winv.ensure(rows * wt.nw * sizeof(float)));
   int n_fresh = 0;
   while (n_fresh < wt.nw && wt.wi[n_fresh].ts == 1) ++n_fresh;
-  HD_CHECK(hd::launch_block_rms_partials(Sbase, sh, wt, bs, 0, n_fresh, n_dm,
-                                         pl->partial.as<float>(), pl->winv.as<float>(), s));
-  HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), bs, wt.nw, 0, n_fresh, n_dm,
-                                       pl->wrms.as<float>(), pl->winv.as<float>(), s));
+  HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, 0, n_fresh, n_dm,
+                                         pl->partial.as<float>(), pl->winv.as<float>(), s, ga));
+  HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, 0, n_fresh, n_dm,
+                                       pl->wrms.as<float>(), pl->winv.as<float>(), s, ga));
   for (int j = n_fresh; j < wt.nw; ++j) {
-    HD_CHECK(hd::launch_block_rms_partials(Sbase, sh, wt, bs, j, 1, n_dm,
-                                           pl->partial.as<float>(), pl->winv.as<float>(), s));
-    HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), bs, wt.nw, j, 1, n_dm,
-                                         pl->wrms.as<float>(), pl->winv.as<float>(), s));
+    HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, j, 1, n_dm,
+                                           pl->partial.as<float>(), pl->winv.as<float>(), s,
+                                           ga));
+    HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, j, 1, n_dm,
+                                         pl->wrms.as<float>(), pl->winv.as<float>(), s, ga));
   }
   record(pl, 6);
   // threshold + ordered compaction
   hd::GiantBuffers gb{};
   gb.chunk = 4096;
-  gb.nchunks_per_w = (wt.wi[0].n_w + gb.chunk - 1) / gb.chunk;
-  const size_t nkeys = (size_t)n_dm * wt.nw * gb.nchunks_per_w;
+  size_t n_w0 = 0;
+  for (int g = 0; g < G; ++g) n_w0 = std::max<size_t>(n_w0, pl->hgd[g].n_w[0]);
+  gb.nchunks_per_w = (n_w0 + gb.chunk - 1) / gb.chunk;
+  const size_t nkeys = rows * wt.nw * gb.nchunks_per_w;
   HD_CHECK(pl->ccount.ensure(nkeys * sizeof(unsigned)));
   HD_CHECK(pl->coff.ensure(nkeys * sizeof(unsigned)));
   HD_CHECK(pl->cpos.ensure(nkeys * sizeof(unsigned)));
@@ -689,8 +783,8 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
     gb.cpos = pl->cpos.as<unsigned>();
     gb.scan_tmp = pl->scan_tmp.as<unsigned>();
     HD_CHECK(hipMemsetAsync(gb.counter, 0, sizeof(unsigned), s));
-    HD_CHECK(hd::launch_threshold(Sbase, sh.spitch, sh, wt, pl->winv.as<float>(),
-                                  n_dm, p.detect_thresh, cs.bc692, gb, s));
+    HD_CHECK(hd::launch_threshold(Sbase, spitch, wt, pl->winv.as<float>(), n_dm,
+                                  p.detect_thresh, gb, s, ga));
     HD_CHECK(hipMemcpyAsync(pl->h_pinned, gb.counter, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HD_CHECK(hipStreamSynchronize(s));
     nsamp = pl->h_pinned[0];
@@ -727,27 +821,77 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
     HD_CHECK(pl->g_end.ensure(n_giants * sizeof(uint64_t)));
     HD_CHECK(pl->g_filter.ensure(n_giants * sizeof(uint32_t)));
     HD_CHECK(pl->g_dm.ensure(n_giants * sizeof(uint32_t)));
+    HD_CHECK(pl->g_group.ensure(n_giants * sizeof(uint32_t)));
     hd::GiantOut go{pl->g_peak.as<float>(), pl->g_ind.as<uint64_t>(), pl->g_begin.as<uint64_t>(),
                     pl->g_end.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
-                    pl->g_dm.as<uint32_t>()};
+                    pl->g_dm.as<uint32_t>(), pl->g_group.as<uint32_t>()};
     HD_CHECK(pl->g_best.ensure(std::max<size_t>(n_giants, 1) * sizeof(uint64_t)));
-    HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, (unsigned)n_giants, go,
+    HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, (unsigned)n_giants, n_dm, go,
                                        pl->g_best.as<uint64_t>(), s));
   }
   pl->n_giants = n_giants;
   record(pl, 7);
-  // giant-rate limit (pipeline.cu:780-787): the running total only grows, so
-  // the limit trips during the loop exactly when the final total exceeds it.
-  if (p.max_giant_rate != 0) {
-    const float mins = (float)((double)((float)nsamps * p.dt) / 60.0);
-    if ((float)n_giants / mins > p.max_giant_rate) *too_many = true;
+  // giant-rate limit (pipeline.cu:780-787), per gulp: the running total only
+  // grows, so the limit trips during the loop exactly when the final total
+  // exceeds it.  A gulp that trips it has no clustering and no candidates.
+  std::vector<char> excluded(G, 0);
+  bool any_excluded = false;
+  if (p.max_giant_rate != 0 && n_giants > 0) {
+    std::vector<uint32_t> grp(n_giants);
+    HD_CHECK(hipMemcpyAsync(grp.data(), pl->g_group.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipStreamSynchronize(s));
+    std::vector<size_t> cnt(G, 0);
+    for (uint32_t g : grp) ++cnt[g];
+    for (int g = 0; g < G; ++g) {
+      const float mins = (float)((double)((float)pl->hgd[g].nsamps * p.dt) / 60.0);
+      if ((float)cnt[g] / mins > p.max_giant_rate) {
+        excluded[g] = 1;
+        any_excluded = true;
+        pl->gulps[active[g]].too_many = true;
+      }
+    }
   }
-  if (*too_many || n_giants == 0) {
+  size_t n_cluster = n_giants;
+  if (any_excluded) {
+    // keep the giants of the other gulps, in order (a host round trip: the
+    // limiter is a rare, off-by-default path)
+    std::vector<uint32_t> grp(n_giants), filt(n_giants), dm(n_giants);
+    std::vector<float> peak(n_giants);
+    std::vector<uint64_t> ind(n_giants), beg(n_giants), end(n_giants);
+    HD_CHECK(hipMemcpyAsync(grp.data(), pl->g_group.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipMemcpyAsync(filt.data(), pl->g_filter.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipMemcpyAsync(dm.data(), pl->g_dm.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipMemcpyAsync(peak.data(), pl->g_peak.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipMemcpyAsync(ind.data(), pl->g_ind.p, n_giants * 8, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipMemcpyAsync(beg.data(), pl->g_begin.p, n_giants * 8, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipMemcpyAsync(end.data(), pl->g_end.p, n_giants * 8, hipMemcpyDeviceToHost, s));
+    HD_CHECK(hipStreamSynchronize(s));
+    size_t w = 0;
+    for (size_t k = 0; k < n_giants; ++k) {
+      if (excluded[grp[k]]) continue;
+      grp[w] = grp[k]; filt[w] = filt[k]; dm[w] = dm[k]; peak[w] = peak[k];
+      ind[w] = ind[k]; beg[w] = beg[k]; end[w] = end[k];
+      ++w;
+    }
+    n_cluster = w;
+    if (w) {
+      pl->n_giants = w;  // the giants of the gulps that go on to clustering
+      HD_CHECK(hipMemcpyAsync(pl->g_group.p, grp.data(), w * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_filter.p, filt.data(), w * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_dm.p, dm.data(), w * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_peak.p, peak.data(), w * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_ind.p, ind.data(), w * 8, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_begin.p, beg.data(), w * 8, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_end.p, end.data(), w * 8, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipStreamSynchronize(s));
+    }
+  }
+  if (n_cluster == 0) {
     record(pl, 8);
     return HD_NO_ERROR;
   }
   // clustering and merging (pipeline.cu:808-887)
-  const unsigned ng = (unsigned)n_giants;
+  const unsigned ng = (unsigned)n_cluster;
   size_t npad = 1;
   while (npad < ng) npad <<= 1;
   HD_CHECK(pl->c_keys.ensure(npad * sizeof(uint64_t)));
@@ -769,8 +913,9 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   int iters = 0;
   HD_CHECK(hd::launch_cluster(pl->g_ind.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
                               pl->g_dm.as<uint32_t>(), nullptr, ng, p.cand_sep_time,
-                              p.cand_sep_filter, p.cand_sep_dm, nsamps / nbeams, max_filter, cb,
-                              s, pl->h_pinned, &iters));
+                              p.cand_sep_filter, p.cand_sep_dm, pl->hgd[0].nsamps_beam,
+                              max_filter, cb, s, pl->h_pinned, &iters,
+                              pl->g_group.as<uint32_t>(), ga.gd));
   HD_CHECK(pl->o_peak.ensure(ng * sizeof(float)));
   HD_CHECK(pl->o_ind.ensure(ng * sizeof(uint64_t)));
   HD_CHECK(pl->o_begin.ensure(ng * sizeof(uint64_t)));
@@ -778,13 +923,15 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   HD_CHECK(pl->o_filter.ensure(ng * sizeof(uint64_t)));
   HD_CHECK(pl->o_dm.ensure(ng * sizeof(uint64_t)));
   HD_CHECK(pl->o_members.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->o_group.ensure(ng * sizeof(uint32_t)));
   HD_CHECK(hd::launch_merge(pl->g_peak.as<float>(), pl->g_ind.as<uint64_t>(),
                             pl->g_begin.as<uint64_t>(), pl->g_end.as<uint64_t>(),
                             pl->g_filter.as<uint32_t>(), pl->g_dm.as<uint32_t>(), nullptr, ng, cb,
                             pl->o_peak.as<float>(), pl->o_ind.as<uint64_t>(),
                             pl->o_begin.as<uint64_t>(), pl->o_end.as<uint64_t>(),
                             pl->o_filter.as<uint64_t>(), pl->o_dm.as<uint64_t>(),
-                            pl->o_members.as<uint64_t>(), s));
+                            pl->o_members.as<uint64_t>(), s, pl->g_group.as<uint32_t>(),
+                            pl->o_group.as<uint32_t>()));
   HD_CHECK(hipMemcpyAsync(pl->h_pinned, cb.root_pos + (ng - 1), sizeof(unsigned),
                           hipMemcpyDeviceToHost, s));
   HD_CHECK(hipMemcpyAsync(pl->h_pinned + 1, cb.root_flag + (ng - 1), sizeof(unsigned),
@@ -799,6 +946,7 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   pl->r_dm.resize(n_groups);
   pl->r_members.resize(n_groups);
   pl->r_dms.resize(n_groups);
+  pl->r_group.resize(n_groups);
   static_assert(sizeof(hd_size) == sizeof(uint64_t), "hd_size must be 64-bit");
   HD_CHECK(hipMemcpyAsync(pl->r_peak.data(), pl->o_peak.p, n_groups * 4, hipMemcpyDeviceToHost, s));
   HD_CHECK(hipMemcpyAsync(pl->r_ind.data(), pl->o_ind.p, n_groups * 8, hipMemcpyDeviceToHost, s));
@@ -807,14 +955,22 @@ hd_error run_gulp(hd_pipeline pl, const hd_byte* d_fb, size_t nsamps, size_t nbi
   HD_CHECK(hipMemcpyAsync(pl->r_filter.data(), pl->o_filter.p, n_groups * 8, hipMemcpyDeviceToHost, s));
   HD_CHECK(hipMemcpyAsync(pl->r_dm.data(), pl->o_dm.p, n_groups * 8, hipMemcpyDeviceToHost, s));
   HD_CHECK(hipMemcpyAsync(pl->r_members.data(), pl->o_members.p, n_groups * 8, hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipMemcpyAsync(pl->r_group.data(), pl->o_group.p, n_groups * 4, hipMemcpyDeviceToHost, s));
   record(pl, 8);
   HD_CHECK(hipStreamSynchronize(s));
   for (size_t g2 = 0; g2 < n_groups; ++g2) pl->r_dms[g2] = pl->dm_list[pl->r_dm[g2]];  // :872-875
+  // candidates come out in root order, so each gulp's are one run
+  for (size_t k = 0; k < n_groups; ++k) {
+    GulpOut& go = pl->gulps[active[pl->r_group[k]]];
+    if (go.hi == 0) go.lo = k;
+    go.hi = k + 1;
+  }
   return HD_NO_ERROR;
 }
 
-hd_error finish_call(hd_pipeline pl, hd_error err, size_t nsamps, size_t nbeams,
-                     size_t first_idx, size_t processed, bool too_many) {
+// Timings, then each gulp's output in order; returns the first gulp's status
+// (the call's status for a single gulp).
+hd_error finish_call(hd_pipeline pl, hd_error err) {
   if (err != HD_NO_ERROR) return err;
   (void)hipEventSynchronize(pl->ev[8]);
   static const int stage_ev[ST_COUNT][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5},
@@ -824,12 +980,28 @@ hd_error finish_call(hd_pipeline pl, hd_error err, size_t nsamps, size_t nbeams,
     if (hipEventElapsedTime(&ms, pl->ev[stage_ev[k][0]], pl->ev[stage_ev[k][1]]) == hipSuccess)
       pl->timings[k] = ms;
   }
-  if (too_many) {  // no clustering, empty output (pipeline.cu:808)
-    pl->r_peak.clear(); pl->r_ind.clear(); pl->r_begin.clear(); pl->r_end.clear();
-    pl->r_filter.clear(); pl->r_dm.clear(); pl->r_members.clear(); pl->r_dms.clear();
+  hd_error first = HD_NO_ERROR;
+  for (size_t k = 0; k < pl->gulps.size(); ++k) {
+    GulpOut& go = pl->gulps[k];
+    if (go.err != HD_NO_ERROR) {
+      if (k == 0) first = go.err;
+      continue;
+    }
+    if (go.too_many) go.lo = go.hi = 0;  // no clustering, empty output (pipeline.cu:808)
+    hd_error e = emit_output(pl, go);
+    if (e != HD_NO_ERROR) return e;
+    if (go.too_many) go.err = HD_TOO_MANY_EVENTS;
+    if (k == 0) first = go.err;
+    if (pl->output_enabled) {
+      char buffer[64];
+      const hd_params& p = pl->params;
+      const time_t off = p.spectra_per_second ? (time_t)(go.first_idx / p.spectra_per_second) : 0;
+      time_t now = p.utc_start + off;
+      std::strftime(buffer, 64, "%Y-%m-%d-%H:%M:%S", std::gmtime(&now));
+      std::cout << "Dump: " << buffer << " Total Time: " << pl->timings[ST_TOTAL] * 1e-3
+                << std::endl;  // pipeline.cu:1081
+    }
   }
-  hd_error e = emit_output(pl, nsamps, nbeams, first_idx, processed);
-  if (e != HD_NO_ERROR) return e;
   if (pl->params.verbosity >= 1 && pl->output_enabled) {
     const double* t = pl->timings;
     std::cout << "0-DM cleaning time:      " << t[ST_CLEAN] * 1e-3 << "\n"
@@ -842,7 +1014,7 @@ hd_error finish_call(hd_pipeline pl, hd_error err, size_t nsamps, size_t nbeams,
               << "Process candidates time: " << t[ST_CLUSTER] * 1e-3 << "\n"
               << "Total time:              " << t[ST_TOTAL] * 1e-3 << std::endl;
   }
-  return too_many ? HD_TOO_MANY_EVENTS : HD_NO_ERROR;
+  return first;
 }
 
 }  // namespace
@@ -925,12 +1097,10 @@ extern "C" hd_error hd_execute(hd_pipeline pl, const hd_byte* h_filterbank, hd_s
   const size_t bytes = nsamps * pl->params.nchans * nbits / 8;
   HD_CHECK(pl->fb.ensure(bytes));
   HD_CHECK(hipMemcpyAsync(pl->fb.p, h_filterbank, bytes, hipMemcpyHostToDevice, pl->stream));
-  bool too_many = false;
-  size_t processed = 0;
-  hd_error err = run_gulp(pl, pl->fb.as<hd_byte>(), nsamps, nbits, first_idx, nbeams, &processed,
-                          &too_many);
-  *nsamps_processed = processed;
-  return finish_call(pl, err, nsamps, nbeams, first_idx, processed, too_many);
+  const std::vector<GulpIn> in{{pl->fb.as<hd_byte>(), nsamps, nbeams, first_idx, pl->params.beam}};
+  hd_error err = run_gulps(pl, in, nbits);
+  *nsamps_processed = pl->gulps.empty() ? 0 : pl->gulps[0].processed;
+  return finish_call(pl, err);
 }
 
 extern "C" hd_error hd_execute_device(hd_pipeline pl, const hd_byte* d_filterbank,
@@ -941,12 +1111,66 @@ extern "C" hd_error hd_execute_device(hd_pipeline pl, const hd_byte* d_filterban
   if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8)) return HD_INVALID_NBITS;
   if (nbeams == 0 || nsamps < nbeams) return HD_INVALID_STRIDE;
   DeviceGuard dg(pl->device);
-  bool too_many = false;
-  size_t processed = 0;
-  hd_error err = run_gulp(pl, d_filterbank, nsamps, nbits, first_idx, nbeams, &processed,
-                          &too_many);
-  *nsamps_processed = processed;
-  return finish_call(pl, err, nsamps, nbeams, first_idx, processed, too_many);
+  const std::vector<GulpIn> in{{d_filterbank, nsamps, nbeams, first_idx, pl->params.beam}};
+  hd_error err = run_gulps(pl, in, nbits);
+  *nsamps_processed = pl->gulps.empty() ? 0 : pl->gulps[0].processed;
+  return finish_call(pl, err);
+}
+
+extern "C" hd_error hd_execute_batch(hd_pipeline pl, const hd_gulp_desc* gulps, hd_size count,
+                                     hd_size nbits, int on_device, hd_size* nsamps_processed,
+                                     hd_error* errors) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (!gulps || !nsamps_processed) return HD_INVALID_POINTER;
+  if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8)) return HD_INVALID_NBITS;
+  for (hd_size k = 0; k < count; ++k) {
+    if (!gulps[k].filterbank) return HD_INVALID_POINTER;
+    if (gulps[k].nbeams == 0 || gulps[k].nsamps < gulps[k].nbeams) return HD_INVALID_STRIDE;
+  }
+  DeviceGuard dg(pl->device);
+  const size_t row = pl->params.nchans * nbits / 8;
+  std::vector<GulpIn> in(count);
+  if (!on_device) {
+    // one staging buffer for all gulps' host bytes
+    size_t total = 0;
+    for (hd_size k = 0; k < count; ++k) total += gulps[k].nsamps * row;
+    HD_CHECK(pl->fb.ensure(total));
+    size_t off = 0;
+    for (hd_size k = 0; k < count; ++k) {
+      const size_t b = gulps[k].nsamps * row;
+      HD_CHECK(hipMemcpyAsync(pl->fb.as<hd_byte>() + off, gulps[k].filterbank, b,
+                              hipMemcpyHostToDevice, pl->stream));
+      in[k] = {pl->fb.as<hd_byte>() + off, gulps[k].nsamps, gulps[k].nbeams, gulps[k].first_idx,
+               gulps[k].beam};
+      off += b;
+    }
+  } else {
+    for (hd_size k = 0; k < count; ++k)
+      in[k] = {gulps[k].filterbank, gulps[k].nsamps, gulps[k].nbeams, gulps[k].first_idx,
+               gulps[k].beam};
+  }
+  const hd_error err = run_gulps(pl, in, nbits);
+  // finish_call writes every gulp's output and sets its status in pl->gulps
+  if (err == HD_NO_ERROR) (void)finish_call(pl, HD_NO_ERROR);
+  hd_error first = err;
+  for (hd_size k = 0; k < count; ++k) {
+    const GulpOut go = pl->gulps.size() == count ? pl->gulps[k] : GulpOut{};
+    nsamps_processed[k] = go.processed;
+    if (errors) errors[k] = err != HD_NO_ERROR ? err : go.err;
+    if (first == HD_NO_ERROR && go.err != HD_NO_ERROR) first = go.err;
+  }
+  return first;
+}
+
+extern "C" hd_size hd_get_last_candidate_gulps(hd_pipeline pl, hd_size* gulp_index, hd_size cap) {
+  if (!pl) return 0;
+  const size_t n = pl->r_peak.size();
+  if (gulp_index) {
+    // candidates are in gulp order; each gulp holds the run [lo, hi)
+    for (size_t g = 0; g < pl->gulps.size(); ++g)
+      for (size_t k = pl->gulps[g].lo; k < pl->gulps[g].hi && k < cap; ++k) gulp_index[k] = g;
+  }
+  return n;
 }
 
 extern "C" void hd_destroy_pipeline(hd_pipeline pl) {

@@ -69,6 +69,33 @@ struct BlockRmsShape {
   size_t np;           // n / 5^L0 partial values
 };
 
+// One gulp of a batched call: its own input and sample count (the shape of a
+// separate hd_execute call), sharing the pipeline's plan (trial list,
+// channels, boxcar widths).  An array of these lives in device memory; the
+// kernels of a batched call index it with their group coordinate.  Rows of
+// the DM plane and of the prefix sums are (group, trial) pairs, row
+// g*n_dm + m, all with the same pitch.
+struct GroupDev {
+  SeriesShape sh;
+  BlockRmsShape bs;
+  size_t bc692;               // stride - (max_delay + boxcar_max)
+  const uint8_t* fb;          // input bytes in device memory
+  size_t nsamps;              // input time samples (all STF beams of the group)
+  unsigned n_w[kMaxWidths];   // fresh boxcar outputs per width
+  uint64_t t_off;             // clustering coordinate of the group's sample 0
+  uint64_t nsamps_beam;       // nsamps / nbeams
+};
+
+// Group arguments of a launch: gd == nullptr means one group described by
+// the launch's own shape arguments.
+struct GroupArgs {
+  const GroupDev* gd = nullptr;      // device table (G entries)
+  const GroupDev* host = nullptr;    // host mirror of the table (launch-time maxima)
+  int G = 1;
+  unsigned nbmax = 0;                // max STF beams per group (rows of the RMS table)
+  size_t prow = 0;                   // baseline points per row (max nbeams * sample_count)
+};
+
 // Fills the baseline and per-beam RMS fields of sh (nbeams, stride, nout and
 // bc set by the caller); returns an hd_error code.
 int series_shape(float cur_dt, float baseline_length, SeriesShape& sh);
@@ -94,7 +121,7 @@ hipError_t run_clean(const CleanPlan& cp, const uint8_t* d_in, uint8_t* d_out, v
 // ---- launchers (return hipError_t of the launch) ----
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
-                         hipStream_t s);
+                         hipStream_t s, const GroupArgs& ga = GroupArgs{});
 
 struct DedispGeom {
   int n_dm, n_live;
@@ -104,35 +131,43 @@ struct DedispGeom {
   int wlen;            // LDS dwords per staged copy (the largest block's need)
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
-                             const DedispGeom& g, float* plane, hipStream_t s);
+                             const DedispGeom& g, float* plane, hipStream_t s,
+                             const GroupArgs& ga = GroupArgs{});
 int dedisp_tile_time();     // outputs per block in time
 int dedisp_tile_dms();      // DMs per block
 int dedisp_window_dwords(int spread);  // staged dwords per copy for a delay span
 int dedisp_chunk_channels();           // channels staged per step
 
 hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
-                                  float* pts, hipStream_t s);
+                                  float* pts, hipStream_t s, const GroupArgs& ga = GroupArgs{});
 // Per-beam RMS of every series; with scratch (beam_rms_scratch_words floats)
 // large series take the two-phase kernels.
-size_t beam_rms_scratch_words(const SeriesShape& sh, int n_dm);
+size_t beam_rms_scratch_words(const SeriesShape& sh, int n_dm,
+                              const GroupArgs& ga = GroupArgs{});
 hipError_t launch_beam_rms(const float* plane, const float* pts, const SeriesShape& sh,
                            int n_dm, float* rms, hipStream_t s, float* scratch = nullptr,
-                           size_t scratch_words = 0);
+                           size_t scratch_words = 0, const GroupArgs& ga = GroupArgs{});
 hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
                             const SeriesShape& sh, int n_dm, hipStream_t s);
 // Prefix sums of n_series rows; with rms (and pts, sh) the rows are first
 // normalised as launch_normalise would.
+// With ga.gd the rows are the groups' (group, trial) rows, n_series trials each.
 hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series, float* S,
                        size_t spitch, hipStream_t s, const float* pts = nullptr,
-                       const float* rms = nullptr, const SeriesShape* sh = nullptr);
+                       const float* rms = nullptr, const SeriesShape* sh = nullptr,
+                       const GroupArgs& ga = GroupArgs{});
 
-hipError_t launch_block_rms_partials(const float* S, const SeriesShape& sh, const WidthTable& wt,
-                                     const BlockRmsShape& bs, int first_w, int n_w_group,
-                                     int n_dm, float* partial, const float* winv,
-                                     hipStream_t s);
-hipError_t launch_block_rms_finish(const float* partial, const BlockRmsShape& bs, int nw_total,
-                                   int first_w, int n_w_group, int n_dm, float* wrms,
-                                   float* winv, hipStream_t s);
+// Block RMS, thresholding and giant assembly run over the groups of ga
+// (ga.gd required); partials are laid out with np_max = max over groups.
+size_t block_rms_np_max(const GroupArgs& ga);
+hipError_t launch_block_rms_partials(const float* S, size_t spitch, const WidthTable& wt,
+                                     int first_w, int n_w_group, int n_dm, float* partial,
+                                     const float* winv, hipStream_t s, const GroupArgs& ga);
+// Without ga.gd, `single` gives the one shape (rows 0..n_dm-1).
+hipError_t launch_block_rms_finish(const float* partial, int nw_total, int first_w,
+                                   int n_w_group, int n_dm, float* wrms, float* winv,
+                                   hipStream_t s, const GroupArgs& ga,
+                                   const BlockRmsShape* single = nullptr);
 
 struct GiantBuffers {
   uint64_t* skey;      // (dm*NW+w)<<32 | i   (sample records, ordered later)
@@ -150,9 +185,9 @@ struct GiantBuffers {
   size_t nchunks_per_w;
   size_t chunk;        // outputs per chunk
 };
-hipError_t launch_threshold(const float* S, size_t spitch, const SeriesShape& sh,
-                            const WidthTable& wt, const float* winv, int n_dm, float thresh,
-                            size_t bc692, GiantBuffers& gb, hipStream_t s);
+hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
+                            const float* winv, int n_dm, float thresh, GiantBuffers& gb,
+                            hipStream_t s, const GroupArgs& ga);
 hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n,
                                      unsigned* tmp, hipStream_t s);
 size_t exclusive_scan_tmp_words(size_t n);
@@ -167,9 +202,11 @@ struct GiantOut {
   uint64_t* end;
   uint32_t* filter;
   uint32_t* dm;
+  uint32_t* group;     // group of each giant (nullable)
 };
 hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
-                                 unsigned ngiants, GiantOut go, uint64_t* best, hipStream_t s);
+                                 unsigned ngiants, int n_dm, GiantOut go, uint64_t* best,
+                                 hipStream_t s);
 
 // clustering / merge
 struct ClusterBuffers {
@@ -185,16 +222,20 @@ struct ClusterBuffers {
   unsigned* scan_tmp;
   size_t sort_n;
 };
+// With group (per giant) and gd, giants of different groups never coincide
+// and each group uses its own beam length; otherwise nsamps_beam applies.
 hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uint32_t* dm,
                           const uint64_t* members_in, unsigned n, uint64_t time_tol,
                           uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam,
                           unsigned max_filter, ClusterBuffers& cb, hipStream_t s,
-                          unsigned* h_flag_pinned, int* iterations);
+                          unsigned* h_flag_pinned, int* iterations,
+                          const uint32_t* group = nullptr, const GroupDev* gd = nullptr);
 hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* begin,
                         const uint64_t* end, const uint32_t* filter, const uint32_t* dm,
                         const uint64_t* members_in, unsigned n, ClusterBuffers& cb,
                         float* g_peak, uint64_t* g_ind, uint64_t* g_begin, uint64_t* g_end,
                         uint64_t* g_filter, uint64_t* g_dm, uint64_t* g_members,
-                        hipStream_t s);
+                        hipStream_t s, const uint32_t* group = nullptr,
+                        uint32_t* g_group = nullptr);
 
 }  // namespace hd

@@ -25,11 +25,20 @@ namespace hd {
 
 namespace {
 
-__global__ void k_sort_init(const uint64_t* __restrict__ ind, unsigned n, uint64_t* keys,
+// Clustering coordinate of giant k: its sample index, shifted by its group's
+// offset in a batched call so that groups lie further apart than any
+// coincidence window.
+__device__ __forceinline__ uint64_t tcoord(const uint64_t* ind, const uint32_t* group,
+                                           const GroupDev* gd, unsigned k) {
+  return group ? gd[group[k]].t_off + ind[k] : ind[k];
+}
+
+__global__ void k_sort_init(const uint64_t* __restrict__ ind, const uint32_t* __restrict__ group,
+                            const GroupDev* __restrict__ gd, unsigned n, uint64_t* keys,
                             size_t npad) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npad) return;
-  keys[k] = (k < n) ? ((ind[k] << 32) | (uint64_t)k) : ~0ull;
+  keys[k] = (k < n) ? ((tcoord(ind, group, gd, (unsigned)k) << 32) | (uint64_t)k) : ~0ull;
 }
 
 __global__ void k_bitonic_step(uint64_t* keys, size_t npad, size_t j, size_t kk) {
@@ -56,12 +65,13 @@ __device__ __forceinline__ unsigned lower_bound_ind(const uint64_t* keys, unsign
   return lo;
 }
 
-__global__ void k_windows(const uint64_t* __restrict__ ind, unsigned n,
+__global__ void k_windows(const uint64_t* __restrict__ ind, const uint32_t* __restrict__ group,
+                          const GroupDev* __restrict__ gd, unsigned n,
                           const uint64_t* __restrict__ keys, uint64_t span, uint32_t* lo,
                           uint32_t* hi, int32_t* labels) {
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t t = ind[i];
+  const uint64_t t = tcoord(ind, group, gd, i);
   lo[i] = lower_bound_ind(keys, n, t > span ? t - span : 0);
   hi[i] = lower_bound_ind(keys, n, t + span + 1);
   labels[i] = (int32_t)i;
@@ -82,14 +92,17 @@ __device__ __forceinline__ bool coincident(uint64_t ii, uint64_t ij, uint32_t fi
   return (uint64_t)dt <= tt && (uint64_t)df <= filter_tol && (uint64_t)dd <= dm_tol;
 }
 
+// The windows only hold giants of i's own group, which share its beam length.
 __global__ void k_relax(const uint64_t* __restrict__ ind, const uint32_t* __restrict__ filter,
                         const uint32_t* __restrict__ dm, unsigned n,
                         const uint64_t* __restrict__ keys, const uint32_t* __restrict__ lo,
                         const uint32_t* __restrict__ hi, uint64_t time_tol, uint64_t filter_tol,
-                        uint64_t dm_tol, uint64_t nsamps_beam, int32_t* labels,
-                        unsigned* flag) {
+                        uint64_t dm_tol, uint64_t nsamps_beam_one,
+                        const uint32_t* __restrict__ group, const GroupDev* __restrict__ gd,
+                        int32_t* labels, unsigned* flag) {
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const uint64_t nsamps_beam = group ? gd[group[i]].nsamps_beam : nsamps_beam_one;
   const int32_t cur = __atomic_load_n(&labels[i], __ATOMIC_RELAXED);
   int32_t best = cur;
   const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
@@ -139,9 +152,10 @@ __global__ void k_merge_write(unsigned n, const unsigned* __restrict__ root_flag
                               const uint64_t* __restrict__ best,
                               const unsigned* __restrict__ members, const float* peak,
                               const uint64_t* ind, const uint64_t* begin, const uint64_t* end,
-                              const uint32_t* filter, const uint32_t* dm, float* g_peak,
-                              uint64_t* g_ind, uint64_t* g_begin, uint64_t* g_end,
-                              uint64_t* g_filter, uint64_t* g_dm, uint64_t* g_members) {
+                              const uint32_t* filter, const uint32_t* dm, const uint32_t* group,
+                              float* g_peak, uint64_t* g_ind, uint64_t* g_begin, uint64_t* g_end,
+                              uint64_t* g_filter, uint64_t* g_dm, uint64_t* g_members,
+                              uint32_t* g_group) {
   const unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n || !root_flag[r]) return;
   const unsigned g = root_pos[r];
@@ -153,6 +167,7 @@ __global__ void k_merge_write(unsigned n, const unsigned* __restrict__ root_flag
   g_filter[g] = filter[src];
   g_dm[g] = dm[src];
   g_members[g] = members[r];
+  if (g_group) g_group[g] = group[src];
 }
 
 }  // namespace
@@ -161,22 +176,23 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
                           const uint64_t* members_in, unsigned n, uint64_t time_tol,
                           uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam,
                           unsigned max_filter, ClusterBuffers& cb, hipStream_t s,
-                          unsigned* h_flag_pinned, int* iterations) {
+                          unsigned* h_flag_pinned, int* iterations, const uint32_t* group,
+                          const GroupDev* gd) {
   (void)members_in;
   *iterations = 0;
   if (n == 0) return hipSuccess;
   size_t npad = 1;
   while (npad < n) npad <<= 1;
   const unsigned T = 256;
-  hipLaunchKernelGGL(k_sort_init, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s, ind, n,
-                     cb.sort_keys, npad);
+  hipLaunchKernelGGL(k_sort_init, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s, ind,
+                     group, gd, n, cb.sort_keys, npad);
   for (size_t kk = 2; kk <= npad; kk <<= 1)
     for (size_t j = kk >> 1; j > 0; j >>= 1)
       hipLaunchKernelGGL(k_bitonic_step, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s,
                          cb.sort_keys, npad, j, kk);
   const uint64_t span = time_tol << max_filter;
-  hipLaunchKernelGGL(k_windows, dim3((n + T - 1) / T), dim3(T), 0, s, ind, n, cb.sort_keys, span,
-                     cb.lo, cb.hi, cb.labels);
+  hipLaunchKernelGGL(k_windows, dim3((n + T - 1) / T), dim3(T), 0, s, ind, group, gd, n,
+                     cb.sort_keys, span, cb.lo, cb.hi, cb.labels);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   // relaxation passes run in batches between host checks of the change flag
@@ -186,7 +202,7 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     for (int b = 0; b < kBatch; ++b)
       hipLaunchKernelGGL(k_relax, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter, dm, n,
                          cb.sort_keys, cb.lo, cb.hi, time_tol, filter_tol, dm_tol, nsamps_beam,
-                         cb.labels, cb.flag);
+                         group, gd, cb.labels, cb.flag);
     e = hipMemcpyAsync(h_flag_pinned, cb.flag, sizeof(unsigned), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(s);
@@ -202,7 +218,7 @@ hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* 
                         const uint64_t* members_in, unsigned n, ClusterBuffers& cb,
                         float* g_peak, uint64_t* g_ind, uint64_t* g_begin, uint64_t* g_end,
                         uint64_t* g_filter, uint64_t* g_dm, uint64_t* g_members,
-                        hipStream_t s) {
+                        hipStream_t s, const uint32_t* group, uint32_t* g_group) {
   if (n == 0) return hipSuccess;
   const unsigned T = 256;
   const dim3 grid((n + T - 1) / T);
@@ -213,8 +229,8 @@ hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* 
   hipError_t e = launch_exclusive_scan_u32(cb.root_flag, cb.root_pos, n, cb.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_merge_write, grid, dim3(T), 0, s, n, cb.root_flag, cb.root_pos, cb.best,
-                     cb.members, peak, ind, begin, end, filter, dm, g_peak, g_ind, g_begin,
-                     g_end, g_filter, g_dm, g_members);
+                     cb.members, peak, ind, begin, end, filter, dm, group, g_peak, g_ind,
+                     g_begin, g_end, g_filter, g_dm, g_members, g_group);
   return hipGetLastError();
 }
 

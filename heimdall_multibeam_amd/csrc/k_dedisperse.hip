@@ -29,11 +29,18 @@ __device__ __forceinline__ unsigned unpack_sample(const uint8_t* row, int c, int
 }
 
 // 64 time x 64 channel tiles: reads along channels, writes along time.
+// blockIdx.z: the group (its input, sample count and rows of x).
 __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ fb,
                                                 size_t stride_bytes, int nbits, size_t nsamps,
                                                 const int* __restrict__ live, int n_live,
-                                                uint8_t* __restrict__ x, size_t xpitch) {
+                                                uint8_t* __restrict__ x, size_t xpitch,
+                                                const GroupDev* __restrict__ gd) {
   __shared__ uint8_t tile[64][68];
+  if (gd) {
+    fb = gd[blockIdx.z].fb;
+    nsamps = gd[blockIdx.z].nsamps;
+  }
+  x += (size_t)blockIdx.z * n_live * xpitch;
   const size_t t0 = (size_t)blockIdx.x * 64;
   const int l0 = blockIdx.y * 64;
   const int tx = threadIdx.x & 63;
@@ -79,9 +86,14 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
                                                        const int* __restrict__ delays,
                                                        const int* __restrict__ chunk_pairs,
                                                        DedispGeom g, int hw_max, int xcd_remap,
-                                                       float* __restrict__ out) {
+                                                       float* __restrict__ out,
+                                                       const GroupDev* __restrict__ gd) {
   extern __shared__ uint32_t lds16[];  // [D2_CC][2][hw_max]
   __shared__ int sdly[D2_DB][D2_CC];
+  // blockIdx.z: the group, with its own unpacked rows, output rows and length
+  const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
+  x += (size_t)blockIdx.z * g.n_live * g.xpitch;
+  out += (size_t)blockIdx.z * g.n_dm * g.ppitch;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
@@ -99,6 +111,7 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
   const int mb = (int)(lin % nbx);
   const int m0 = mb * D2_DB;
   const size_t t0 = (size_t)(lin / nbx) * D2_TB;
+  if (t0 > nout) return;  // past this group's series (and its zero at nout)
   const int* drow0 = delays + (size_t)m0 * g.n_live;
 
   uint32_t pa[2][D2_Q];
@@ -214,8 +227,8 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
       const size_t t = t0 + 2 * lane + 128 * q;
       if (t + 1 < g.ppitch) {
         float2 v;
-        v.x = (t < g.nout) ? (float)ua[a][q][0] : 0.f;
-        v.y = (t + 1 < g.nout) ? (float)ua[a][q][1] : 0.f;
+        v.x = (t < nout) ? (float)ua[a][q][0] : 0.f;
+        v.y = (t + 1 < nout) ? (float)ua[a][q][1] : 0.f;
         *reinterpret_cast<float2*>(orow + t) = v;
       }
     }
@@ -235,17 +248,19 @@ int dedisp_window_dwords(int spread) {
 
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
-                         hipStream_t s) {
-  dim3 grid((unsigned)((xpitch + 63) / 64), (unsigned)((n_live + 63) / 64));
+                         hipStream_t s, const GroupArgs& ga) {
+  dim3 grid((unsigned)((xpitch + 63) / 64), (unsigned)((n_live + 63) / 64), (unsigned)ga.G);
   hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, s, fb, stride_bytes, nbits, nsamps, d_live,
-                     n_live, x, xpitch);
+                     n_live, x, xpitch, ga.gd);
   return hipGetLastError();
 }
 
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
-                             const DedispGeom& g, float* plane, hipStream_t s) {
+                             const DedispGeom& g, float* plane, hipStream_t s,
+                             const GroupArgs& ga) {
   // DM blocks fastest: blocks resident together share one time tile's input
-  dim3 grid((unsigned)((g.n_dm + D2_DB - 1) / D2_DB), (unsigned)(g.ppitch / D2_TB));
+  dim3 grid((unsigned)((g.n_dm + D2_DB - 1) / D2_DB), (unsigned)(g.ppitch / D2_TB),
+            (unsigned)ga.G);
   const size_t lds = (size_t)D2_CC * 2 * (size_t)g.wlen * sizeof(uint32_t);
   if (lds > 160 * 1024) return hipErrorInvalidValue;  // delay span of a DM block too wide
   if (lds > 64 * 1024) {
@@ -258,7 +273,7 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
     return e ? atoi(e) : 1;
   }();
   hipLaunchKernelGGL(k_dedisperse16, grid, dim3(D2_BS), lds, s, x, d_delays, d_chunk_pairs, g,
-                     g.wlen, remap, plane);
+                     g.wlen, remap, plane, ga.gd);
   return hipGetLastError();
 }
 

@@ -44,11 +44,13 @@ __device__ __forceinline__ float raw_val(const float* S, const WidthInfo& wi, un
 // What the reference's filtered buffer holds at `pos` while width j is being
 // processed: fresh output below n_w, otherwise the normalised value left by
 // the latest earlier width that wrote `pos`.
+// nwv: the group's fresh output counts per width.
 __device__ __forceinline__ float filtered_at(const float* S, const WidthTable& wt,
-                                             const float* winv, int j, size_t pos) {
-  if (pos < wt.wi[j].n_w) return raw_val(S, wt.wi[j], wt.off, pos);
+                                             const unsigned* nwv, const float* winv, int j,
+                                             size_t pos) {
+  if (pos < nwv[j]) return raw_val(S, wt.wi[j], wt.off, pos);
   for (int jj = j - 1; jj >= 0; --jj)
-    if (pos < wt.wi[jj].n_w) return raw_val(S, wt.wi[jj], wt.off, pos) * winv[jj];
+    if (pos < nwv[jj]) return raw_val(S, wt.wi[jj], wt.off, pos) * winv[jj];
   return 0.f;
 }
 
@@ -82,15 +84,19 @@ __device__ float lds_finish_f(float* A, float* B, size_t n) {
 // the staged entries are every ts-th prefix sum, E[x] = S[off - behind +
 // ts*(pos0 + x)], so that F[pos] = E[pos - pos0 + w/ts] - E[pos - pos0]
 // (w = ahead + behind is a multiple of ts); up to tile_words of them.
+// blockIdx: (chunk, trial, group); rows g*n_dm + m.
 __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
-    const float* __restrict__ Sall, size_t spitch, WidthTable wt, BlockRmsShape bs, int first_w,
-    int n_group, int tile_ok, unsigned amax, unsigned bmax, unsigned tile_words,
-    const float* __restrict__ winv_all, float* __restrict__ partial) {
+    const float* __restrict__ Sall, size_t spitch, WidthTable wt, const GroupDev* __restrict__ gd,
+    size_t np_max, int first_w, int n_group, int tile_ok, unsigned amax, unsigned bmax,
+    unsigned tile_words, const float* __restrict__ winv_all, float* __restrict__ partial) {
   extern __shared__ float lds[];
+  const BlockRmsShape bs = gd[blockIdx.z].bs;
+  const unsigned* nwv = gd[blockIdx.z].n_w;
   const size_t c = blockIdx.x;
-  const int m = blockIdx.y;
-  const float* S = Sall + (size_t)m * spitch;
-  const float* winv = winv_all + (size_t)m * wt.nw;
+  if (c >= bs.np) return;
+  const size_t row = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
+  const float* S = Sall + row * spitch;
+  const float* winv = winv_all + row * wt.nw;
   const size_t k0 = c * bs.chunk;
   const size_t pos0 = reblock_pos((unsigned)k0, bs.bc, bs.delta);
   const size_t posl = reblock_pos((unsigned)(k0 + bs.chunk - 1), bs.bc, bs.delta);
@@ -99,15 +105,15 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
   // single decimated width: which width's boxcar this chunk reads
   int src_j = -1;
   if (!tile_ok && n_group == 1 && contiguous && tile_words > 0) {
-    const WidthInfo wj = wt.wi[first_w];
-    if (posl < wj.n_w) {
+    const unsigned nwj = nwv[first_w];
+    if (posl < nwj) {
       src_j = first_w;
-    } else if (pos0 >= wj.n_w) {
+    } else if (pos0 >= nwj) {
       int a = -1, b = -1;
       for (int jj = first_w - 1; jj >= 0 && a < 0; --jj)
-        if (pos0 < wt.wi[jj].n_w) a = jj;
+        if (pos0 < nwv[jj]) a = jj;
       for (int jj = first_w - 1; jj >= 0 && b < 0; --jj)
-        if (posl < wt.wi[jj].n_w) b = jj;
+        if (posl < nwv[jj]) b = jj;
       if (a == b) src_j = a;
     }
     if (src_j >= 0) {
@@ -170,31 +176,35 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     } else {
       value = chunk_value([&](size_t k) -> float {
         const size_t pos = reblock_pos((unsigned)k, bs.bc, bs.delta);
-        return fabsf(filtered_at(S, wt, winv, j, pos));
+        return fabsf(filtered_at(S, wt, nwv, winv, j, pos));
       });
     }
-    if (threadIdx.x == 0) partial[((size_t)m * wt.nw + j) * bs.np + c] = value;
+    if (threadIdx.x == 0) partial[(row * wt.nw + j) * np_max + c] = value;
     __syncthreads();
   }
 }
 
+// blockIdx: (width - first_w, trial, group)
 __global__ __launch_bounds__(64) void k_block_rms_finish(const float* __restrict__ partial,
-                                                         BlockRmsShape bs, int nw_total,
-                                                         int first_w, float* __restrict__ wrms,
+                                                         const GroupDev* __restrict__ gd,
+                                                         size_t np_one, size_t np_max,
+                                                         int nw_total, int first_w,
+                                                         float* __restrict__ wrms,
                                                          float* __restrict__ winv) {
   extern __shared__ float lds[];
+  const size_t np = gd ? gd[blockIdx.z].bs.np : np_one;
   const int j = first_w + blockIdx.x;
-  const int m = blockIdx.y;
-  const float* p = partial + ((size_t)m * nw_total + j) * bs.np;
+  const size_t row = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
+  const float* p = partial + (row * nw_total + j) * np_max;
   float* A = lds;
-  float* B = lds + bs.np;
-  for (size_t q = threadIdx.x; q < bs.np; q += 64) A[q] = p[q];
+  float* B = lds + np;
+  for (size_t q = threadIdx.x; q < np; q += 64) A[q] = p[q];
   __syncthreads();
-  const float med = lds_finish_f<64>(A, B, bs.np);
+  const float med = lds_finish_f<64>(A, B, np);
   if (threadIdx.x == 0) {
     const float r = (float)((double)med * 1.4862);  // get_rms.cu:127-128
-    wrms[(size_t)m * nw_total + j] = r;
-    winv[(size_t)m * nw_total + j] = 1.0f / r;      // get_rms.cu:195
+    wrms[row * nw_total + j] = r;
+    winv[row * nw_total + j] = 1.0f / r;            // get_rms.cu:195
   }
 }
 
@@ -224,16 +234,23 @@ constexpr int TH_CHUNK = FB_BS * TH_PER;  // 4096 prefix-sum positions per block
 // sums (plus the widest boxcar's halo) are staged in LDS once and every width
 // is thresholded from them.  Output i of a width belongs to the chunk holding
 // i*ts, so each (DM, width, chunk) key covers a time-ordered run of outputs.
+// blockIdx: (chunk, trial, group); rows g*n_dm + m.  Chunk keys are
+// (row*nw + width)*nchunks_per_w + chunk, so the compacted samples come out
+// in (group, trial, width, time) order.
 __global__ __launch_bounds__(FB_BS) void k_threshold(
-    const float* __restrict__ Sall, size_t spitch, size_t stride, size_t bc692, WidthTable wt,
-    unsigned amax, unsigned bmax, size_t s_len, const float* __restrict__ winv_all,
+    const float* __restrict__ Sall, size_t spitch, const GroupDev* __restrict__ gd,
+    WidthTable wt, unsigned amax, unsigned bmax, const float* __restrict__ winv_all,
     float thresh, GiantBuffers gb) {
   // T holds prefix sums with one pad word after every 16, so the 16-apart
   // positions that a wave's lanes read fall in different LDS banks.
   extern __shared__ float T[];  // (TH_CHUNK + amax + bmax) * 17/16
   const size_t c = blockIdx.x;
-  const int m = blockIdx.y;
-  const float* S = Sall + (size_t)m * spitch;
+  const size_t row = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
+  const size_t stride = gd[blockIdx.z].sh.stride;
+  const size_t bc692 = gd[blockIdx.z].bc692;
+  const size_t s_len = gd[blockIdx.z].sh.nout + 1;
+  const unsigned* nwv = gd[blockIdx.z].n_w;
+  const float* S = Sall + row * spitch;
   const size_t p0 = c * TH_CHUNK;                 // first position of the chunk
   const size_t tbase = p0 + wt.off - bmax;        // S index held in T[0]
   const size_t tlen = TH_CHUNK + amax + bmax;
@@ -245,12 +262,13 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   __shared__ unsigned sbase;
   for (int j = 0; j < wt.nw; ++j) {
     const WidthInfo wi = wt.wi[j];
-    const float inv = winv_all[(size_t)m * wt.nw + j];
+    const float inv = winv_all[row * wt.nw + j];
     // outputs i with i*ts in [p0, p0 + TH_CHUNK)
     const size_t i0 = (p0 + wi.ts - 1) / wi.ts;
     size_t i1 = (p0 + TH_CHUNK + wi.ts - 1) / wi.ts;
-    if (i1 > wi.n_w) i1 = wi.n_w;
-    const size_t key = ((size_t)m * wt.nw + j) * gb.nchunks_per_w + c;
+    const size_t n_w = nwv[j];
+    if (i1 > n_w) i1 = n_w;
+    const size_t key = (row * wt.nw + j) * gb.nchunks_per_w + c;
     if (i0 >= i1) {
       if (threadIdx.x == 0) gb.ccount[key] = 0;
       continue;
@@ -300,7 +318,7 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     const size_t base = sbase;
     if (mask && base + total <= gb.scap) {  // overflow: detected by the host
       unsigned rr = rank;
-      const uint64_t dmw = (uint64_t)m * wt.nw + j;
+      const uint64_t dmw = (uint64_t)row * wt.nw + j;
 #pragma unroll
       for (int q = 0; q < TH_PER; ++q) {
         if (mask & (1u << q)) {
@@ -382,8 +400,8 @@ __global__ void k_segment_heads(GiantBuffers gb, WidthTable wt, unsigned nsamp) 
 // (value, earliest position) to its giant with a 64-bit atomic max, which
 // keeps the first maximum (maximum_first); the head sample writes the
 // begin, the last sample of a run the end.
-__global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp, GiantOut go,
-                                  uint64_t* best) {
+__global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp, unsigned n_dm,
+                                  GiantOut go, uint64_t* best) {
   const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nsamp) return;
   const unsigned h = gb.head[k];
@@ -397,9 +415,11 @@ __global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp
   const uint64_t bk = ((uint64_t)float_key(gb.oval[k]) << 32) | (uint64_t)(0xffffffffu - k);
   atomicMax((unsigned long long*)&best[g], (unsigned long long)bk);
   if (h) {
+    const uint64_t row = dmw / (uint64_t)wt.nw;  // group * n_dm + trial
     go.begin[g] = (i + ro) * sc;
     go.filter[g] = wi.filter_idx;
-    go.dm[g] = (uint32_t)(dmw / (uint64_t)wt.nw);
+    go.dm[g] = (uint32_t)(row % n_dm);
+    if (go.group) go.group[g] = (uint32_t)(row / n_dm);
   }
   if (k + 1 == nsamp || gb.head[k + 1]) go.end[g] = (i + 1 + ro) * sc;
 }
@@ -422,10 +442,20 @@ __global__ void k_zero_u64(uint64_t* p, size_t n) {
 
 }  // namespace
 
-hipError_t launch_block_rms_partials(const float* S, const SeriesShape& sh, const WidthTable& wt,
-                                     const BlockRmsShape& bs, int first_w, int n_group, int n_dm,
-                                     float* partial, const float* winv, hipStream_t s) {
-  if (bs.np == 0 || n_group == 0) return hipSuccess;
+size_t block_rms_np_max(const GroupArgs& ga) {
+  size_t np = 0;
+  for (int g = 0; g < ga.G; ++g) np = np > ga.host[g].bs.np ? np : ga.host[g].bs.np;
+  return np;
+}
+
+hipError_t launch_block_rms_partials(const float* S, size_t spitch, const WidthTable& wt,
+                                     int first_w, int n_group, int n_dm, float* partial,
+                                     const float* winv, hipStream_t s, const GroupArgs& ga) {
+  if (!ga.gd || n_group == 0) return hipSuccess;
+  const size_t np_max = block_rms_np_max(ga);
+  if (np_max == 0) return hipSuccess;
+  size_t chunk = 0;
+  for (int g = 0; g < ga.G; ++g) chunk = chunk > ga.host[g].bs.chunk ? chunk : ga.host[g].bs.chunk;
   int tile_ok = 1;
   unsigned amax = 0, bmax = 0;
   for (int g = 0; g < n_group; ++g) {
@@ -434,42 +464,47 @@ hipError_t launch_block_rms_partials(const float* S, const SeriesShape& sh, cons
     if (wi.ahead > amax) amax = wi.ahead;
     if (wi.behind > bmax) bmax = wi.behind;
   }
-  const size_t tlen = bs.chunk + amax + bmax;
-  const size_t n1 = bs.chunk / 5;
+  const size_t tlen = chunk + amax + bmax;
+  const size_t n1 = chunk / 5;
   const size_t rest = (n1 ? n1 : 1) + n1 / 5 + 8;
   constexpr size_t kMaxLds = 65536 / sizeof(float);
   if (tlen + rest > kMaxLds) tile_ok = 0;  // keep LDS <= 64 KiB
   // single decimated width: room for the widest range it or an earlier width reads
   size_t tile_words = 0;
-  if (!tile_ok && n_group == 1 && bs.chunk > 0) {
+  if (!tile_ok && n_group == 1 && chunk > 0) {
     for (int jj = 0; jj <= first_w; ++jj) {
       const WidthInfo& w = wt.wi[jj];
-      const size_t span = bs.chunk + (w.ahead + w.behind) / w.ts;
+      const size_t span = chunk + (w.ahead + w.behind) / w.ts;
       if (span > tile_words) tile_words = span;
     }
     if (tile_words + rest > kMaxLds) tile_words = kMaxLds > rest ? kMaxLds - rest : 0;
   }
   size_t lds = ((tile_ok ? tlen : tile_words) + rest) * sizeof(float);
-  hipLaunchKernelGGL(k_block_rms_partials, dim3((unsigned)bs.np, n_dm), dim3(FB_BS), lds, s, S,
-                     sh.spitch, wt, bs, first_w, n_group, tile_ok, amax, bmax,
+  hipLaunchKernelGGL(k_block_rms_partials, dim3((unsigned)np_max, n_dm, ga.G), dim3(FB_BS), lds,
+                     s, S, spitch, wt, ga.gd, np_max, first_w, n_group, tile_ok, amax, bmax,
                      (unsigned)tile_words, winv, partial);
   return hipGetLastError();
 }
 
-hipError_t launch_block_rms_finish(const float* partial, const BlockRmsShape& bs, int nw_total,
-                                   int first_w, int n_group, int n_dm, float* wrms, float* winv,
-                                   hipStream_t s) {
-  if (n_group == 0) return hipSuccess;
-  const size_t lds = (bs.np + bs.np / 5 + 8) * sizeof(float);
+hipError_t launch_block_rms_finish(const float* partial, int nw_total, int first_w,
+                                   int n_group, int n_dm, float* wrms, float* winv,
+                                   hipStream_t s, const GroupArgs& ga,
+                                   const BlockRmsShape* single) {
+  if ((!ga.gd && !single) || n_group == 0) return hipSuccess;
+  const size_t np_max = ga.gd ? block_rms_np_max(ga) : single->np;
+  if (np_max == 0) return hipSuccess;
+  const size_t lds = (np_max + np_max / 5 + 8) * sizeof(float);
   if (lds > 65536) return hipErrorInvalidValue;  // pooled count beyond ~40M samples
-  hipLaunchKernelGGL(k_block_rms_finish, dim3(n_group, n_dm), dim3(64), lds, s, partial, bs,
-                     nw_total, first_w, wrms, winv);
+  hipLaunchKernelGGL(k_block_rms_finish, dim3(n_group, n_dm, ga.gd ? ga.G : 1), dim3(64), lds, s,
+                     partial, ga.gd, single ? single->np : (size_t)0, np_max, nw_total, first_w,
+                     wrms, winv);
   return hipGetLastError();
 }
 
-hipError_t launch_threshold(const float* S, size_t spitch, const SeriesShape& sh,
-                            const WidthTable& wt, const float* winv, int n_dm, float thresh,
-                            size_t bc692, GiantBuffers& gb, hipStream_t s) {
+hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
+                            const float* winv, int n_dm, float thresh, GiantBuffers& gb,
+                            hipStream_t s, const GroupArgs& ga) {
+  if (!ga.gd) return hipErrorInvalidValue;
   gb.chunk = TH_CHUNK;
   unsigned amax = 0, bmax = 0;
   for (int j = 0; j < wt.nw; ++j) {
@@ -479,9 +514,9 @@ hipError_t launch_threshold(const float* S, size_t spitch, const SeriesShape& sh
   const size_t tlen = (size_t)TH_CHUNK + amax + bmax;
   const size_t lds = (tlen + tlen / 16 + 1) * sizeof(float);
   if (lds > 65536) return hipErrorInvalidValue;  // boxcar_max beyond ~11000
-  dim3 grid((unsigned)gb.nchunks_per_w, n_dm);
-  hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), lds, s, S, spitch, sh.stride, bc692, wt,
-                     amax, bmax, sh.nout + 1, winv, thresh, gb);
+  dim3 grid((unsigned)gb.nchunks_per_w, n_dm, ga.G);
+  hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), lds, s, S, spitch, ga.gd, wt, amax, bmax,
+                     winv, thresh, gb);
   return hipGetLastError();
 }
 
@@ -524,12 +559,13 @@ hipError_t launch_segment_heads(const GiantBuffers& gb, const WidthTable& wt, un
 }
 
 hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
-                                 unsigned ngiants, GiantOut go, uint64_t* best, hipStream_t s) {
+                                 unsigned ngiants, int n_dm, GiantOut go, uint64_t* best,
+                                 hipStream_t s) {
   if (!nsamp || !ngiants) return hipSuccess;
   hipLaunchKernelGGL(k_zero_u64, dim3((ngiants + 255) / 256), dim3(256), 0, s, best,
                      (size_t)ngiants);
   hipLaunchKernelGGL(k_segment_scatter, dim3((nsamp + 255) / 256), dim3(256), 0, s, gb, wt,
-                     nsamp, go, best);
+                     nsamp, (unsigned)n_dm, go, best);
   hipLaunchKernelGGL(k_segment_finish, dim3((ngiants + 255) / 256), dim3(256), 0, s, gb, wt,
                      ngiants, go, best);
   return hipGetLastError();

@@ -14,6 +14,8 @@
 //              prefix sum in the reference's sequential order, one lane per
 //              series over row tiles staged through LDS so that HBM reads and
 //              writes stay coalesced.
+#include <algorithm>
+
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
 #include "hd_series.hpp"
@@ -54,13 +56,21 @@ __device__ float lds_finish(float* A, float* B, size_t n) {
   return result;
 }
 
+// blockIdx: (beam, trial, group); the group's rows are g*n_dm + m and each
+// row holds prow baseline points.
 __global__ __launch_bounds__(NB_BS) void k_baseline_points(const float* __restrict__ plane,
-                                                           SeriesShape sh,
-                                                           float* __restrict__ pts) {
+                                                           SeriesShape sh_arg,
+                                                           float* __restrict__ pts,
+                                                           const GroupDev* __restrict__ gd,
+                                                           size_t prow_arg) {
   extern __shared__ float lds[];
+  const SeriesShape sh = gd ? gd[blockIdx.z].sh : sh_arg;
   const unsigned b = blockIdx.x;
+  if (b >= sh.nbeams || sh.sc == 0) return;
   const int m = blockIdx.y;
-  const float* d = plane + (size_t)m * sh.ppitch;
+  const size_t row = (size_t)blockIdx.z * gridDim.y + m;
+  const size_t prow = gd ? prow_arg : (size_t)sh.nbeams * sh.sc;
+  const float* d = plane + row * sh.ppitch;
   const int Ld = sh.bl_direct;
   const size_t per = pow5(Ld);
   const size_t nl = sh.count_round / per;
@@ -80,7 +90,7 @@ __global__ __launch_bounds__(NB_BS) void k_baseline_points(const float* __restri
   };
   for (size_t j = threadIdx.x; j < nl; j += NB_BS) A[j] = tree_value(leaf, j * per, Ld);
   float* res = lds_pyramid<NB_BS>(A, B, (int)nl, sh.nscr - Ld);
-  float* o = pts + ((size_t)m * sh.nbeams + b) * sh.sc;
+  float* o = pts + row * prow + (size_t)b * sh.sc;
   for (size_t j = threadIdx.x; j < sh.sc; j += NB_BS) o[j] = res[j];
 }
 
@@ -126,23 +136,39 @@ __device__ __forceinline__ float leaf_abs_dev(const float* d, const float* P,
   return fabsf(d[pos] - baseline_at(P, sh, pos));
 }
 
+__device__ __forceinline__ int beam_rms_levels(size_t bc) {
+  int L0 = 0;
+  while (L0 < 5 && bc >= pow5(L0 + 1)) ++L0;
+  return L0;
+}
+
+// blockIdx: (chunk, trial, group * nbm + beam).  Rows g*n_dm + m; the RMS
+// table has nbm entries per row and the partials np_max per (row, beam).
 __global__ __launch_bounds__(NB_BS) void k_beam_rms_partials(const float* __restrict__ plane,
                                                              const float* __restrict__ pts,
-                                                             SeriesShape sh, int L0, size_t np,
+                                                             SeriesShape sh_arg, size_t np_max,
                                                              int stage_pts,
-                                                             float* __restrict__ partial) {
+                                                             float* __restrict__ partial,
+                                                             const GroupDev* __restrict__ gd,
+                                                             size_t prow_arg, unsigned nbm) {
   extern __shared__ float lds[];
-  const size_t c = blockIdx.x;
-  const int m = blockIdx.y;
-  const unsigned b = blockIdx.z;
+  const unsigned gz = blockIdx.z / nbm, b = blockIdx.z % nbm;
+  const SeriesShape sh = gd ? gd[gz].sh : sh_arg;
+  if (b >= sh.nbeams) return;
+  const int L0 = beam_rms_levels(sh.bc);
   const size_t chunk = pow5(L0);
-  const size_t prow = (size_t)sh.nbeams * sh.sc;
-  const float* d = plane + (size_t)m * sh.ppitch;
+  const size_t np = sh.bc / chunk;
+  const size_t c = blockIdx.x;
+  if (c >= np) return;
+  const int m = blockIdx.y;
+  const size_t row = (size_t)gz * gridDim.y + m;
+  const size_t prow = gd ? prow_arg : (size_t)sh.nbeams * sh.sc;
+  const float* d = plane + row * sh.ppitch;
   float* A = lds;
   float* B = lds + chunk;
   // the row's baseline points (and the float after them) staged in LDS
   float* sP = B + chunk / 5 + 8;
-  const float* P = pts + (size_t)m * prow;
+  const float* P = pts + row * prow;
   if (stage_pts) {
     for (size_t i = threadIdx.x; i <= prow; i += NB_BS) sP[i] = P[i];
     __syncthreads();
@@ -152,21 +178,26 @@ __global__ __launch_bounds__(NB_BS) void k_beam_rms_partials(const float* __rest
 #pragma unroll 4
   for (size_t i = threadIdx.x; i < chunk; i += NB_BS) A[i] = leaf_abs_dev(d, P, sh, k0 + i);
   float* res = lds_pyramid<NB_BS>(A, B, (int)chunk, L0);
-  if (threadIdx.x == 0) partial[((size_t)m * sh.nbeams + b) * np + c] = res[0];
+  if (threadIdx.x == 0) partial[(row * nbm + b) * np_max + c] = res[0];
 }
 
+// blockIdx: (beam, trial, group)
 __global__ __launch_bounds__(64) void k_beam_rms_finish(const float* __restrict__ partial,
-                                                        size_t np, unsigned nbeams,
-                                                        float* __restrict__ rms) {
+                                                        SeriesShape sh_arg, size_t np_max,
+                                                        unsigned nbm, float* __restrict__ rms,
+                                                        const GroupDev* __restrict__ gd) {
   extern __shared__ float lds[];
+  const SeriesShape sh = gd ? gd[blockIdx.z].sh : sh_arg;
   const unsigned b = blockIdx.x;
-  const int m = blockIdx.y;
-  const float* p = partial + ((size_t)m * nbeams + b) * np;
+  if (b >= sh.nbeams) return;
+  const size_t np = sh.bc / pow5(beam_rms_levels(sh.bc));
+  const size_t row = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
+  const float* p = partial + (row * nbm + b) * np_max;
   float* A = lds;
   float* B = lds + np;
   for (size_t i = threadIdx.x; i < np; i += 64) A[i] = p[i];
   const float med = lds_finish<64>(A, B, np);
-  if (threadIdx.x == 0) rms[(size_t)m * nbeams + b] = med * 1.4862f;  // get_rms.cu:94-97
+  if (threadIdx.x == 0) rms[row * nbm + b] = med * 1.4862f;  // get_rms.cu:94-97
 }
 
 // z = (x - baseline) / rms[pos / stride], in place (remove_baseline.cu:99-102,
@@ -225,32 +256,47 @@ constexpr int SC_RMAX = 1024;                    // staged beam RMS values per b
 // from global memory.
 template <bool STAGED>
 __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z, size_t zpitch,
-                                                     size_t count, int n_series,
+                                                     size_t count_arg, int n_series,
                                                      float* __restrict__ S, size_t spitch,
                                                      int s_vec_ok, const float* __restrict__ pts,
                                                      const float* __restrict__ rms,
-                                                     SeriesShape sh) {
+                                                     SeriesShape sh_arg,
+                                                     const GroupDev* __restrict__ gd,
+                                                     size_t prow_arg, unsigned nbm_arg) {
   __shared__ float buf[2][SC_ROWS][SC_T + SC_PAD];
   __shared__ float sP[STAGED ? SC_PMAX + 1 : 1];
   __shared__ float sR[STAGED ? SC_RMAX : 1];
   const int tid = threadIdx.x;
-  const int r0 = blockIdx.x * SC_ROWS;
+  // blocks: (group, SC_ROWS-row slice of its n_series rows); rows are
+  // g*n_series + r in z, S, pts and rms
+  const int bpg = (n_series + SC_ROWS - 1) / SC_ROWS;
+  const int gz = blockIdx.x / bpg;
+  const int r0 = (blockIdx.x % bpg) * SC_ROWS;
+  const SeriesShape sh = gd ? gd[gz].sh : sh_arg;
+  const size_t count = gd ? sh.nout : count_arg;
+  const size_t rowbase = (size_t)gz * n_series;
+  z += rowbase * zpitch;
+  S += rowbase * spitch;
+  const size_t prow = gd ? prow_arg : (size_t)sh.nbeams * sh.sc;  // baseline points per row
+  const unsigned nbm = gd ? nbm_arg : sh.nbeams;                   // RMS values per row
+  if (rms) {
+    pts += rowbase * prow;
+    rms += rowbase * nbm;
+  }
   const size_t ntiles = (count + SC_T - 1) / SC_T;
   const bool z_vec = (zpitch % 4 == 0) && ((reinterpret_cast<uintptr_t>(z) & 15) == 0);
-  const size_t prow = (size_t)sh.nbeams * sh.sc;  // baseline points per series
   if (STAGED) {
     // rows r0.. of pts (plus the float after them, which the stretch may
     // touch at the very end, as it would in global memory) and of rms
     const int nrow = min(SC_ROWS, n_series - r0);
     const size_t np = (size_t)nrow * prow + 1;
     for (size_t i = tid; i < np; i += SC_THREADS) sP[i] = pts[(size_t)r0 * prow + i];
-    for (size_t i = tid; i < (size_t)nrow * sh.nbeams; i += SC_THREADS)
-      sR[i] = rms[(size_t)r0 * sh.nbeams + i];
+    for (size_t i = tid; i < (size_t)nrow * nbm; i += SC_THREADS) sR[i] = rms[(size_t)r0 * nbm + i];
   }
 
   auto norm = [&](int row, size_t t, float x) -> float {
     const float* P = STAGED ? sP + (size_t)row * prow : pts + (size_t)(r0 + row) * prow;
-    const float* R = STAGED ? sR + (size_t)row * sh.nbeams : rms + (size_t)(r0 + row) * sh.nbeams;
+    const float* R = STAGED ? sR + (size_t)row * nbm : rms + (size_t)(r0 + row) * nbm;
     return normalised_at(x, P, R, sh, t);
   };
   auto load_tile = [&](size_t tile, int b) {
@@ -398,39 +444,74 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
 }  // namespace
 
 hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
-                                  float* pts, hipStream_t s) {
-  if (sh.sc == 0) return hipSuccess;
-  const size_t nl = sh.count_round / pow5_host(sh.bl_direct);
-  const size_t lds = (nl + nl / 5 + 8) * sizeof(float);
-  hipLaunchKernelGGL(k_baseline_points, dim3(sh.nbeams, n_dm), dim3(NB_BS), lds, s, plane, sh,
-                     pts);
+                                  float* pts, hipStream_t s, const GroupArgs& ga) {
+  size_t nl_max = 0;
+  unsigned nb = sh.nbeams;
+  if (ga.gd) {
+    nb = ga.nbmax;
+    for (int g = 0; g < ga.G; ++g) {
+      const SeriesShape& q = ga.host[g].sh;
+      if (q.sc) nl_max = std::max(nl_max, q.count_round / pow5_host(q.bl_direct));
+    }
+  } else if (sh.sc) {
+    nl_max = sh.count_round / pow5_host(sh.bl_direct);
+  }
+  if (nl_max == 0) return hipSuccess;  // no baseline anywhere (sample_count 0)
+  const size_t lds = (nl_max + nl_max / 5 + 8) * sizeof(float);
+  hipLaunchKernelGGL(k_baseline_points, dim3(nb, n_dm, (unsigned)ga.G), dim3(NB_BS), lds, s,
+                     plane, sh, pts, ga.gd, ga.prow);
   return hipGetLastError();
 }
 
-size_t beam_rms_scratch_words(const SeriesShape& sh, int n_dm) {
+namespace {
+int rms_levels_host(size_t bc) {
   int L0 = 0;
-  while (L0 < 5 && sh.bc >= pow5_host(L0 + 1)) ++L0;
-  return (size_t)n_dm * sh.nbeams * (sh.bc / pow5_host(L0));
+  while (L0 < 5 && bc >= pow5_host(L0 + 1)) ++L0;
+  return L0;
+}
+// largest partial count and chunk over the groups (or the one shape)
+void rms_maxima(const SeriesShape& sh, const GroupArgs& ga, size_t* np_max, size_t* chunk_max) {
+  *np_max = 0;
+  *chunk_max = 0;
+  const int G = ga.gd ? ga.G : 1;
+  for (int g = 0; g < G; ++g) {
+    const SeriesShape& q = ga.gd ? ga.host[g].sh : sh;
+    const size_t chunk = pow5_host(rms_levels_host(q.bc));
+    *np_max = std::max(*np_max, q.bc / chunk);
+    *chunk_max = std::max(*chunk_max, chunk);
+  }
+}
+}  // namespace
+
+size_t beam_rms_scratch_words(const SeriesShape& sh, int n_dm, const GroupArgs& ga) {
+  size_t np_max, chunk_max;
+  rms_maxima(sh, ga, &np_max, &chunk_max);
+  const size_t nbm = ga.gd ? ga.nbmax : sh.nbeams;
+  return (size_t)n_dm * ga.G * nbm * np_max;
 }
 
 hipError_t launch_beam_rms(const float* plane, const float* pts, const SeriesShape& sh,
                            int n_dm, float* rms, hipStream_t s, float* scratch,
-                           size_t scratch_words) {
-  int L0 = 0;
-  while (L0 < 5 && sh.bc >= pow5_host(L0 + 1)) ++L0;
-  const size_t np = sh.bc / pow5_host(L0);
-  if (scratch && L0 >= 3 && (size_t)n_dm * sh.nbeams * np <= scratch_words &&
-      (np + np / 5 + 8) * sizeof(float) <= 65536) {
-    const size_t chunk = pow5_host(L0);
-    const size_t prow = (size_t)sh.nbeams * sh.sc;
-    const int stage_pts = (pts && sh.sc > 0 && prow + 1 <= 4096) ? 1 : 0;
-    const size_t lds = (chunk + chunk / 5 + 8 + (stage_pts ? prow + 1 : 0)) * sizeof(float);
-    hipLaunchKernelGGL(k_beam_rms_partials, dim3((unsigned)np, n_dm, sh.nbeams), dim3(NB_BS),
-                       lds, s, plane, pts, sh, L0, np, stage_pts, scratch);
-    hipLaunchKernelGGL(k_beam_rms_finish, dim3(sh.nbeams, n_dm), dim3(64),
-                       (np + np / 5 + 8) * sizeof(float), s, scratch, np, sh.nbeams, rms);
+                           size_t scratch_words, const GroupArgs& ga) {
+  size_t np_max, chunk_max;
+  rms_maxima(sh, ga, &np_max, &chunk_max);
+  const unsigned nbm = ga.gd ? ga.nbmax : sh.nbeams;
+  const bool fits = (size_t)n_dm * ga.G * nbm * np_max <= scratch_words &&
+                    (np_max + np_max / 5 + 8) * sizeof(float) <= 65536;
+  if (scratch && fits && (ga.gd || chunk_max >= 125)) {
+    const size_t prow = ga.gd ? ga.prow : (size_t)sh.nbeams * sh.sc;
+    const int stage_pts = (pts && prow > 0 && prow + 1 <= 4096) ? 1 : 0;
+    const size_t lds =
+        (chunk_max + chunk_max / 5 + 8 + (stage_pts ? prow + 1 : 0)) * sizeof(float);
+    hipLaunchKernelGGL(k_beam_rms_partials, dim3((unsigned)np_max, n_dm, ga.G * nbm),
+                       dim3(NB_BS), lds, s, plane, pts, sh, np_max, stage_pts, scratch, ga.gd,
+                       ga.prow, nbm);
+    hipLaunchKernelGGL(k_beam_rms_finish, dim3(nbm, n_dm, ga.G), dim3(64),
+                       (np_max + np_max / 5 + 8) * sizeof(float), s, scratch, sh, np_max, nbm,
+                       rms, ga.gd);
     return hipGetLastError();
   }
+  if (ga.gd) return hipErrorInvalidValue;  // batched calls always bring scratch
   const size_t nl = sh.bc / pow5_host(sh.rms_direct);
   const size_t lds = (nl + nl / 5 + 8) * sizeof(float);
   hipLaunchKernelGGL(k_beam_rms, dim3(sh.nbeams, n_dm), dim3(NB_BS), lds, s, plane, pts, sh,
@@ -449,19 +530,23 @@ hipError_t launch_normalise(float* plane, const float* pts, const float* rms,
 
 hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series, float* S,
                        size_t spitch, hipStream_t s, const float* pts, const float* rms,
-                       const SeriesShape* sh) {
+                       const SeriesShape* sh, const GroupArgs& ga) {
   const int s_vec_ok = (spitch % 4 == 0) && (((reinterpret_cast<uintptr_t>(S) + 4) & 15) == 0);
   SeriesShape none{};
-  if (rms && !sh) return hipErrorInvalidValue;
-  const dim3 grid((n_series + SC_ROWS - 1) / SC_ROWS);
-  const bool staged = rms && (size_t)SC_ROWS * sh->nbeams * sh->sc + 1 <= (size_t)SC_PMAX &&
-                      (size_t)SC_ROWS * sh->nbeams <= (size_t)SC_RMAX;
+  if (rms && !sh && !ga.gd) return hipErrorInvalidValue;
+  const SeriesShape one = sh ? *sh : none;
+  const int bpg = (n_series + SC_ROWS - 1) / SC_ROWS;
+  const dim3 grid((unsigned)(bpg * ga.G));
+  const size_t prow = ga.gd ? ga.prow : (size_t)one.nbeams * one.sc;
+  const unsigned nbm = ga.gd ? ga.nbmax : one.nbeams;
+  const bool staged = rms && (size_t)SC_ROWS * prow + 1 <= (size_t)SC_PMAX &&
+                      (size_t)SC_ROWS * nbm <= (size_t)SC_RMAX;
   if (staged)
     hipLaunchKernelGGL(k_scan<true>, grid, dim3(SC_THREADS), 0, s, z, zpitch, count, n_series, S,
-                       spitch, s_vec_ok, pts, rms, *sh);
+                       spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
   else
     hipLaunchKernelGGL(k_scan<false>, grid, dim3(SC_THREADS), 0, s, z, zpitch, count, n_series,
-                       S, spitch, s_vec_ok, pts, rms, rms ? *sh : none);
+                       S, spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
   return hipGetLastError();
 }
 

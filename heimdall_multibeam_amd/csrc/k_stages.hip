@@ -306,7 +306,8 @@ extern "C" hd_error hd_block_rms(const hd_float* d_data, hd_float* d_block_rms,
   hipLaunchKernelGGL(k_block_rms_direct, dim3((unsigned)bs.np, (unsigned)n_series), dim3(ST_BS),
                      lds, s, d_data, series_stride, bs, partial);
   ST_CHECK(hipGetLastError());
-  ST_CHECK(launch_block_rms_finish(partial, bs, 1, 0, 1, (int)n_series, d_block_rms, winv, s));
+  ST_CHECK(launch_block_rms_finish(partial, 1, 0, 1, (int)n_series, d_block_rms, winv, s,
+                                   GroupArgs{}, &bs));
   return HD_NO_ERROR;
 }
 
@@ -423,7 +424,7 @@ extern "C" hd_error hd_find_giants(const hd_float* d_data, hd_size count, hd_siz
   ST_CHECK(tmp.get(&go.dm, ng));
   uint64_t* best;
   ST_CHECK(tmp.get(&best, ng));
-  ST_CHECK(launch_segment_reduce(gb, wt, nsamp, (unsigned)ng, go, best, s));
+  ST_CHECK(launch_segment_reduce(gb, wt, nsamp, (unsigned)ng, 1, go, best, s));
   const size_t k = std::min<size_t>(ng, out->capacity);
   if (k) {
     if (out->peaks) ST_CHECK(hipMemcpyAsync(out->peaks, go.peak, k * 4, hipMemcpyDeviceToDevice, s));

@@ -91,7 +91,17 @@ CONFIGS = {
     "full_1024ch": Workload("full_1024ch", 1024, 8, 64e-6, 1517.8359375, -0.328125, 262144, 0.0,
                             2000.0, 4096, 6.0, 50, (0.0, 2000.0), (1.0, 512.0), (6.0, 50.0),
                             0.05, 10.0, 3, 20.0),
+    # ragged multibeam batch: 320 ch over 835-865 MHz, 655.36 us, per-beam gulps of
+    # 4096-16384 samples (nsamps below is the longest), 240-320 live channels with
+    # the rest held at the noise median, 0-3 pulses per beam, DM 0-2000, boxcars 1..1024
+    "batch_320ch": Workload("batch_320ch", 320, 8, 655.36e-6, 864.953125, -0.09375, 16384, 0.0,
+                            2000.0, 1024, 6.0, 3, (0.0, 2000.0), (1.0, 64.0), (6.0, 40.0)),
 }
+
+# beams and per-beam ranges of the ragged batch
+BATCH_BEAMS = 352
+BATCH_NSAMPS = (4096, 16384)
+BATCH_LIVE = (240, 320)
 
 
 def trial_dms(w: Workload, dm_tol: float = 1.25, pulse_width_us: float = 40.0) -> np.ndarray:
@@ -103,6 +113,27 @@ def trial_dms(w: Workload, dm_tol: float = 1.25, pulse_width_us: float = 40.0) -
     out = np.zeros(n, np.float32)
     L.hd_synth_dm_list(out.ctypes.data_as(fp), n, w.dm_min, w.dm_max, w.dt, pulse_width_us,
                        w.f0, w.df, w.nchans, dm_tol)
+    return out
+
+
+def make_batch(w: Workload, seed: int, nbeams: int, nsamps_range=BATCH_NSAMPS,
+               live_range=BATCH_LIVE, dm_trials=None):
+    """Ragged beams: list of (bytes, nsamps) with per-beam gulp lengths drawn
+    from nsamps_range, live-channel counts from live_range (the others held at
+    the noise median, not masked) and 0..w.n_pulses pulses each."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if dm_trials is None:
+        dm_trials = trial_dms(w)
+    out = []
+    for b in range(nbeams):
+        ns = int(rng.integers(nsamps_range[0], nsamps_range[1] + 1))
+        n_live = int(rng.integers(live_range[0], live_range[1] + 1))
+        dead = np.zeros(w.nchans, np.int32)
+        dead[rng.choice(w.nchans, w.nchans - n_live, replace=False)] = 1
+        npul = int(rng.integers(0, w.n_pulses + 1))
+        fb = make_gulp(w, seed * 100003 + b, nsamps=ns, dm_trials=dm_trials, dead=dead,
+                       dead_value=w.mean, n_pulses=npul)
+        out.append((fb, ns))
     return out
 
 

@@ -37,6 +37,32 @@ hd_error hd_execute_device(hd_pipeline pl, const hd_byte* d_filterbank, hd_size 
                            hd_size nbits, hd_size first_idx, hd_size nbeams,
                            hd_size* nsamps_processed);
 
+/* One gulp of a batched call: what a separate hd_execute call would get. */
+typedef struct hd_gulp_desc {
+  const hd_byte* filterbank;  /* host or device bytes, [nsamps][nchans*nbits/8] */
+  hd_size nsamps;             /* time samples (all STF beams of this gulp) */
+  hd_size nbeams;             /* STF beams in this gulp (1 for a ragged beam batch) */
+  hd_size first_idx;          /* index of the gulp's first sample in its stream */
+  hd_size beam;               /* params.beam of the separate call (output beam number) */
+} hd_gulp_desc;
+
+/* Searches `count` gulps of different lengths in one call: one launch per
+ * stage over all of them.  Each gulp's candidates, .cand file and status are
+ * those of a separate hd_execute(pl, filterbank, nsamps, nbits, first_idx,
+ * nbeams) with params.beam = beam.  The reference interface takes one gulp
+ * per call (Pipeline/hd/pipeline.h:21-23); this entry point has no
+ * counterpart there.  on_device: the filterbank pointers are device memory.
+ * nsamps_processed (count entries) receives each gulp's processed count and
+ * errors (count entries, may be NULL) each gulp's status; the return value is
+ * a call error, else the first gulp error in order, else HD_NO_ERROR. */
+hd_error hd_execute_batch(hd_pipeline pl, const hd_gulp_desc* gulps, hd_size count,
+                          hd_size nbits, int on_device, hd_size* nsamps_processed,
+                          hd_error* errors);
+
+/* Gulp index (position in the last batch) of each candidate returned by
+ * hd_get_last_candidates; candidates are in gulp order.  Returns the count. */
+hd_size hd_get_last_candidate_gulps(hd_pipeline pl, hd_size* gulp_index, hd_size cap);
+
 /* Number of DM trials and the maximum delay (samples) of the plan. */
 hd_size hd_get_dm_count(hd_pipeline pl);
 hd_size hd_get_max_delay(hd_pipeline pl);
