@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=4,
                     help="CPU baseline: time every k-th DM trial of the same gulp")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--beams", type=int, default=0,
+                    help="batch workloads: number of beams (default: the workload's)")
+    ap.add_argument("--cpu-beams", type=int, default=16,
+                    help="batch workloads: beams timed for the CPU baseline")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="before the gulps, run one device-to-device copy of this many MiB "
                          "(the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -79,6 +83,99 @@ def cpu_baseline(w, fb, stride, n_dm, beam_s):
                       % (stride, n_dm, n_sub, dt)}
 
 
+def batch_main(a):
+    """Ragged multibeam batch: one step = one hd_execute_batch over all beams,
+    inputs resident in HBM."""
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import hip, synth
+    w = synth.CONFIGS[a.workload]
+    nb = a.beams or synth.BATCH_BEAMS
+    print("generating %d ragged beams of %s (seed %d)" % (nb, w.name, a.seed), file=sys.stderr,
+          flush=True)
+    beams = synth.make_batch(w, a.seed, nb)
+    params = H.default_params(nchans=w.nchans, dt=w.dt, f0=w.f0, df=w.df, dm_min=w.dm_min,
+                              dm_max=w.dm_max, boxcar_max=w.boxcar_max,
+                              detect_thresh=w.detect_thresh, use_scrunching=False)
+    pl = H.Pipeline(params)
+    pl.set_output(False)
+    n_dm = len(pl.dm_list())
+    max_delay = pl.max_delay()
+    total = sum(fb.nbytes for fb, _ in beams)
+    dbuf = hip.DeviceBuffer(total)
+    host = np.concatenate([fb for fb, _ in beams])
+    dbuf.upload(host)
+    gulps, off = [], 0
+    for k, (fb, ns) in enumerate(beams):
+        gulps.append((dbuf.address + off, ns, 1, 0, k))
+        off += fb.nbytes
+    beam_s = sum(max(ns - w.boxcar_max - max_delay, 0) for _, ns in beams) * w.dt
+    for i in range(a.warmup):
+        pl.execute_batch(gulps, w.nbits, device=True)
+        print("warmup %d done" % i, file=sys.stderr, flush=True)
+    hip.synchronize()
+    stage = {k: [] for k in H.Pipeline.STAGES}
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        pl.execute_batch(gulps, w.nbits, device=True)
+        for k, v in pl.timings().items():
+            stage[k].append(v)
+    hip.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_cand = len(pl.candidates()["peak"])
+    n_giants = len(pl.giants()["peak"])
+    value = beam_s * a.steps / elapsed
+    t_dd = float(np.mean(stage["dedisperse"])) * 1e-3
+    nout = sum(max(ns - max_delay, 0) for _, ns in beams)
+    alg = total + 4 * n_dm * nout
+    adds = float(n_dm) * nout * w.nchans
+    out = {
+        "metric": "beam-seconds searched/s/GPU at DM 0-2000 (ragged multibeam batch)",
+        "value": round(value, 3), "unit": "beam-seconds searched/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(elapsed / a.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8 samples, exact u32 sums, f32 search",
+        "data": "synthetic (seeded Gaussian noise, 0-3 dispersed pulses per beam, dead "
+                "channels held at the noise median)",
+        "config": {"workload": "%s: %d beams, %d ch x %d-bit, fch1 %.4f MHz, foff %.6f MHz, "
+                               "tsamp %g us, gulps %d-%d samples, DM %g-%g (%d trials, max "
+                               "delay %d), boxcars 1..%d"
+                               % (w.name, nb, w.nchans, w.nbits, w.f0, w.df, w.dt * 1e6,
+                                  synth.BATCH_NSAMPS[0], synth.BATCH_NSAMPS[1], w.dm_min,
+                                  w.dm_max, n_dm, max_delay, w.boxcar_max),
+                   "beam_seconds_per_step": round(beam_s, 3), "candidates": n_cand,
+                   "giants": n_giants,
+                   "parallelism": "1 GPU"},
+        "stage_ms": {k: round(float(np.mean(v)), 4) for k, v in stage.items() if v},
+        "roofline": {"bound": "hbm", "achieved": round(alg / t_dd / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / t_dd / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_dedisperse", "kernel_ms": round(t_dd * 1e3, 4),
+                     "algorithmic_bytes": alg, "int_adds_per_s": adds / t_dd,
+                     "valu_int32_frac": round(adds / t_dd / VALU_INT_PEAK, 4)},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        from oracle import oracle as O
+        nthreads = min(16, os.cpu_count() or 1)
+        p = O.make_params(nchans=w.nchans, dt=w.dt, f0=w.f0, df=w.df, dm_min=w.dm_min,
+                          dm_max=w.dm_max, boxcar_max=w.boxcar_max,
+                          detect_thresh=w.detect_thresh, nthreads=nthreads)
+        sample = beams[:a.cpu_beams]
+        t0 = time.perf_counter()
+        for fb, ns in sample:
+            O.execute(p, fb, ns, w.nbits, 0, 1, want_text=False)
+        dt = time.perf_counter() - t0
+        bs = sum(max(ns - w.boxcar_max - max_delay, 0) for _, ns in sample) * w.dt
+        out["cpu_baseline"] = {"value": bs / dt, "unit": "beam-seconds searched/s",
+                               "cores": nthreads, "kind": "port",
+                               "sample": "CPU oracle on the first %d beams of the batch, one "
+                                         "call per beam (%.1f s wall)" % (len(sample), dt)}
+    pl.close()
+    dbuf.free()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -86,6 +183,8 @@ def main():
               "(beams or DM ranges would shard one process per GPU); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if a.workload.startswith("batch"):
+        return batch_main(a)
     import heimdall_multibeam_amd as H
     from heimdall_multibeam_amd import hip, synth
     w = synth.CONFIGS[a.workload]
@@ -126,6 +225,7 @@ def main():
             stage[k].append(v)
         n_cand = len(pl.candidates()["peak"])
     hip.synchronize()
+    n_giants = len(pl.giants()["peak"])
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / a.steps * 1e3
     value = beam_s * a.steps / elapsed
@@ -163,6 +263,7 @@ def main():
                                   w.dm_min, w.dm_max, n_dm, max_delay, w.boxcar_max,
                                   w.detect_thresh),
                    "beam_seconds_per_gulp": round(beam_s, 4), "candidates": n_cand,
+                   "giants": n_giants,
                    "parallelism": "1 GPU"},
         "stage_ms": stages_ms,
         "roofline": roof,

@@ -103,6 +103,7 @@ struct hd_pipeline_t {
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
   DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm, g_best;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
+  DevBuf c_sind, c_sfd;
   DevBuf o_peak, o_ind, o_begin, o_end, o_filter, o_dm, o_members;
   DevBuf clean_out, clean_work;
   DevBuf gdev, g_group, o_group;        // group table, group of each giant / candidate
@@ -710,7 +711,8 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   const size_t xpitch = round_up(ppitch + max_delay + 2 * pl->wlen + 256, 256);
   HD_CHECK(pl->x.ensure((size_t)G * std::max(n_live, 1) * xpitch));
   HD_CHECK(hd::launch_unpack(nullptr, p.nchans * nbits / 8, (int)nbits, 0, pl->live.as<int>(),
-                             n_live, pl->x.as<uint8_t>(), xpitch, s, ga));
+                             n_live, pl->x.as<uint8_t>(), xpitch, s, ga,
+                             (size_t)n_live == p.nchans));
   record(pl, 1);
   // dedisperse
   HD_CHECK(pl->plane.ensure(rows * ppitch * sizeof(float)));
@@ -895,8 +897,10 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   size_t npad = 1;
   while (npad < ng) npad <<= 1;
   HD_CHECK(pl->c_keys.ensure(npad * sizeof(uint64_t)));
-  HD_CHECK(pl->c_lo.ensure(ng * sizeof(uint32_t)));
-  HD_CHECK(pl->c_hi.ensure(ng * sizeof(uint32_t)));
+  // window bounds: one pair per giant, or one per filter band
+  const unsigned nbands = (unsigned)std::min<size_t>(2 * p.cand_sep_filter + 1, 16);
+  HD_CHECK(pl->c_lo.ensure((size_t)ng * nbands * sizeof(uint32_t)));
+  HD_CHECK(pl->c_hi.ensure((size_t)ng * nbands * sizeof(uint32_t)));
   HD_CHECK(pl->c_labels.ensure(ng * sizeof(int32_t)));
   HD_CHECK(pl->c_flag.ensure(sizeof(unsigned) * 4));
   HD_CHECK(pl->c_best.ensure(ng * sizeof(uint64_t)));
@@ -904,18 +908,21 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   HD_CHECK(pl->c_rflag.ensure(ng * sizeof(unsigned)));
   HD_CHECK(pl->c_rpos.ensure(ng * sizeof(unsigned)));
   HD_CHECK(pl->c_tmp.ensure(hd::exclusive_scan_tmp_words(ng) * sizeof(unsigned)));
+  HD_CHECK(pl->c_sind.ensure(ng * sizeof(uint64_t)));
+  HD_CHECK(pl->c_sfd.ensure(ng * sizeof(uint32_t)));
   hd::ClusterBuffers cb{pl->c_keys.as<uint64_t>(), pl->c_lo.as<uint32_t>(), pl->c_hi.as<uint32_t>(),
                         pl->c_labels.as<int32_t>(), pl->c_flag.as<unsigned>(),
                         pl->c_best.as<uint64_t>(), pl->c_members.as<unsigned>(),
                         pl->c_rflag.as<unsigned>(), pl->c_rpos.as<unsigned>(),
-                        pl->c_tmp.as<unsigned>(), npad};
+                        pl->c_tmp.as<unsigned>(), npad, pl->c_sind.as<uint64_t>(),
+                        pl->c_sfd.as<uint32_t>(), nbands};
   const unsigned max_filter = wt.wi[wt.nw - 1].filter_idx;
   int iters = 0;
   HD_CHECK(hd::launch_cluster(pl->g_ind.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
                               pl->g_dm.as<uint32_t>(), nullptr, ng, p.cand_sep_time,
                               p.cand_sep_filter, p.cand_sep_dm, pl->hgd[0].nsamps_beam,
                               max_filter, cb, s, pl->h_pinned, &iters,
-                              pl->g_group.as<uint32_t>(), ga.gd));
+                              pl->g_group.as<uint32_t>(), ga.gd, t_off));
   HD_CHECK(pl->o_peak.ensure(ng * sizeof(float)));
   HD_CHECK(pl->o_ind.ensure(ng * sizeof(uint64_t)));
   HD_CHECK(pl->o_begin.ensure(ng * sizeof(uint64_t)));
@@ -1186,7 +1193,8 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->c_lo, &pl->c_hi, &pl->c_labels, &pl->c_flag, &pl->c_best,
                       &pl->c_members, &pl->c_rflag, &pl->c_rpos, &pl->c_tmp, &pl->o_peak,
                       &pl->o_ind, &pl->o_begin, &pl->o_end, &pl->o_filter, &pl->o_dm,
-                      &pl->o_members};
+                      &pl->o_members, &pl->gdev, &pl->g_group, &pl->o_group, &pl->c_sind,
+                      &pl->c_sfd};
     for (DevBuf* b : bufs) b->release();
     for (hipEvent_t& ev : pl->ev)
       if (ev) (void)hipEventDestroy(ev);

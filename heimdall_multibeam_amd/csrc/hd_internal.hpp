@@ -119,9 +119,11 @@ hipError_t run_clean(const CleanPlan& cp, const uint8_t* d_in, uint8_t* d_out, v
                      hipStream_t s);
 
 // ---- launchers (return hipError_t of the launch) ----
+// live_is_identity: live channel l is channel l (no channel masked).
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
-                         hipStream_t s, const GroupArgs& ga = GroupArgs{});
+                         hipStream_t s, const GroupArgs& ga = GroupArgs{},
+                         bool live_is_identity = false);
 
 struct DedispGeom {
   int n_dm, n_live;
@@ -221,6 +223,11 @@ struct ClusterBuffers {
   unsigned* root_pos;
   unsigned* scan_tmp;
   size_t sort_n;
+  // giants' fields in the sorted order (optional): the window scans then read
+  // contiguous memory
+  uint64_t* s_ind = nullptr;
+  uint32_t* s_fd = nullptr;    // dm << 5 | filter
+  unsigned band_cap = 0;       // lo/hi entries per giant (filter-banded windows)
 };
 // With group (per giant) and gd, giants of different groups never coincide
 // and each group uses its own beam length; otherwise nsamps_beam applies.
@@ -229,7 +236,8 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
                           uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam,
                           unsigned max_filter, ClusterBuffers& cb, hipStream_t s,
                           unsigned* h_flag_pinned, int* iterations,
-                          const uint32_t* group = nullptr, const GroupDev* gd = nullptr);
+                          const uint32_t* group = nullptr, const GroupDev* gd = nullptr,
+                          uint64_t max_coord = ~0ull);
 hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* begin,
                         const uint64_t* end, const uint32_t* filter, const uint32_t* dm,
                         const uint64_t* members_in, unsigned n, ClusterBuffers& cb,

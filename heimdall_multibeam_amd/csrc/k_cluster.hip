@@ -124,6 +124,199 @@ __global__ void k_relax(const uint64_t* __restrict__ ind, const uint32_t* __rest
   }
 }
 
+// The same relaxation reading the window from copies of ind, filter and dm
+// laid out in the sorted order (s_ind, s_fd = dm << 5 | filter).
+__global__ void k_relax_sorted(const uint64_t* __restrict__ ind,
+                               const uint32_t* __restrict__ filter,
+                               const uint32_t* __restrict__ dm, unsigned n,
+                               const uint64_t* __restrict__ keys,
+                               const uint64_t* __restrict__ s_ind,
+                               const uint32_t* __restrict__ s_fd,
+                               const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+                               uint64_t time_tol, uint64_t filter_tol, uint64_t dm_tol,
+                               uint64_t nsamps_beam_one, const uint32_t* __restrict__ group,
+                               const GroupDev* __restrict__ gd, int32_t* labels,
+                               unsigned* flag) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t nsamps_beam = group ? gd[group[i]].nsamps_beam : nsamps_beam_one;
+  const int32_t cur = __atomic_load_n(&labels[i], __ATOMIC_RELAXED);
+  int32_t best = cur;
+  const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
+  if (jump < best) best = jump;
+  const uint64_t ii = ind[i];
+  const uint32_t fi = filter[i], di = dm[i];
+  const unsigned p1 = hi[i];
+  for (unsigned p = lo[i]; p < p1; ++p) {
+    const unsigned j = (unsigned)(keys[p] & 0xffffffffull);
+    if (j >= i) continue;
+    const uint32_t fd = s_fd[p];
+    if (!coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol, dm_tol,
+                    nsamps_beam))
+      continue;
+    const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
+    if (lj < best) best = lj;
+  }
+  if (best < cur) {
+    __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
+    *flag = 1u;
+  }
+}
+
+__global__ void k_gather_sorted(const uint64_t* __restrict__ keys, unsigned n,
+                                const uint64_t* __restrict__ ind,
+                                const uint32_t* __restrict__ filter,
+                                const uint32_t* __restrict__ dm, uint64_t* __restrict__ s_ind,
+                                uint32_t* __restrict__ s_fd) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned k = (unsigned)(keys[p] & 0xffffffffull);
+  s_ind[p] = ind[k];
+  s_fd[p] = (dm[k] << 5) | (filter[k] & 31u);
+}
+
+// Filter-banded windows: the giants are sorted by (filter, coordinate), and
+// giant i scans, for each filter band fb within filter_tol of its own, only
+// the coordinates within time_tol * 2^max(fi, fb) -- the exact time window
+// of a pair -- instead of the window of the widest filter.
+constexpr int kMaxBands = 16;
+__global__ void k_sort_init_banded(const uint64_t* __restrict__ ind,
+                                   const uint32_t* __restrict__ filter,
+                                   const uint32_t* __restrict__ group,
+                                   const GroupDev* __restrict__ gd, unsigned n, uint64_t* keys,
+                                   size_t npad) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npad) return;
+  keys[k] = (k < n) ? (((uint64_t)filter[k] << 59) |
+                       (tcoord(ind, group, gd, (unsigned)k) << 32) | (uint64_t)k)
+                    : ~0ull;
+}
+
+__device__ __forceinline__ unsigned lower_bound_key(const uint64_t* keys, unsigned n,
+                                                    uint64_t v) {
+  unsigned lo = 0, hi = n;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi) >> 1;
+    if (keys[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_windows_banded(const uint64_t* __restrict__ ind,
+                                 const uint32_t* __restrict__ filter,
+                                 const uint32_t* __restrict__ group,
+                                 const GroupDev* __restrict__ gd, unsigned n,
+                                 const uint64_t* __restrict__ keys, uint64_t time_tol,
+                                 unsigned filter_tol, unsigned max_filter, uint32_t* lo,
+                                 uint32_t* hi, int32_t* labels) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t t = tcoord(ind, group, gd, i);
+  const unsigned fi = filter[i];
+  const unsigned nb = 2 * filter_tol + 1;
+  for (unsigned b = 0; b < nb; ++b) {
+    const long long fb = (long long)fi - (long long)filter_tol + (long long)b;
+    uint32_t l = 0, h = 0;
+    if (fb >= 0 && fb <= (long long)max_filter) {
+      const unsigned fm = fi > (unsigned)fb ? fi : (unsigned)fb;
+      const uint64_t tol = time_tol << fm;
+      const uint64_t base = (uint64_t)fb << 59;
+      const uint64_t t_lo = t > tol ? t - tol : 0;
+      l = lower_bound_key(keys, n, base | (t_lo << 32));
+      h = lower_bound_key(keys, n, base | ((t + tol + 1) << 32));
+    }
+    lo[(size_t)i * nb + b] = l;
+    hi[(size_t)i * nb + b] = h;
+  }
+  labels[i] = (int32_t)i;
+}
+
+__global__ void k_relax_banded(const uint64_t* __restrict__ ind,
+                               const uint32_t* __restrict__ filter,
+                               const uint32_t* __restrict__ dm, unsigned n, unsigned nb,
+                               const uint64_t* __restrict__ keys,
+                               const uint64_t* __restrict__ s_ind,
+                               const uint32_t* __restrict__ s_fd,
+                               const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi,
+                               uint64_t time_tol, uint64_t filter_tol, uint64_t dm_tol,
+                               uint64_t nsamps_beam_one, const uint32_t* __restrict__ group,
+                               const GroupDev* __restrict__ gd, int32_t* labels,
+                               unsigned* flag) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t nsamps_beam = group ? gd[group[i]].nsamps_beam : nsamps_beam_one;
+  const int32_t cur = __atomic_load_n(&labels[i], __ATOMIC_RELAXED);
+  int32_t best = cur;
+  const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
+  if (jump < best) best = jump;
+  const uint64_t ii = ind[i];
+  const uint32_t fi = filter[i], di = dm[i];
+  for (unsigned b = 0; b < nb; ++b) {
+    const unsigned p1 = hi[(size_t)i * nb + b];
+    for (unsigned p = lo[(size_t)i * nb + b]; p < p1; ++p) {
+      const unsigned j = (unsigned)(keys[p] & 0xffffffffull);
+      if (j >= i) continue;
+      const uint32_t fd = s_fd[p];
+      if (!coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol, dm_tol,
+                      nsamps_beam))
+        continue;
+      const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
+      if (lj < best) best = lj;
+    }
+  }
+  if (best < cur) {
+    __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
+    *flag = 1u;
+  }
+}
+
+// The banded relaxation with one wave per giant: the lanes share the window
+// entries (contiguous reads) and the minimum is reduced across the wave.
+__global__ __launch_bounds__(256) void k_relax_banded_wave(
+    const uint64_t* __restrict__ ind, const uint32_t* __restrict__ filter,
+    const uint32_t* __restrict__ dm, unsigned n, unsigned nb, const uint64_t* __restrict__ keys,
+    const uint64_t* __restrict__ s_ind, const uint32_t* __restrict__ s_fd,
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, uint64_t time_tol,
+    uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam_one,
+    const uint32_t* __restrict__ group, const GroupDev* __restrict__ gd, int32_t* labels,
+    unsigned* flag) {
+  const unsigned i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const unsigned lane = threadIdx.x & 63;
+  if (i >= n) return;  // whole wave
+  const uint64_t nsamps_beam = group ? gd[group[i]].nsamps_beam : nsamps_beam_one;
+  const int32_t cur = __atomic_load_n(&labels[i], __ATOMIC_RELAXED);
+  int32_t best = cur;
+  if (lane == 0) {
+    const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
+    if (jump < best) best = jump;
+  }
+  const uint64_t ii = ind[i];
+  const uint32_t fi = filter[i], di = dm[i];
+  for (unsigned b = 0; b < nb; ++b) {
+    const unsigned p1 = hi[(size_t)i * nb + b];
+    for (unsigned p = lo[(size_t)i * nb + b] + lane; p < p1; p += 64) {
+      const unsigned j = (unsigned)(keys[p] & 0xffffffffull);
+      if (j >= i) continue;
+      const uint32_t fd = s_fd[p];
+      if (!coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol, dm_tol,
+                      nsamps_beam))
+        continue;
+      const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
+      if (lj < best) best = lj;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t other = __shfl_xor(best, o);
+    best = other < best ? other : best;
+  }
+  if (lane == 0 && best < cur) {
+    __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
+    *flag = 1u;
+  }
+}
+
 __global__ void k_clear_flag(unsigned* flag) { *flag = 0u; }
 
 __global__ void k_merge_init(unsigned n, const int32_t* __restrict__ labels, uint64_t* best,
@@ -177,32 +370,64 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
                           uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam,
                           unsigned max_filter, ClusterBuffers& cb, hipStream_t s,
                           unsigned* h_flag_pinned, int* iterations, const uint32_t* group,
-                          const GroupDev* gd) {
+                          const GroupDev* gd, uint64_t max_coord) {
   (void)members_in;
   *iterations = 0;
   if (n == 0) return hipSuccess;
   size_t npad = 1;
   while (npad < n) npad <<= 1;
   const unsigned T = 256;
-  hipLaunchKernelGGL(k_sort_init, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s, ind,
-                     group, gd, n, cb.sort_keys, npad);
+  // banded windows need the sorted copies, a packed filter and coordinates
+  // below 2^27 (the key holds filter:5 | coordinate:27 | index:32), and
+  // per-giant window arrays of nb entries (lo/hi sized by the caller)
+  const unsigned nb = (unsigned)(2 * filter_tol + 1);
+  const bool banded = cb.s_ind && cb.s_fd && cb.band_cap >= nb && max_filter < 32 &&
+                      max_coord < (1ull << 27);
+  if (banded) {
+    hipLaunchKernelGGL(k_sort_init_banded, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s,
+                       ind, filter, group, gd, n, cb.sort_keys, npad);
+  } else {
+    hipLaunchKernelGGL(k_sort_init, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s, ind,
+                       group, gd, n, cb.sort_keys, npad);
+  }
   for (size_t kk = 2; kk <= npad; kk <<= 1)
     for (size_t j = kk >> 1; j > 0; j >>= 1)
       hipLaunchKernelGGL(k_bitonic_step, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s,
                          cb.sort_keys, npad, j, kk);
-  const uint64_t span = time_tol << max_filter;
-  hipLaunchKernelGGL(k_windows, dim3((n + T - 1) / T), dim3(T), 0, s, ind, group, gd, n,
-                     cb.sort_keys, span, cb.lo, cb.hi, cb.labels);
+  if (banded) {
+    hipLaunchKernelGGL(k_windows_banded, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter,
+                       group, gd, n, cb.sort_keys, time_tol, (unsigned)filter_tol, max_filter,
+                       cb.lo, cb.hi, cb.labels);
+  } else {
+    const uint64_t span = time_tol << max_filter;
+    hipLaunchKernelGGL(k_windows, dim3((n + T - 1) / T), dim3(T), 0, s, ind, group, gd, n,
+                       cb.sort_keys, span, cb.lo, cb.hi, cb.labels);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  // the sorted copies need filter < 32 and dm < 2^27 to pack
+  const bool sorted = cb.s_ind && cb.s_fd && max_filter < 32;
+  if (sorted)
+    hipLaunchKernelGGL(k_gather_sorted, dim3((n + T - 1) / T), dim3(T), 0, s, cb.sort_keys, n,
+                       ind, filter, dm, cb.s_ind, cb.s_fd);
   // relaxation passes run in batches between host checks of the change flag
   constexpr int kBatch = 4;
   for (int it = 0; it < 100000; it += kBatch) {
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, cb.flag);
-    for (int b = 0; b < kBatch; ++b)
-      hipLaunchKernelGGL(k_relax, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter, dm, n,
-                         cb.sort_keys, cb.lo, cb.hi, time_tol, filter_tol, dm_tol, nsamps_beam,
-                         group, gd, cb.labels, cb.flag);
+    for (int b = 0; b < kBatch; ++b) {
+      if (banded)
+        hipLaunchKernelGGL(k_relax_banded_wave, dim3((n + 3) / 4), dim3(256), 0, s, ind, filter,
+                           dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi, time_tol,
+                           filter_tol, dm_tol, nsamps_beam, group, gd, cb.labels, cb.flag);
+      else if (sorted)
+        hipLaunchKernelGGL(k_relax_sorted, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter, dm,
+                           n, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi, time_tol,
+                           filter_tol, dm_tol, nsamps_beam, group, gd, cb.labels, cb.flag);
+      else
+        hipLaunchKernelGGL(k_relax, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter, dm, n,
+                           cb.sort_keys, cb.lo, cb.hi, time_tol, filter_tol, dm_tol,
+                           nsamps_beam, group, gd, cb.labels, cb.flag);
+    }
     e = hipMemcpyAsync(h_flag_pinned, cb.flag, sizeof(unsigned), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(s);

@@ -61,6 +61,59 @@ __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ fb,
   }
 }
 
+// 8-bit samples with every channel live: 64 time x 256 channel tiles moved as
+// 32-bit words both ways (four channels per word read, four time samples per
+// word written).
+constexpr int UP_T = 64, UP_C = 256;
+__global__ __launch_bounds__(256) void k_unpack8(const uint8_t* __restrict__ fb,
+                                                 size_t stride_bytes, size_t nsamps, int nchans,
+                                                 uint8_t* __restrict__ x, size_t xpitch,
+                                                 const GroupDev* __restrict__ gd) {
+  __shared__ uint32_t tile[UP_T][UP_C / 4 + 1];
+  if (gd) {
+    fb = gd[blockIdx.z].fb;
+    nsamps = gd[blockIdx.z].nsamps;
+  }
+  x += (size_t)blockIdx.z * nchans * xpitch;
+  const size_t t0 = (size_t)blockIdx.x * UP_T;
+  const int c0 = blockIdx.y * UP_C;
+  const int tid = threadIdx.x;
+  const bool words = (stride_bytes % 4 == 0) && (reinterpret_cast<uintptr_t>(fb) % 4 == 0);
+#pragma unroll
+  for (int k = 0; k < UP_T * UP_C / 4 / 256; ++k) {
+    const int idx = tid + 256 * k;
+    const int r = idx / (UP_C / 4), dw = idx % (UP_C / 4);
+    const size_t t = t0 + r;
+    const int c = c0 + 4 * dw;
+    uint32_t v = 0;
+    if (t < nsamps && c < nchans) {
+      const uint8_t* row = fb + t * stride_bytes;
+      if (words && c + 3 < nchans) {
+        v = *reinterpret_cast<const uint32_t*>(row + c);
+      } else {
+        for (int i = 0; i < 4 && c + i < nchans; ++i) v |= (uint32_t)row[c + i] << (8 * i);
+      }
+    }
+    tile[r][dw] = v;
+  }
+  __syncthreads();
+  const uint8_t* tb = reinterpret_cast<const uint8_t*>(&tile[0][0]);
+  constexpr int RB = (UP_C / 4 + 1) * 4;  // bytes per tile row
+#pragma unroll
+  for (int k = 0; k < UP_T * UP_C / 4 / 256; ++k) {
+    const int idx = tid + 256 * k;
+    const int cc = idx / (UP_T / 4), q = idx % (UP_T / 4);
+    const int c = c0 + cc;
+    const size_t t = t0 + 4 * q;
+    if (c >= nchans || t >= xpitch) continue;
+    const uint32_t v = (uint32_t)tb[(4 * q + 0) * RB + cc] |
+                       ((uint32_t)tb[(4 * q + 1) * RB + cc] << 8) |
+                       ((uint32_t)tb[(4 * q + 2) * RB + cc] << 16) |
+                       ((uint32_t)tb[(4 * q + 3) * RB + cc] << 24);
+    *reinterpret_cast<uint32_t*>(x + (size_t)c * xpitch + t) = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Dedispersion, 16-bit staging.  A 512-thread block owns 16 consecutive DM
 // trials x 1024 output samples; wave w sums trials w and w+8.  Per channel
@@ -82,6 +135,7 @@ constexpr int D2_PF = 6;                  // staging items prefetched per thread
 // chunk_pairs[block][chunk]: staged E/O dword pairs each channel of a
 // 4-channel chunk needs for this block of trials (host-computed from the
 // delay spans, so top-of-band chunks stage short windows).
+template <int DB>
 __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restrict__ x,
                                                        const int* __restrict__ delays,
                                                        const int* __restrict__ chunk_pairs,
@@ -89,7 +143,8 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
                                                        float* __restrict__ out,
                                                        const GroupDev* __restrict__ gd) {
   extern __shared__ uint32_t lds16[];  // [D2_CC][2][hw_max]
-  __shared__ int sdly[D2_DB][D2_CC];
+  constexpr int NA = DB / D2_WAVES;  // trials per wave
+  __shared__ int sdly[DB][D2_CC];
   // blockIdx.z: the group, with its own unpacked rows, output rows and length
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
   x += (size_t)blockIdx.z * g.n_live * g.xpitch;
@@ -109,15 +164,15 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
     lin = x * q + min(x, r) + lin / 8;
   }
   const int mb = (int)(lin % nbx);
-  const int m0 = mb * D2_DB;
+  const int m0 = mb * DB;
   const size_t t0 = (size_t)(lin / nbx) * D2_TB;
   if (t0 > nout) return;  // past this group's series (and its zero at nout)
   const int* drow0 = delays + (size_t)m0 * g.n_live;
 
-  uint32_t pa[2][D2_Q];
-  uint32_t ua[2][D2_Q][2];
+  uint32_t pa[NA][D2_Q];
+  uint32_t ua[NA][D2_Q][2];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int q = 0; q < D2_Q; ++q) {
       pa[a][q] = 0;
@@ -169,7 +224,7 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
   prefetch(0);
   int pf_delay = 0;
   auto prefetch_delay = [&](int k) {
-    if (k < nchunk && tid < D2_DB * D2_CC) {
+    if (k < nchunk && tid < DB * D2_CC) {
       const int c = min(k * D2_CC + tid % D2_CC, g.n_live - 1);
       const int mm = min(m0 + tid / D2_CC, g.n_dm - 1);
       pf_delay = delays[(size_t)mm * g.n_live + c] - (drow0[c] & ~3);
@@ -181,7 +236,7 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
     const int ncc = min(D2_CC, g.n_live - c0);
     const int pairs = cpairs[k];
     __syncthreads();
-    if (tid < D2_DB * D2_CC) sdly[tid / D2_CC][tid % D2_CC] = pf_delay;
+    if (tid < DB * D2_CC) sdly[tid / D2_CC][tid % D2_CC] = pf_delay;
     if (scc < ncc) {
 #pragma unroll
       for (int r = 0; r < D2_PF; ++r) {
@@ -197,7 +252,7 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
     prefetch_delay(k + 1);
     for (int cc = 0; cc < ncc; ++cc) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
+      for (int a = 0; a < NA; ++a) {
         const int o = sdly[wave + D2_WAVES * a][cc];
         const uint32_t* src = lds16 + (cc * 2 + (o & 1)) * hw_max + (o >> 1) + lane;
 #pragma unroll
@@ -207,7 +262,7 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
     pending += ncc;
     if (pending > 256 - D2_CC || c0 + D2_CC >= g.n_live) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int q = 0; q < D2_Q; ++q) {
           ua[a][q][0] += pa[a][q] & 0xFFFFu;
@@ -218,7 +273,7 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
     }
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
+  for (int a = 0; a < NA; ++a) {
     const int m = m0 + wave + D2_WAVES * a;
     if (m >= g.n_dm) continue;
     float* orow = out + (size_t)m * g.ppitch;
@@ -238,7 +293,15 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
 }  // namespace
 
 int dedisp_tile_time() { return D2_TB; }
-int dedisp_tile_dms() { return D2_DB; }
+// Trials per block: 16 (default) or 32 (HD_DEDISP_TRIALS=32); the host plans
+// its per-block staging windows with the same value.
+int dedisp_tile_dms() {
+  static const int db = [] {
+    const char* e = getenv("HD_DEDISP_TRIALS");
+    return (e && atoi(e) == 32) ? 32 : D2_DB;
+  }();
+  return db;
+}
 int dedisp_chunk_channels() { return D2_CC; }
 int dedisp_window_dwords(int spread) {
   // E/O copies: dword (o>>1) + 511 with o <= spread + 3, rounded to 64
@@ -248,7 +311,14 @@ int dedisp_window_dwords(int spread) {
 
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
-                         hipStream_t s, const GroupArgs& ga) {
+                         hipStream_t s, const GroupArgs& ga, bool live_is_identity) {
+  if (nbits == 8 && live_is_identity && xpitch % 4 == 0) {
+    dim3 grid((unsigned)((xpitch + UP_T - 1) / UP_T), (unsigned)((n_live + UP_C - 1) / UP_C),
+              (unsigned)ga.G);
+    hipLaunchKernelGGL(k_unpack8, grid, dim3(256), 0, s, fb, stride_bytes, nsamps, n_live, x,
+                       xpitch, ga.gd);
+    return hipGetLastError();
+  }
   dim3 grid((unsigned)((xpitch + 63) / 64), (unsigned)((n_live + 63) / 64), (unsigned)ga.G);
   hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, s, fb, stride_bytes, nbits, nsamps, d_live,
                      n_live, x, xpitch, ga.gd);
@@ -259,21 +329,28 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
                              const DedispGeom& g, float* plane, hipStream_t s,
                              const GroupArgs& ga) {
   // DM blocks fastest: blocks resident together share one time tile's input
-  dim3 grid((unsigned)((g.n_dm + D2_DB - 1) / D2_DB), (unsigned)(g.ppitch / D2_TB),
+  const int db = dedisp_tile_dms();
+  dim3 grid((unsigned)((g.n_dm + db - 1) / db), (unsigned)(g.ppitch / D2_TB),
             (unsigned)ga.G);
   const size_t lds = (size_t)D2_CC * 2 * (size_t)g.wlen * sizeof(uint32_t);
   if (lds > 160 * 1024) return hipErrorInvalidValue;  // delay span of a DM block too wide
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_dedisperse16),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute(
+        db == 32 ? reinterpret_cast<const void*>(&k_dedisperse16<32>)
+                 : reinterpret_cast<const void*>(&k_dedisperse16<D2_DB>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   static const int remap = [] {
     const char* e = getenv("HD_DEDISP_XCD_REMAP");
     return e ? atoi(e) : 1;
   }();
-  hipLaunchKernelGGL(k_dedisperse16, grid, dim3(D2_BS), lds, s, x, d_delays, d_chunk_pairs, g,
-                     g.wlen, remap, plane, ga.gd);
+  if (db == 32)
+    hipLaunchKernelGGL(k_dedisperse16<32>, grid, dim3(D2_BS), lds, s, x, d_delays,
+                       d_chunk_pairs, g, g.wlen, remap, plane, ga.gd);
+  else
+    hipLaunchKernelGGL(k_dedisperse16<D2_DB>, grid, dim3(D2_BS), lds, s, x, d_delays,
+                       d_chunk_pairs, g, g.wlen, remap, plane, ga.gd);
   return hipGetLastError();
 }
 

@@ -212,19 +212,27 @@ __global__ __launch_bounds__(64) void k_block_rms_finish(const float* __restrict
 // writes the total to *total.
 template <int BS>
 __device__ unsigned block_exclusive_scan(unsigned v, unsigned* total) {
-  __shared__ unsigned s[BS];
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int d = 1; d < BS; d <<= 1) {
-    const unsigned t = (threadIdx.x >= (unsigned)d) ? s[threadIdx.x - d] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
+  // inclusive scan within each wave (shuffles), then the wave totals
+  __shared__ unsigned wsum[BS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned y = __shfl_up(x, d);
+    if (lane >= d) x += y;
   }
-  const unsigned incl = s[threadIdx.x];
-  *total = s[BS - 1];
+  if (lane == 63) wsum[w] = x;
   __syncthreads();
-  return incl - v;
+  unsigned pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < BS / 64; ++k) {
+    const unsigned t = wsum[k];
+    if (k < w) pre += t;
+    tot += t;
+  }
+  __syncthreads();  // wsum is reused by the next call
+  *total = tot;
+  return pre + x - v;
 }
 
 constexpr int TH_PER = 16;
@@ -252,6 +260,18 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const unsigned* nwv = gd[blockIdx.z].n_w;
   const float* S = Sall + row * spitch;
   const size_t p0 = c * TH_CHUNK;                 // first position of the chunk
+  {
+    // chunks past every width's output (shorter groups of a batch) only
+    // record empty counts
+    bool any = false;
+    for (int j = 0; j < wt.nw; ++j)
+      if ((p0 + wt.wi[j].ts - 1) / wt.wi[j].ts < nwv[j]) any = true;
+    if (!any) {
+      for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
+        gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
+      return;
+    }
+  }
   const size_t tbase = p0 + wt.off - bmax;        // S index held in T[0]
   const size_t tlen = TH_CHUNK + amax + bmax;
   for (size_t t = threadIdx.x; t < tlen; t += FB_BS) {
@@ -287,18 +307,37 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     const unsigned ra0 = r0 + wi.ahead, rb0 = r0 - wi.behind;
     float vals[TH_PER];
     unsigned mask = 0;
+    // valid outputs of this thread: q < nq
+    const unsigned nq = ib >= i1 ? 0u : (unsigned)min((size_t)per, i1 - ib);
+    if (one_period) {
+      // every valid output is in one beam period: keep i < lim
+      const unsigned nk = ib >= lim ? 0u : (unsigned)min((size_t)nq, lim - ib);
+      // outputs past nk are not kept; their reads are clamped into the tile
+      const unsigned tmax = (unsigned)tlen - 1;
 #pragma unroll
-    for (int q = 0; q < TH_PER; ++q) {
-      vals[q] = 0.f;
-      const size_t i = ib + q;
-      if ((unsigned)q < per && i < i1) {
-        const unsigned ra = ra0 + (unsigned)q * wi.ts, rb = rb0 + (unsigned)q * wi.ts;
+      for (int q = 0; q < TH_PER; ++q) {
+        const unsigned ra = min(ra0 + (unsigned)q * wi.ts, tmax);
+        const unsigned rb = min(rb0 + (unsigned)q * wi.ts, tmax);
         const float a = T[ra + (ra >> 4)];
         const float b = T[rb + (rb >> 4)];
         const float v = (a - b) * inv;               // get_rms.cu:194-197
         vals[q] = v;
-        const bool in_beam = one_period ? (i < lim) : ((size_t)(unsigned)i % stride < bc692);
-        if (in_beam && v > thresh) mask |= 1u << q;
+        if ((unsigned)q < nk && v > thresh) mask |= 1u << q;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < TH_PER; ++q) {
+        vals[q] = 0.f;
+        const size_t i = ib + q;
+        if ((unsigned)q < nq) {
+          const unsigned ra = ra0 + (unsigned)q * wi.ts, rb = rb0 + (unsigned)q * wi.ts;
+          const float a = T[ra + (ra >> 4)];
+          const float b = T[rb + (rb >> 4)];
+          const float v = (a - b) * inv;
+          vals[q] = v;
+          const bool in_beam = (size_t)(unsigned)i % stride < bc692;
+          if (in_beam && v > thresh) mask |= 1u << q;
+        }
       }
     }
     const int any = __syncthreads_or(mask != 0);
@@ -306,9 +345,8 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       if (threadIdx.x == 0) gb.ccount[key] = 0;
       continue;
     }
-    const unsigned cnt = __popc(mask);
     unsigned total;
-    const unsigned rank = block_exclusive_scan<FB_BS>(cnt, &total);
+    const unsigned rank = block_exclusive_scan<FB_BS>(__popc(mask), &total);
     if (threadIdx.x == 0) {
       sbase = atomicAdd(gb.counter, total);
       gb.ccount[key] = total;

@@ -248,8 +248,8 @@ __device__ __forceinline__ void scan4(float4& v, float& run) {
   v.w = run;
 }
 
-constexpr int SC_PMAX = 4096;                    // staged baseline points per block
-constexpr int SC_RMAX = 1024;                    // staged beam RMS values per block
+constexpr int SC_PMAX = 1024;                    // staged baseline points per block
+constexpr int SC_RMAX = 256;                     // staged beam RMS values per block
 
 // STAGED: the block's baseline points and beam RMS values live in LDS (the
 // normalising loader reads them for every sample); otherwise they are read
