@@ -136,7 +136,7 @@ constexpr int D2_PF = 6;                  // staging items prefetched per thread
 // 4-channel chunk needs for this block of trials (host-computed from the
 // delay spans, so top-of-band chunks stage short windows).
 template <int DB>
-__global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restrict__ x,
+__global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __restrict__ x,
                                                        const int* __restrict__ delays,
                                                        const int* __restrict__ chunk_pairs,
                                                        DedispGeom g, int hw_max, int xcd_remap,
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
                                                        const GroupDev* __restrict__ gd) {
   extern __shared__ uint32_t lds16[];  // [D2_CC][2][hw_max]
   constexpr int NA = DB / D2_WAVES;  // trials per wave
-  __shared__ int sdly[DB][D2_CC];
+  __shared__ __align__(16) int sdly[DB][D2_CC];
   // blockIdx.z: the group, with its own unpacked rows, output rows and length
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
   x += (size_t)blockIdx.z * g.n_live * g.xpitch;
@@ -227,7 +227,10 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
     if (k < nchunk && tid < DB * D2_CC) {
       const int c = min(k * D2_CC + tid % D2_CC, g.n_live - 1);
       const int mm = min(m0 + tid / D2_CC, g.n_dm - 1);
-      pf_delay = delays[(size_t)mm * g.n_live + c] - (drow0[c] & ~3);
+      // the dword of the trial's staged copy (E or O of channel slot cc)
+      // where lane 0 reads: cc*2 + (o&1) selects the copy, o>>1 the pair
+      const int o = delays[(size_t)mm * g.n_live + c] - (drow0[c] & ~3);
+      pf_delay = ((tid % D2_CC) * 2 + (o & 1)) * hw_max + (o >> 1);
     }
   };
   prefetch_delay(0);
@@ -250,13 +253,29 @@ __global__ __launch_bounds__(D2_BS) void k_dedisperse16(const uint8_t* __restric
     __syncthreads();
     prefetch(k + 1);
     prefetch_delay(k + 1);
-    for (int cc = 0; cc < ncc; ++cc) {
+    if (ncc == D2_CC) {
+      // all four channels: every offset of the wave's trials read up front
+      int4 off[NA];
 #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const int o = sdly[wave + D2_WAVES * a][cc];
-        const uint32_t* src = lds16 + (cc * 2 + (o & 1)) * hw_max + (o >> 1) + lane;
+      for (int a = 0; a < NA; ++a) off[a] = *reinterpret_cast<const int4*>(&sdly[wave + D2_WAVES * a][0]);
 #pragma unroll
-        for (int q = 0; q < D2_Q; ++q) pa[a][q] += src[64 * q];
+      for (int cc = 0; cc < D2_CC; ++cc) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+          const int o = cc == 0 ? off[a].x : cc == 1 ? off[a].y : cc == 2 ? off[a].z : off[a].w;
+          const uint32_t* src = lds16 + o + lane;
+#pragma unroll
+          for (int q = 0; q < D2_Q; ++q) pa[a][q] += src[64 * q];
+        }
+      }
+    } else {
+      for (int cc = 0; cc < ncc; ++cc) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+          const uint32_t* src = lds16 + sdly[wave + D2_WAVES * a][cc] + lane;
+#pragma unroll
+          for (int q = 0; q < D2_Q; ++q) pa[a][q] += src[64 * q];
+        }
       }
     }
     pending += ncc;
