@@ -2,6 +2,8 @@
 
   libheimdall_mi355x.so  the search (hd_* C ABI + gfx950 kernels), hipcc
   libhd_synth.so         synthetic filterbank generator, g++/OpenMP
+  bin/heimdall           gulp driver over sigproc files (app/heimdall_main.cpp)
+  bin/generate_dmlist    DM trial list tool (app/generate_dmlist.cpp)
 
 Every HIP source is compiled for gfx950 only, with -ffp-contract=off (the
 reference performs no fused multiply-adds, and results are compared bit for
@@ -19,8 +21,11 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libheimdall_mi355x.so")
 SYNTH = os.path.join(HERE, "libhd_synth.so")
 
-HIP_SOURCES = ["hd_api.cpp", "k_dedisperse.hip", "k_normalise.hip", "k_filter.hip",
-               "k_cluster.hip", "k_stages.hip", "k_clean.hip"]
+HIP_SOURCES = ["hd_api.cpp", "formats.cpp", "k_dedisperse.hip", "k_normalise.hip",
+               "k_filter.hip", "k_cluster.hip", "k_stages.hip", "k_clean.hip"]
+# command-line tools linked against the library: bin/<name> from app/<name>.cpp
+APPS = {"heimdall": "heimdall_main.cpp", "generate_dmlist": "generate_dmlist.cpp"}
+BIN = os.path.join(HERE, "bin")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -55,8 +60,15 @@ def build(verbose=False):
     _run(["g++", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-std=c++17",
           "-o", tmp, os.path.join(CSRC, "synth.cpp")])
     os.replace(tmp, SYNTH)
+    os.makedirs(BIN, exist_ok=True)
+    for name, src in APPS.items():
+        out = os.path.join(BIN, name)
+        _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+              os.path.join(HERE, "app", src), "-o", out + ".tmp", "-L" + HERE,
+              "-lheimdall_mi355x", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(out + ".tmp", out)
     if verbose:
-        print("built", LIB, SYNTH)
+        print("built", LIB, SYNTH, BIN)
     return LIB
 
 
