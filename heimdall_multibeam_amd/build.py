@@ -22,7 +22,8 @@ LIB = os.path.join(HERE, "libheimdall_mi355x.so")
 SYNTH = os.path.join(HERE, "libhd_synth.so")
 
 HIP_SOURCES = ["hd_api.cpp", "formats.cpp", "k_dedisperse.hip", "k_normalise.hip",
-               "k_filter.hip", "k_cluster.hip", "k_stages.hip", "k_clean.hip"]
+               "k_filter.hip", "k_cluster.hip", "k_stages.hip", "k_clean.hip",
+               "k_proposal.hip", "k_cnn.hip"]
 # command-line tools linked against the library: bin/<name> from app/<name>.cpp
 APPS = {"heimdall": "heimdall_main.cpp", "generate_dmlist": "generate_dmlist.cpp"}
 BIN = os.path.join(HERE, "bin")

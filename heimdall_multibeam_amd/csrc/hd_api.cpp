@@ -29,6 +29,7 @@
 #include "hd/mi355x.h"
 #include "hd/parse_command_line.h"
 #include "hd/pipeline.h"
+#include "hd/proposal.h"
 #include "hd/stages.h"
 #include "hd_internal.hpp"
 #include "hd_plan.hpp"
@@ -71,7 +72,7 @@ unsigned log2_pow2(unsigned v) {  // get_filter_index (pipeline.cu:126-136)
 }
 
 enum Stage { ST_UNPACK, ST_DEDISP, ST_BASELINE, ST_RMS, ST_SCAN, ST_BLOCKRMS, ST_GIANTS,
-             ST_CLUSTER, ST_TOTAL, ST_CLEAN, ST_COUNT };
+             ST_CLUSTER, ST_TOTAL, ST_CLEAN, ST_PROPOSAL, ST_COUNT };
 
 // One gulp's output.  Candidates [lo, hi) of the result arrays belong to it;
 // `beam` is the params.beam of the call that searched it.
@@ -110,6 +111,24 @@ struct hd_pipeline_t {
   std::vector<hd::GroupDev> hgd;        // host copy of the group table
   std::vector<GulpOut> gulps;           // per-gulp status and candidate ranges
   std::vector<uint32_t> r_group;        // active-group index of each candidate
+  std::vector<hd::GroupDev> hrg;        // row groups of the search (gulps, or gated units)
+  DevBuf rdev;
+  // learned proposal: the network, the coarse plan and plane, the last gating
+  hd_cnn net = nullptr;
+  bool gating = false;
+  float prop_thresh = 0.f;
+  bool coarse_only = false;             // run_gulps stops after the coarse plane
+  std::vector<float> co_dms, co_kc;
+  size_t co_max_delay = 0, co_wlen = 0;
+  std::vector<int> co_live_cached;
+  DevBuf co_delays, co_pairs, co_xc, co_plane, co_pts, co_rms, co_brms, co_bf, co_gdev;
+  DevBuf co_tiles, co_scores, co_toff, co_ntpos, co_ncols, co_flags, co_pos;
+  std::vector<hd::GroupDev> co_hgd;
+  int co_G = 0, co_rows = 0, co_units = 0;
+  size_t co_bpitch = 0;
+  std::vector<uint32_t> co_toff_h, co_ntpos_h, co_ncols_h;
+  std::vector<uint8_t> prop_flags;      // per gulp x unit of the last gated call
+  int prop_units = 0;
   unsigned* h_pinned = nullptr;  // small pinned staging word(s)
   size_t sample_cap = 1u << 22;
   // delay table state for the current live-channel set
@@ -379,6 +398,36 @@ struct DeviceGuard {  // binds the pipeline's device, restores the caller's
   }
 };
 
+// LDS window per DM block: widest (last - first) delay span over channels,
+// the first trial's delay rounded down to a dword (and per chunk of channels
+// staged together).  d: n_rows x nl delays; returns the pairs and the widest
+// window.
+std::vector<int> staging_pairs(const std::vector<int>& d, size_t n_rows, size_t nl, int* hw_out) {
+  const int DB = hd::dedisp_tile_dms();
+  const int CC = hd::dedisp_chunk_channels();
+  const size_t nblk = (n_rows + DB - 1) / DB;
+  const size_t nchunk = (nl + CC - 1) / CC;
+  std::vector<int> pairs(std::max<size_t>(nblk * nchunk, 1));
+  int hw_max = 2;
+  for (size_t b = 0; b < nblk; ++b) {
+    const size_t m0 = b * DB;
+    const size_t ml = std::min(m0 + DB, n_rows) - 1;
+    for (size_t k = 0; k < nchunk; ++k) {
+      int spread = 0;
+      for (size_t l = k * CC; l < std::min((k + 1) * CC, nl); ++l)
+        spread = std::max(spread, d[ml * nl + l] - (d[m0 * nl + l] & ~3));
+      const int hw = hd::dedisp_window_dwords(spread);
+      pairs[b * nchunk + k] = hw / 2;
+      hw_max = std::max(hw_max, hw);
+    }
+  }
+  *hw_out = hw_max;
+  return pairs;
+}
+
+// Delay table of the live channels.  Its rows run past the last trial to a
+// whole number of gated units (repeating the last trial), so that a gated
+// group's slice of HD_PROPOSAL_UNIT trials always has rows to read.
 hd_error build_live_and_delays(hd_pipeline pl) {
   const hd_params& p = pl->params;
   std::vector<int> km(p.nchans, 1);
@@ -389,32 +438,14 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   for (size_t c = 0; c < p.nchans; ++c)
     if (km[c]) pl->live_list.push_back((int)c);
   const size_t n_dm = pl->dm_list.size();
+  const size_t n_rows = round_up(n_dm, HD_PROPOSAL_UNIT);
   const size_t nl = pl->live_list.size();
-  std::vector<int> d(n_dm * std::max<size_t>(nl, 1));
-  for (size_t m = 0; m < n_dm; ++m)
+  std::vector<int> d(n_rows * std::max<size_t>(nl, 1));
+  for (size_t m = 0; m < n_rows; ++m)
     for (size_t l = 0; l < nl; ++l)
-      d[m * nl + l] = hdplan::delay(pl->dm_list[m], pl->kc[pl->live_list[l]]);
-  // LDS window per DM block: widest (last - first) delay span over channels,
-  // the first trial's delay rounded down to a dword
-  // (and per chunk of channels staged together)
-  const int DB = hd::dedisp_tile_dms();
-  const int CC = hd::dedisp_chunk_channels();
-  const size_t nblk = (n_dm + DB - 1) / DB;
-  const size_t nchunk = (nl + CC - 1) / CC;
-  std::vector<int> pairs(std::max<size_t>(nblk * nchunk, 1));
+      d[m * nl + l] = hdplan::delay(pl->dm_list[std::min(m, n_dm - 1)], pl->kc[pl->live_list[l]]);
   int hw_max = 2;
-  for (size_t b = 0; b < nblk; ++b) {
-    const size_t m0 = b * DB;
-    const size_t ml = std::min(m0 + DB, n_dm) - 1;
-    for (size_t k = 0; k < nchunk; ++k) {
-      int spread = 0;
-      for (size_t l = k * CC; l < std::min((k + 1) * CC, nl); ++l)
-        spread = std::max(spread, d[ml * nl + l] - (d[m0 * nl + l] & ~3));
-      const int hw = hd::dedisp_window_dwords(spread);
-      pairs[b * nchunk + k] = hw / 2;
-      hw_max = std::max(hw_max, hw);
-    }
-  }
+  const std::vector<int> pairs = staging_pairs(d, n_rows, nl, &hw_max);
   pl->wlen = (size_t)hw_max;
   HD_CHECK(pl->live.ensure(std::max<size_t>(nl, 1) * sizeof(int)));
   HD_CHECK(pl->delays.ensure(d.size() * sizeof(int)));
@@ -423,6 +454,40 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
   HD_CHECK(hipMemcpy(pl->chunk_pairs.p, pairs.data(), pairs.size() * sizeof(int),
                      hipMemcpyHostToDevice));
+  return HD_NO_ERROR;
+}
+
+// Coarse plan of the proposal: trial j of the coarse plane is trial
+// min(8j + 4, n_dm - 1) of the plan, dedispersed at 16x the sampling time
+// (delays from the same rule with dt*16) over the live channels.
+hd_error build_coarse(hd_pipeline pl) {
+  if (!pl->co_dms.empty() && pl->co_live_cached == pl->live_list) return HD_NO_ERROR;
+  const hd_params& p = pl->params;
+  const size_t n_dm = pl->dm_list.size();
+  const size_t nc = (n_dm + HD_PROPOSAL_DM - 1) / HD_PROPOSAL_DM;
+  pl->co_dms.resize(nc);
+  for (size_t j = 0; j < nc; ++j)
+    pl->co_dms[j] = pl->dm_list[std::min(HD_PROPOSAL_DM * j + HD_PROPOSAL_DM / 2, n_dm - 1)];
+  pl->co_kc = hdplan::delay_slopes(p.nchans, p.dt * (float)HD_PROPOSAL_TIME, p.f0, p.df);
+  pl->co_max_delay = 0;
+  for (size_t c = 0; c < p.nchans; ++c) {
+    const int d = hdplan::delay(pl->co_dms.back(), pl->co_kc[c]);
+    if (d > 0 && (size_t)d > pl->co_max_delay) pl->co_max_delay = (size_t)d;
+  }
+  const size_t nl = pl->live_list.size();
+  std::vector<int> d(nc * std::max<size_t>(nl, 1));
+  for (size_t m = 0; m < nc; ++m)
+    for (size_t l = 0; l < nl; ++l)
+      d[m * nl + l] = hdplan::delay(pl->co_dms[m], pl->co_kc[pl->live_list[l]]);
+  int hw_max = 2;
+  const std::vector<int> pairs = staging_pairs(d, nc, nl, &hw_max);
+  pl->co_wlen = (size_t)hw_max;
+  HD_CHECK(pl->co_delays.ensure(d.size() * sizeof(int)));
+  HD_CHECK(pl->co_pairs.ensure(pairs.size() * sizeof(int)));
+  HD_CHECK(hipMemcpy(pl->co_delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
+  HD_CHECK(hipMemcpy(pl->co_pairs.p, pairs.data(), pairs.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  pl->co_live_cached = pl->live_list;
   return HD_NO_ERROR;
 }
 
@@ -600,6 +665,154 @@ struct GulpIn {
   size_t nsamps, nbeams, first_idx, beam;
 };
 
+// The coarse plane of the G gulps of pl->hgd, whose unpacked rows are slots
+// 0..G-1 of pl->x (pitch xpitch): decimation, coarse dedispersion, the
+// search's baseline and RMS kernels on the coarse series, bfloat16 output,
+// and the tile grid (units x time positions per gulp).
+hd_error coarse_pass(hd_pipeline pl, int G, size_t xpitch, size_t nbits, hipStream_t s) {
+  const hd_params& p = pl->params;
+  hd_error e = build_coarse(pl);
+  if (e != HD_NO_ERROR) return e;
+  const int nc = (int)pl->co_dms.size();
+  const int n_live = (int)pl->live_list.size();
+  const int TB = hd::dedisp_tile_time();
+  std::vector<hd::GroupDev>& cg = pl->co_hgd;
+  cg.assign(G, hd::GroupDev{});
+  size_t ppc = 0, spc = 0, prow = 0, nout_max = 0;
+  for (int g = 0; g < G; ++g) {
+    const size_t ns = pl->hgd[g].nsamps / HD_PROPOSAL_TIME;
+    if (ns < pl->co_max_delay + 64) return HD_TOO_FEW_NSAMPS;
+    SeriesShape& sh = cg[g].sh;
+    sh.nbeams = 1;
+    sh.stride = ns;
+    sh.nout = ns - pl->co_max_delay;
+    sh.bc = sh.nout;
+    sh.ppitch = round_up(sh.nout + 1, (size_t)TB);
+    sh.spitch = round_up(sh.nout + 1, 64);
+    const int se = hd::series_shape(p.dt * (float)HD_PROPOSAL_TIME, p.baseline_length, sh);
+    if (se != HD_NO_ERROR) return (hd_error)se;
+    cg[g].nsamps = ns;
+    cg[g].nsamps_beam = ns;
+    cg[g].xslot = (uint32_t)g;
+    cg[g].cgroup = (uint32_t)g;
+    cg[g].rows_valid = (uint32_t)nc;
+    ppc = std::max(ppc, sh.ppitch);
+    spc = std::max(spc, sh.spitch);
+    prow = std::max(prow, sh.sc);
+    nout_max = std::max(nout_max, sh.nout);
+  }
+  for (int g = 0; g < G; ++g) {
+    cg[g].sh.ppitch = ppc;
+    cg[g].sh.spitch = spc;
+  }
+  HD_CHECK(pl->co_gdev.ensure((size_t)G * sizeof(hd::GroupDev)));
+  HD_CHECK(hipMemcpyAsync(pl->co_gdev.p, cg.data(), (size_t)G * sizeof(hd::GroupDev),
+                          hipMemcpyHostToDevice, s));
+  hd::GroupArgs cga;
+  cga.gd = pl->co_gdev.as<hd::GroupDev>();
+  cga.host = cg.data();
+  cga.G = G;
+  cga.nbmax = 1;
+  cga.prow = std::max<size_t>(prow, 1);
+  const size_t xpc = round_up(ppc + pl->co_max_delay + 2 * pl->co_wlen + 256, 256);
+  HD_CHECK(pl->co_xc.ensure((size_t)G * std::max(n_live, 1) * xpc));
+  HD_CHECK(hd::launch_decimate(pl->x.as<uint8_t>(), xpitch, n_live, (int)nbits,
+                               pl->co_xc.as<uint8_t>(), xpc, cga.gd, G, s));
+  const size_t rows = (size_t)G * nc;
+  HD_CHECK(pl->co_plane.ensure(rows * ppc * sizeof(float)));
+  hd::DedispGeom dg{nc, n_live, cg[0].sh.nout, xpc, ppc, (int)pl->co_wlen};
+  HD_CHECK(hd::launch_dedisperse(pl->co_xc.as<uint8_t>(), pl->co_delays.as<int>(),
+                                 pl->co_pairs.as<int>(), dg, pl->co_plane.as<float>(), s, cga));
+  HD_CHECK(pl->co_pts.ensure((rows * cga.prow + 64) * sizeof(float)));
+  HD_CHECK(pl->co_rms.ensure(rows * sizeof(float)));
+  HD_CHECK(hd::launch_baseline_points(pl->co_plane.as<float>(), cg[0].sh, nc,
+                                      pl->co_pts.as<float>(), s, cga));
+  const size_t words = hd::beam_rms_scratch_words(cg[0].sh, nc, cga);
+  HD_CHECK(pl->co_brms.ensure(std::max<size_t>(words, 1) * sizeof(float)));
+  HD_CHECK(hd::launch_beam_rms(pl->co_plane.as<float>(), pl->co_pts.as<float>(), cg[0].sh, nc,
+                               pl->co_rms.as<float>(), s, pl->co_brms.as<float>(), words, cga));
+  const size_t bpitch = round_up(nout_max, 64);
+  HD_CHECK(pl->co_bf.ensure(rows * bpitch * sizeof(uint16_t)));
+  HD_CHECK(hd::launch_coarse_bf16(pl->co_plane.as<float>(), ppc, pl->co_pts.as<float>(),
+                                  cga.prow, pl->co_rms.as<float>(), 1, cga.gd, G, nc,
+                                  pl->co_bf.as<uint16_t>(), bpitch, s));
+  // tile grid
+  const int units = (int)((pl->dm_list.size() + HD_PROPOSAL_UNIT - 1) / HD_PROPOSAL_UNIT);
+  pl->co_toff_h.assign(G + 1, 0);
+  pl->co_ntpos_h.assign(G, 0);
+  pl->co_ncols_h.assign(G, 0);
+  for (int g = 0; g < G; ++g) {
+    const size_t nv = (cg[g].sh.nout + HD_PROPOSAL_TSTEP - 1) / HD_PROPOSAL_TSTEP;
+    pl->co_ntpos_h[g] = (uint32_t)nv;
+    pl->co_ncols_h[g] = (uint32_t)cg[g].sh.nout;
+    const uint64_t next = (uint64_t)pl->co_toff_h[g] + (uint64_t)units * nv;
+    if (next >= (1ull << 32)) return HD_INVALID_STRIDE;
+    pl->co_toff_h[g + 1] = (uint32_t)next;
+  }
+  HD_CHECK(pl->co_toff.ensure((G + 1) * sizeof(uint32_t)));
+  HD_CHECK(pl->co_ntpos.ensure(G * sizeof(uint32_t)));
+  HD_CHECK(pl->co_ncols.ensure(G * sizeof(uint32_t)));
+  HD_CHECK(hipMemcpyAsync(pl->co_toff.p, pl->co_toff_h.data(), (G + 1) * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+  HD_CHECK(hipMemcpyAsync(pl->co_ntpos.p, pl->co_ntpos_h.data(), G * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+  HD_CHECK(hipMemcpyAsync(pl->co_ncols.p, pl->co_ncols_h.data(), G * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+  pl->co_G = G;
+  pl->co_rows = nc;
+  pl->co_units = units;
+  pl->co_bpitch = bpitch;
+  return HD_NO_ERROR;
+}
+
+hd::TileGrid tile_grid(hd_pipeline pl) {
+  hd::TileGrid tg;
+  tg.bf = pl->co_bf.as<uint16_t>();
+  tg.bpitch = pl->co_bpitch;
+  tg.n_rows = pl->co_rows;
+  tg.ncols = pl->co_ncols.as<uint32_t>();
+  tg.n_units = pl->co_units;
+  tg.tile_off = pl->co_toff.as<uint32_t>();
+  tg.n_tpos = pl->co_ntpos.as<uint32_t>();
+  tg.G = pl->co_G;
+  return tg;
+}
+
+// Logits of every tile of the coarse plane into co_scores, in batches of
+// the network's capacity (tiles extracted, then scored, on stream s).
+hd_error score_tiles(hd_pipeline pl, hipStream_t s) {
+  if (!pl->net || pl->co_G == 0) return HD_INVALID_POINTER;
+  const size_t total = pl->co_toff_h.back();
+  const size_t B = hd::cnn_max_batch(pl->net);
+  HD_CHECK(pl->co_tiles.ensure(B * HD_PROPOSAL_TILE * HD_PROPOSAL_TILE * sizeof(uint16_t)));
+  HD_CHECK(pl->co_scores.ensure(std::max<size_t>(total, 1) * sizeof(float)));
+  const hd::TileGrid tg = tile_grid(pl);
+  for (size_t first = 0; first < total; first += B) {
+    const size_t n = std::min(B, total - first);
+    HD_CHECK(hd::launch_extract_tiles(tg, nullptr, first, n, pl->co_tiles.as<uint16_t>(), s));
+    HD_CHECK(hd::cnn_forward_async(pl->net, pl->co_tiles.as<uint16_t>(), n,
+                                   pl->co_scores.as<float>() + first, s));
+  }
+  return HD_NO_ERROR;
+}
+
+// Units of every gulp whose tiles score above the threshold (host flags).
+hd_error propose(hd_pipeline pl, hipStream_t s) {
+  hd_error e = score_tiles(pl, s);
+  if (e != HD_NO_ERROR) return e;
+  const int G = pl->co_G, U = pl->co_units;
+  HD_CHECK(pl->co_flags.ensure(std::max(G * U, 1)));
+  HD_CHECK(hd::launch_unit_flags(pl->co_scores.as<float>(), pl->co_toff.as<uint32_t>(),
+                                 pl->co_ntpos.as<uint32_t>(), G, U, pl->prop_thresh,
+                                 pl->co_flags.as<uint8_t>(), s));
+  pl->prop_flags.assign((size_t)G * U, 0);
+  pl->prop_units = U;
+  HD_CHECK(hipMemcpyAsync(pl->prop_flags.data(), pl->co_flags.p, (size_t)G * U,
+                          hipMemcpyDeviceToHost, s));
+  HD_CHECK(hipStreamSynchronize(s));
+  return HD_NO_ERROR;
+}
+
 // The device part of a call: every gulp of `in` is searched with one launch
 // per stage (the gulps are the groups of the kernels' group dimension).
 // Per-gulp results land in pl->gulps; a returned error concerns the call.
@@ -611,6 +824,8 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   pl->r_filter.clear(); pl->r_dm.clear(); pl->r_members.clear(); pl->r_dms.clear();
   pl->r_group.clear();
   pl->cand_text.clear();
+  pl->prop_flags.clear();
+  pl->prop_units = 0;
   for (double& t : pl->timings) t = 0;
   const size_t W = p.boxcar_max;
   const size_t max_delay = pl->max_delay;
@@ -694,6 +909,12 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
       off += pl->hgd[g].nsamps * row_bytes;
     }
   }
+  for (int g = 0; g < G; ++g) {  // every trial of the gulp, its own input slot
+    pl->hgd[g].trial0 = 0;
+    pl->hgd[g].rows_valid = (uint32_t)n_dm;
+    pl->hgd[g].xslot = (uint32_t)g;
+    pl->hgd[g].cgroup = (uint32_t)g;
+  }
   HD_CHECK(pl->gdev.ensure((size_t)G * sizeof(hd::GroupDev)));
   HD_CHECK(hipMemcpyAsync(pl->gdev.p, pl->hgd.data(), (size_t)G * sizeof(hd::GroupDev),
                           hipMemcpyHostToDevice, s));
@@ -704,7 +925,6 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   ga.nbmax = nbmax;
   ga.prow = std::max<size_t>(prow, 1);
   const SeriesShape& sh0 = pl->hgd[0].sh;
-  const size_t rows = (size_t)G * n_dm;
 
   record(pl, 0);
   // unpack to channel-major bytes
@@ -713,50 +933,94 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   HD_CHECK(hd::launch_unpack(nullptr, p.nchans * nbits / 8, (int)nbits, 0, pl->live.as<int>(),
                              n_live, pl->x.as<uint8_t>(), xpitch, s, ga,
                              (size_t)n_live == p.nchans));
+  record(pl, 10);
+  // The search runs over row groups: a whole gulp each, or with gating one
+  // per proposed unit of HD_PROPOSAL_UNIT trials (rows past the last trial
+  // are padding).  Row groups keep their gulp's shapes, input and
+  // clustering group, so every value they compute is the full search's.
+  int R = n_dm;
+  std::vector<hd::GroupDev>& hrg = pl->hrg;
+  if (pl->gating || pl->coarse_only) {
+    for (int k : active)
+      if (in[k].nbeams != 1) return HD_INVALID_STRIDE;  // one beam per gulp
+    hd_error e = coarse_pass(pl, G, xpitch, nbits, s);
+    if (e != HD_NO_ERROR) return e;
+    if (pl->coarse_only) {
+      for (int ev = 1; ev <= 8; ++ev) record(pl, ev);
+      return HD_NO_ERROR;
+    }
+    e = propose(pl, s);
+    if (e != HD_NO_ERROR) return e;
+    R = HD_PROPOSAL_UNIT;
+    hrg.clear();
+    for (int g = 0; g < G; ++g)
+      for (int u = 0; u < pl->prop_units; ++u) {
+        if (!pl->prop_flags[(size_t)g * pl->prop_units + u]) continue;
+        hd::GroupDev d = pl->hgd[g];
+        d.trial0 = (uint32_t)(u * HD_PROPOSAL_UNIT);
+        d.rows_valid = (uint32_t)std::min<size_t>(HD_PROPOSAL_UNIT, n_dm - d.trial0);
+        hrg.push_back(d);
+      }
+  } else {
+    hrg = pl->hgd;
+  }
   record(pl, 1);
+  const int Gr = (int)hrg.size();
+  if (Gr == 0) {  // nothing proposed: no giants, no candidates
+    for (int ev = 2; ev <= 8; ++ev) record(pl, ev);
+    return HD_NO_ERROR;
+  }
+  HD_CHECK(pl->rdev.ensure((size_t)Gr * sizeof(hd::GroupDev)));
+  HD_CHECK(hipMemcpyAsync(pl->rdev.p, hrg.data(), (size_t)Gr * sizeof(hd::GroupDev),
+                          hipMemcpyHostToDevice, s));
+  hd::GroupArgs gr = ga;
+  gr.gd = pl->rdev.as<hd::GroupDev>();
+  gr.host = hrg.data();
+  gr.G = Gr;
+  const size_t rows = (size_t)Gr * R;
   // dedisperse
   HD_CHECK(pl->plane.ensure(rows * ppitch * sizeof(float)));
-  hd::DedispGeom dg{n_dm, n_live, sh0.nout, xpitch, ppitch, (int)pl->wlen};
+  hd::DedispGeom dg{R, n_live, sh0.nout, xpitch, ppitch, (int)pl->wlen};
   HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
-                                 pl->chunk_pairs.as<int>(), dg, pl->plane.as<float>(), s, ga));
+                                 pl->chunk_pairs.as<int>(), dg, pl->plane.as<float>(), s, gr));
   record(pl, 2);
   // baseline points and per-beam RMS (normalisation is fused into the scan)
-  HD_CHECK(pl->pts.ensure((rows * ga.prow + 64) * sizeof(float)));
+  HD_CHECK(pl->pts.ensure((rows * gr.prow + 64) * sizeof(float)));
   HD_CHECK(pl->rms.ensure(rows * nbmax * sizeof(float)));
-  HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh0, n_dm, pl->pts.as<float>(), s,
-                                      ga));
+  HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh0, R, pl->pts.as<float>(), s,
+                                      gr));
   record(pl, 3);
   {
-    const size_t words = hd::beam_rms_scratch_words(sh0, n_dm, ga);
+    const size_t words = hd::beam_rms_scratch_words(sh0, R, gr);
     HD_CHECK(pl->brms.ensure(std::max<size_t>(words, 1) * sizeof(float)));
-    HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh0, n_dm,
-                                 pl->rms.as<float>(), s, pl->brms.as<float>(), words, ga));
+    HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh0, R,
+                                 pl->rms.as<float>(), s, pl->brms.as<float>(), words, gr));
   }
   record(pl, 4);
   // normalised prefix sums; rows start 3 floats into the allocation so that
   // S[1..] is 16-byte aligned for the scan's vector stores
   HD_CHECK(pl->S.ensure((rows * spitch + 8) * sizeof(float)));
   float* const Sbase = pl->S.as<float>() + 3;
-  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), ppitch, 0, n_dm, Sbase, spitch, s,
-                           pl->pts.as<float>(), pl->rms.as<float>(), nullptr, ga));
+  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), ppitch, 0, R, Sbase, spitch, s,
+                           pl->pts.as<float>(), pl->rms.as<float>(), nullptr, gr));
   record(pl, 5);
   // block RMS: the ts == 1 widths together, then each decimated width in order
-  const size_t np_max = hd::block_rms_np_max(ga);
+  const size_t np_max = hd::block_rms_np_max(gr);
   HD_CHECK(pl->partial.ensure(rows * wt.nw * std::max<size_t>(np_max, 1) * sizeof(float)));
   HD_CHECK(pl->wrms.ensure(rows * wt.nw * sizeof(float)));
   HD_CHECK(pl->winv.ensure(rows * wt.nw * sizeof(float)));
   int n_fresh = 0;
   while (n_fresh < wt.nw && wt.wi[n_fresh].ts == 1) ++n_fresh;
-  HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, 0, n_fresh, n_dm,
-                                         pl->partial.as<float>(), pl->winv.as<float>(), s, ga));
-  HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, 0, n_fresh, n_dm,
-                                       pl->wrms.as<float>(), pl->winv.as<float>(), s, ga));
+  HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, 0, n_fresh, R,
+                                         pl->partial.as<float>(), pl->winv.as<float>(), s, gr));
+  HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, 0, n_fresh, R,
+                                       pl->wrms.as<float>(), pl->winv.as<float>(), s, gr));
   for (int j = n_fresh; j < wt.nw; ++j) {
-    HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, j, 1, n_dm,
+    HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, j, 1, R,
                                            pl->partial.as<float>(), pl->winv.as<float>(), s,
-                                           ga));
-    HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, j, 1, n_dm,
-                                         pl->wrms.as<float>(), pl->winv.as<float>(), s, ga));
+                                           gr));
+    HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, j, 1, R,
+                                         pl->wrms.as<float>(), pl->winv.as<float>(), s, gr));
   }
   record(pl, 6);
   // threshold + ordered compaction
@@ -785,8 +1049,8 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     gb.cpos = pl->cpos.as<unsigned>();
     gb.scan_tmp = pl->scan_tmp.as<unsigned>();
     HD_CHECK(hipMemsetAsync(gb.counter, 0, sizeof(unsigned), s));
-    HD_CHECK(hd::launch_threshold(Sbase, spitch, wt, pl->winv.as<float>(), n_dm,
-                                  p.detect_thresh, gb, s, ga));
+    HD_CHECK(hd::launch_threshold(Sbase, spitch, wt, pl->winv.as<float>(), R,
+                                  p.detect_thresh, gb, s, gr));
     HD_CHECK(hipMemcpyAsync(pl->h_pinned, gb.counter, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HD_CHECK(hipStreamSynchronize(s));
     nsamp = pl->h_pinned[0];
@@ -828,8 +1092,8 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
                     pl->g_end.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
                     pl->g_dm.as<uint32_t>(), pl->g_group.as<uint32_t>()};
     HD_CHECK(pl->g_best.ensure(std::max<size_t>(n_giants, 1) * sizeof(uint64_t)));
-    HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, (unsigned)n_giants, n_dm, go,
-                                       pl->g_best.as<uint64_t>(), s));
+    HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, (unsigned)n_giants, R, go,
+                                       pl->g_best.as<uint64_t>(), s, gr.gd));
   }
   pl->n_giants = n_giants;
   record(pl, 7);
@@ -980,8 +1244,8 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
 hd_error finish_call(hd_pipeline pl, hd_error err) {
   if (err != HD_NO_ERROR) return err;
   (void)hipEventSynchronize(pl->ev[8]);
-  static const int stage_ev[ST_COUNT][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5},
-                                            {5, 6}, {6, 7}, {7, 8}, {9, 8}, {9, 0}};
+  static const int stage_ev[ST_COUNT][2] = {{0, 10}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6},
+                                            {6, 7}, {7, 8}, {9, 8}, {9, 0}, {10, 1}};
   for (int k = 0; k < ST_COUNT; ++k) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, pl->ev[stage_ev[k][0]], pl->ev[stage_ev[k][1]]) == hipSuccess)
@@ -1013,6 +1277,7 @@ hd_error finish_call(hd_pipeline pl, hd_error err) {
     const double* t = pl->timings;
     std::cout << "0-DM cleaning time:      " << t[ST_CLEAN] * 1e-3 << "\n"
               << "Unpack time:             " << t[ST_UNPACK] * 1e-3 << "\n"
+              << "Proposal time:           " << t[ST_PROPOSAL] * 1e-3 << "\n"
               << "Dedispersion time:       " << t[ST_DEDISP] * 1e-3 << "\n"
               << "Baselining time:         " << t[ST_BASELINE] * 1e-3 << "\n"
               << "Normalisation time:      " << t[ST_RMS] * 1e-3 << "\n"
@@ -1194,8 +1459,12 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->c_members, &pl->c_rflag, &pl->c_rpos, &pl->c_tmp, &pl->o_peak,
                       &pl->o_ind, &pl->o_begin, &pl->o_end, &pl->o_filter, &pl->o_dm,
                       &pl->o_members, &pl->gdev, &pl->g_group, &pl->o_group, &pl->c_sind,
-                      &pl->c_sfd};
+                      &pl->c_sfd, &pl->rdev, &pl->co_delays, &pl->co_pairs, &pl->co_xc,
+                      &pl->co_plane, &pl->co_pts, &pl->co_rms, &pl->co_brms, &pl->co_bf,
+                      &pl->co_gdev, &pl->co_tiles, &pl->co_scores, &pl->co_toff, &pl->co_ntpos,
+                      &pl->co_ncols, &pl->co_flags, &pl->co_pos};
     for (DevBuf* b : bufs) b->release();
+    if (pl->net) hd_cnn_destroy(pl->net);
     for (hipEvent_t& ev : pl->ev)
       if (ev) (void)hipEventDestroy(ev);
     if (pl->h_pinned) (void)hipHostFree(pl->h_pinned);
@@ -1285,4 +1554,104 @@ extern "C" void hd_set_output_enabled(hd_pipeline pl, int enabled) {
 }
 extern "C" void hd_set_cleaning(hd_pipeline pl, int enabled) {
   if (pl) pl->cleaning = enabled != 0;
+}
+
+// ---------------------------------------------------------------------------
+// learned proposal (hd/proposal.h)
+
+extern "C" hd_error hd_set_proposal(hd_pipeline pl, const hd_float* params, hd_float threshold) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (!params) {
+    pl->gating = false;
+    return HD_NO_ERROR;
+  }
+  DeviceGuard dg(pl->device);
+  if (!pl->net) {
+    const hd_error e = hd_cnn_create(&pl->net, pl->device, 4096);
+    if (e != HD_NO_ERROR) return e;
+  }
+  const hd_error e = hd_cnn_set_params(pl->net, params);
+  if (e != HD_NO_ERROR) return e;
+  pl->prop_thresh = threshold;
+  pl->gating = true;
+  return HD_NO_ERROR;
+}
+
+extern "C" hd_error hd_proposal_coarse(hd_pipeline pl, const hd_byte* d_filterbank,
+                                       hd_size nsamps, hd_size nbits, hd_size* n_units,
+                                       hd_size* n_tpos) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (!d_filterbank) return HD_INVALID_POINTER;
+  if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8)) return HD_INVALID_NBITS;
+  DeviceGuard dg(pl->device);
+  const std::vector<GulpIn> in{{d_filterbank, nsamps, 1, 0, pl->params.beam}};
+  pl->coarse_only = true;
+  hd_error err = run_gulps(pl, in, nbits);
+  pl->coarse_only = false;
+  if (err == HD_NO_ERROR && !pl->gulps.empty()) err = pl->gulps[0].err;
+  if (err == HD_NO_ERROR && pl->co_G != 1) err = HD_TOO_FEW_NSAMPS;
+  if (err != HD_NO_ERROR) {
+    pl->co_G = 0;
+    return err;
+  }
+  HD_CHECK(hipStreamSynchronize(pl->stream));
+  if (n_units) *n_units = (hd_size)pl->co_units;
+  if (n_tpos) *n_tpos = (hd_size)pl->co_ntpos_h[0];
+  return HD_NO_ERROR;
+}
+
+extern "C" hd_error hd_proposal_extract(hd_pipeline pl, const unsigned* pos, hd_size n,
+                                        unsigned short* d_tiles) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if ((!pos || !d_tiles) && n) return HD_INVALID_POINTER;
+  if (pl->co_G == 0) return HD_INVALID_POINTER;
+  if (n == 0) return HD_NO_ERROR;
+  for (hd_size i = 0; i < n; ++i)
+    if (pos[2 * i] >= (unsigned)pl->co_units || pos[2 * i + 1] >= pl->co_ntpos_h[0])
+      return HD_INVALID_STRIDE;
+  DeviceGuard dg(pl->device);
+  hipStream_t s = pl->stream;
+  HD_CHECK(pl->co_pos.ensure(2 * n * sizeof(uint32_t)));
+  HD_CHECK(hipMemcpyAsync(pl->co_pos.p, pos, 2 * n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HD_CHECK(hd::launch_extract_tiles(tile_grid(pl), pl->co_pos.as<uint32_t>(), 0, n, d_tiles, s));
+  HD_CHECK(hipStreamSynchronize(s));
+  return HD_NO_ERROR;
+}
+
+extern "C" hd_size hd_proposal_get_plane(hd_pipeline pl, unsigned short* host, hd_size cap,
+                                         hd_size* rows, hd_size* cols) {
+  if (!pl || pl->co_G == 0) return 0;
+  const size_t r = (size_t)pl->co_rows, c = pl->co_ncols_h[0];
+  if (rows) *rows = r;
+  if (cols) *cols = c;
+  if (host && cap >= r * c) {
+    DeviceGuard dg(pl->device);
+    if (hipMemcpy2D(host, c * sizeof(uint16_t), pl->co_bf.p, pl->co_bpitch * sizeof(uint16_t),
+                    c * sizeof(uint16_t), r, hipMemcpyDeviceToHost) != hipSuccess)
+      return 0;
+  }
+  return r * c;
+}
+
+extern "C" hd_size hd_proposal_scores(hd_pipeline pl, hd_float* host, hd_size cap) {
+  if (!pl || !pl->net || pl->co_G == 0) return 0;
+  DeviceGuard dg(pl->device);
+  if (score_tiles(pl, pl->stream) != HD_NO_ERROR) return 0;
+  const size_t n = pl->co_toff_h.back();
+  if (host && cap >= n) {
+    if (hipMemcpyAsync(host, pl->co_scores.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                       pl->stream) != hipSuccess)
+      return 0;
+  }
+  if (hipStreamSynchronize(pl->stream) != hipSuccess) return 0;
+  return n;
+}
+
+extern "C" hd_size hd_get_last_proposal(hd_pipeline pl, unsigned char* flags, hd_size cap,
+                                        hd_size* units_per_gulp) {
+  if (!pl) return 0;
+  if (units_per_gulp) *units_per_gulp = (hd_size)pl->prop_units;
+  const size_t n = pl->prop_flags.size();
+  if (flags) std::memcpy(flags, pl->prop_flags.data(), std::min<size_t>(n, cap));
+  return n;
 }

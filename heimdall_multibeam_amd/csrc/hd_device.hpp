@@ -90,6 +90,15 @@ __device__ __forceinline__ size_t reblock_pos(unsigned o, size_t length, float d
   return (size_t)q;
 }
 
+// bfloat16 bits of a float, rounded to nearest with ties to even (finite
+// inputs), and back.
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 // Orderable key of a float: unsigned compare of keys matches float compare
 // (for non-NaN values).
 __device__ __forceinline__ uint32_t float_key(float f) {

@@ -84,6 +84,15 @@ struct GroupDev {
   unsigned n_w[kMaxWidths];   // fresh boxcar outputs per width
   uint64_t t_off;             // clustering coordinate of the group's sample 0
   uint64_t nsamps_beam;       // nsamps / nbeams
+  // A group may hold a slice of a gulp's trials (the gated search): its rows
+  // are trials trial0 .. trial0 + n_rows - 1 of the plan (trial0 a multiple
+  // of the dedispersion block), rows >= rows_valid are padding and produce
+  // no giants, its unpacked input is slot xslot of x, and its giants belong
+  // to clustering group cgroup (the gulp).
+  uint32_t trial0 = 0;
+  uint32_t rows_valid = 0xFFFFFFFFu;
+  uint32_t xslot = 0;
+  uint32_t cgroup = 0;
 };
 
 // Group arguments of a launch: gd == nullptr means one group described by
@@ -206,9 +215,11 @@ struct GiantOut {
   uint32_t* dm;
   uint32_t* group;     // group of each giant (nullable)
 };
+// With gd (the launch's group table), a giant's trial is its group's trial0
+// plus its row and its group is the group's cgroup.
 hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
                                  unsigned ngiants, int n_dm, GiantOut go, uint64_t* best,
-                                 hipStream_t s);
+                                 hipStream_t s, const GroupDev* gd = nullptr);
 
 // clustering / merge
 struct ClusterBuffers {
@@ -246,4 +257,43 @@ hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* 
                         hipStream_t s, const uint32_t* group = nullptr,
                         uint32_t* g_group = nullptr);
 
+// ---- learned proposal (k_proposal.hip, k_cnn.hip) ----
+// Coarse input: x (slot g of each group, xpitch) summed over 16 samples into
+// xc (xpitch_c): (sum + 8) >> 4 for 8-bit samples, the plain sum otherwise.
+// gd[g].nsamps is the group's coarse sample count.
+hipError_t launch_decimate(const uint8_t* x, size_t xpitch, int n_live, int nbits, uint8_t* xc,
+                           size_t xpitch_c, const GroupDev* gd, int G, hipStream_t s);
+// Normalised coarse plane as bfloat16: bf[(g*n_rows + m)*bpitch + t].
+hipError_t launch_coarse_bf16(const float* plane, size_t ppitch, const float* pts, size_t prow,
+                              const float* rms, unsigned nbmax, const GroupDev* gd, int G,
+                              int n_rows, uint16_t* bf, size_t bpitch, hipStream_t s);
+// Tiles of the coarse plane.  Explicit: n tiles at (u, v) = pos[2i..2i+1] of
+// group 0.  Enumerated: tiles first .. first+n-1 of the order (g, u, v),
+// tile_off[g] the first tile of group g (G+1 entries), n_tpos[g] its time
+// positions, n_units units per group.
+struct TileGrid {
+  const uint16_t* bf;
+  size_t bpitch;
+  int n_rows;                 // coarse rows per group
+  const uint32_t* ncols;      // coarse samples of each group's plane
+  int n_units;
+  const uint32_t* tile_off;   // enumerated mode
+  const uint32_t* n_tpos;
+  int G;
+};
+hipError_t launch_extract_tiles(const TileGrid& tg, const uint32_t* pos, size_t first, size_t n,
+                                uint16_t* tiles, hipStream_t s);
+// flags[g*n_units + u] = 1 if any tile (g, u, v) has score > thresh.
+hipError_t launch_unit_flags(const float* scores, const uint32_t* tile_off,
+                             const uint32_t* n_tpos, int G, int n_units, float thresh,
+                             uint8_t* flags, hipStream_t s);
+
+}  // namespace hd
+
+// The proposal network (k_cnn.hip): forward of n tiles on stream s.
+struct hd_cnn_t;
+namespace hd {
+hipError_t cnn_forward_async(hd_cnn_t* net, const uint16_t* tiles, size_t n, float* logits,
+                             hipStream_t s);
+size_t cnn_max_batch(hd_cnn_t* net);
 }  // namespace hd

@@ -146,8 +146,11 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
   constexpr int NA = DB / D2_WAVES;  // trials per wave
   __shared__ __align__(16) int sdly[DB][D2_CC];
   // blockIdx.z: the group, with its own unpacked rows, output rows and length
+  // its trials start at trial0 of the plan's delay table (a whole number of
+  // blocks in) and its unpacked input is slot xslot of x
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
-  x += (size_t)blockIdx.z * g.n_live * g.xpitch;
+  const int trial0 = gd ? (int)gd[blockIdx.z].trial0 : 0;
+  x += (size_t)(gd ? gd[blockIdx.z].xslot : blockIdx.z) * g.n_live * g.xpitch;
   out += (size_t)blockIdx.z * g.n_dm * g.ppitch;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
@@ -167,7 +170,7 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
   const int m0 = mb * DB;
   const size_t t0 = (size_t)(lin / nbx) * D2_TB;
   if (t0 > nout) return;  // past this group's series (and its zero at nout)
-  const int* drow0 = delays + (size_t)m0 * g.n_live;
+  const int* drow0 = delays + (size_t)(trial0 + m0) * g.n_live;
 
   uint32_t pa[NA][D2_Q];
   uint32_t ua[NA][D2_Q][2];
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
   // are issued into registers before chunk k is summed, so their latency
   // hides behind the LDS work.
   const int nchunk = (g.n_live + D2_CC - 1) / D2_CC;
-  const int* cpairs = chunk_pairs + (size_t)mb * nchunk;
+  const int* cpairs = chunk_pairs + (size_t)(trial0 / DB + mb) * nchunk;
   // D2_BS/D2_CC threads per staged channel; thread jj handles pairs jj + TPC*r
   constexpr int TPC = D2_BS / D2_CC;
   const int scc = tid / TPC;
@@ -229,7 +232,7 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
       const int mm = min(m0 + tid / D2_CC, g.n_dm - 1);
       // the dword of the trial's staged copy (E or O of channel slot cc)
       // where lane 0 reads: cc*2 + (o&1) selects the copy, o>>1 the pair
-      const int o = delays[(size_t)mm * g.n_live + c] - (drow0[c] & ~3);
+      const int o = delays[(size_t)(trial0 + mm) * g.n_live + c] - (drow0[c] & ~3);
       pf_delay = ((tid % D2_CC) * 2 + (o & 1)) * hw_max + (o >> 1);
     }
   };

@@ -263,9 +263,11 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   {
     // chunks past every width's output (shorter groups of a batch) only
     // record empty counts
+    // (and padding rows of a gated group)
     bool any = false;
-    for (int j = 0; j < wt.nw; ++j)
-      if ((p0 + wt.wi[j].ts - 1) / wt.wi[j].ts < nwv[j]) any = true;
+    if (blockIdx.y < gd[blockIdx.z].rows_valid)
+      for (int j = 0; j < wt.nw; ++j)
+        if ((p0 + wt.wi[j].ts - 1) / wt.wi[j].ts < nwv[j]) any = true;
     if (!any) {
       for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
         gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
@@ -439,7 +441,7 @@ __global__ void k_segment_heads(GiantBuffers gb, WidthTable wt, unsigned nsamp) 
 // keeps the first maximum (maximum_first); the head sample writes the
 // begin, the last sample of a run the end.
 __global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp, unsigned n_dm,
-                                  GiantOut go, uint64_t* best) {
+                                  GiantOut go, uint64_t* best, const GroupDev* __restrict__ gd) {
   const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nsamp) return;
   const unsigned h = gb.head[k];
@@ -456,8 +458,9 @@ __global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp
     const uint64_t row = dmw / (uint64_t)wt.nw;  // group * n_dm + trial
     go.begin[g] = (i + ro) * sc;
     go.filter[g] = wi.filter_idx;
-    go.dm[g] = (uint32_t)(row % n_dm);
-    if (go.group) go.group[g] = (uint32_t)(row / n_dm);
+    const uint32_t grp = (uint32_t)(row / n_dm);
+    go.dm[g] = (uint32_t)(row % n_dm) + (gd ? gd[grp].trial0 : 0u);
+    if (go.group) go.group[g] = gd ? gd[grp].cgroup : grp;
   }
   if (k + 1 == nsamp || gb.head[k + 1]) go.end[g] = (i + 1 + ro) * sc;
 }
@@ -598,12 +601,12 @@ hipError_t launch_segment_heads(const GiantBuffers& gb, const WidthTable& wt, un
 
 hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
                                  unsigned ngiants, int n_dm, GiantOut go, uint64_t* best,
-                                 hipStream_t s) {
+                                 hipStream_t s, const GroupDev* gd) {
   if (!nsamp || !ngiants) return hipSuccess;
   hipLaunchKernelGGL(k_zero_u64, dim3((ngiants + 255) / 256), dim3(256), 0, s, best,
                      (size_t)ngiants);
   hipLaunchKernelGGL(k_segment_scatter, dim3((nsamp + 255) / 256), dim3(256), 0, s, gb, wt,
-                     nsamp, (unsigned)n_dm, go, best);
+                     nsamp, (unsigned)n_dm, go, best, gd);
   hipLaunchKernelGGL(k_segment_finish, dim3((ngiants + 255) / 256), dim3(256), 0, s, gb, wt,
                      ngiants, go, best);
   return hipGetLastError();
