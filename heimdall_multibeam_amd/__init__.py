@@ -295,7 +295,7 @@ class Pipeline:
         return buf.value.decode()
 
     STAGES = ("unpack", "dedisperse", "baseline", "rms", "scan", "block_rms", "giants",
-              "cluster", "total", "clean")
+              "cluster", "total", "clean", "proposal")
 
     def timings(self):
         out = (C.c_double * len(self.STAGES))()

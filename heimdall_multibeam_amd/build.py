@@ -23,7 +23,7 @@ SYNTH = os.path.join(HERE, "libhd_synth.so")
 
 HIP_SOURCES = ["hd_api.cpp", "formats.cpp", "k_dedisperse.hip", "k_normalise.hip",
                "k_filter.hip", "k_cluster.hip", "k_stages.hip", "k_clean.hip",
-               "k_proposal.hip", "k_cnn.hip"]
+               "k_proposal.hip", "k_cnn.hip", "k_synth.hip"]
 # command-line tools linked against the library: bin/<name> from app/<name>.cpp
 APPS = {"heimdall": "heimdall_main.cpp", "generate_dmlist": "generate_dmlist.cpp"}
 BIN = os.path.join(HERE, "bin")
@@ -59,7 +59,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     tmp = SYNTH + ".tmp"
     _run(["g++", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-std=c++17",
-          "-o", tmp, os.path.join(CSRC, "synth.cpp")])
+          "-I" + os.path.join(ROOT, "include"), "-o", tmp, os.path.join(CSRC, "synth.cpp")])
     os.replace(tmp, SYNTH)
     os.makedirs(BIN, exist_ok=True)
     for name, src in APPS.items():

@@ -19,40 +19,9 @@
 #include <cstring>
 #include <vector>
 
+#include "hd/synth.h"
 #include "hd_plan.hpp"
 
-extern "C" {
-
-struct hd_synth_pulse {
-  double t0;     // arrival sample at channel 0
-  float dm;      // dispersion measure
-  float width;   // FWHM in samples
-  float snr;     // optimal matched-filter S/N
-};
-
-struct hd_synth_tone {
-  int chan;
-  float amp_sigma;  // amplitude in units of sigma
-  float period;     // samples
-  float phase;      // radians
-};
-
-struct hd_synth_spec {
-  std::size_t nsamps, nchans, nbits;
-  float dt, f0, df;
-  std::uint64_t seed;
-  float mean, sigma;
-  const int* dead;          // nchans flags (1 = dead) or null
-  float dead_value;
-  const hd_synth_pulse* pulses;
-  std::size_t npulses;
-  const hd_synth_pulse* bursts;  // zero-DM: dm ignored
-  std::size_t nbursts;
-  const hd_synth_tone* tones;
-  std::size_t ntones;
-};
-
-}  // extern "C"
 
 namespace {
 

@@ -141,10 +141,12 @@ def delay_samples(w: Workload, dm: float, chan: int) -> int:
     return _synth().hd_synth_delay(dm, chan, w.dt, w.f0, w.df)
 
 
-def make_gulp(w: Workload, seed: int, nsamps: Optional[int] = None, dm_trials=None,
+def gulp_spec(w: Workload, seed: int, nsamps: Optional[int] = None, dm_trials=None,
               dead: Optional[np.ndarray] = None, dead_value: float = 128.0,
-              n_pulses: Optional[int] = None, return_truth: bool = False):
-    """Bytes [nsamps][nchans*nbits/8] for one beam of workload `w`."""
+              n_pulses: Optional[int] = None, pulses=None):
+    """The generator spec of one beam of workload `w` (pulses, tones and bursts
+    drawn from PCG64(seed) unless `pulses` gives (t0, dm, width, snr) tuples).
+    Returns (spec, keepalive, truth)."""
     ns = int(nsamps if nsamps is not None else w.nsamps)
     rng = np.random.Generator(np.random.PCG64(seed))
     if dm_trials is None:
@@ -152,16 +154,17 @@ def make_gulp(w: Workload, seed: int, nsamps: Optional[int] = None, dm_trials=No
     dm_trials = np.asarray(dm_trials, np.float32)
     max_delay = max(delay_samples(w, float(dm_trials[-1]), w.nchans - 1), 0)
     npul = w.n_pulses if n_pulses is None else n_pulses
-    pulses = []
     lo_t = w.boxcar_max
     hi_t = max(ns - max_delay - w.boxcar_max, lo_t + 1)
-    for _ in range(npul):
-        dm = rng.uniform(*w.pulse_dm)
-        dm = float(dm_trials[int(np.argmin(np.abs(dm_trials - dm)))])
-        width = float(rng.uniform(*w.pulse_width))
-        snr = float(rng.uniform(*w.pulse_snr))
-        t0 = float(rng.integers(lo_t, hi_t))
-        pulses.append((t0, dm, width, snr))
+    if pulses is None:
+        pulses = []
+        for _ in range(npul):
+            dm = rng.uniform(*w.pulse_dm)
+            dm = float(dm_trials[int(np.argmin(np.abs(dm_trials - dm)))])
+            width = float(rng.uniform(*w.pulse_width))
+            snr = float(rng.uniform(*w.pulse_snr))
+            t0 = float(rng.integers(lo_t, hi_t))
+            pulses.append((t0, dm, width, snr))
     tones = []
     if w.tone_frac > 0:
         ntone = int(round(w.tone_frac * w.nchans))
@@ -173,7 +176,6 @@ def make_gulp(w: Workload, seed: int, nsamps: Optional[int] = None, dm_trials=No
     for _ in range(w.n_bursts):
         bursts.append((float(rng.integers(lo_t, hi_t)), 0.0, float(rng.uniform(1.0, 16.0)),
                        w.burst_snr))
-    out = np.empty(ns * w.nchans * w.nbits // 8, np.uint8)
     spec = _Spec()
     spec.nsamps = ns
     spec.nchans = w.nchans
@@ -190,12 +192,37 @@ def make_gulp(w: Workload, seed: int, nsamps: Optional[int] = None, dm_trials=No
     P = (_Pulse * max(len(pulses), 1))(*[_Pulse(*p) for p in pulses])
     B = (_Pulse * max(len(bursts), 1))(*[_Pulse(*p) for p in bursts])
     T = (_Tone * max(len(tones), 1))(*[_Tone(*t) for t in tones])
+    keep += [P, B, T]
     spec.pulses, spec.npulses = P, len(pulses)
     spec.bursts, spec.nbursts = B, len(bursts)
     spec.tones, spec.ntones = T, len(tones)
+    return spec, keep, {"pulses": pulses, "tones": tones, "bursts": bursts}
+
+
+def make_gulp(w: Workload, seed: int, nsamps: Optional[int] = None, dm_trials=None,
+              dead: Optional[np.ndarray] = None, dead_value: float = 128.0,
+              n_pulses: Optional[int] = None, return_truth: bool = False, pulses=None):
+    """Bytes [nsamps][nchans*nbits/8] for one beam of workload `w` (host generator)."""
+    spec, keep, truth = gulp_spec(w, seed, nsamps, dm_trials, dead, dead_value, n_pulses,
+                                  pulses)
+    out = np.empty(spec.nsamps * w.nchans * w.nbits // 8, np.uint8)
     rc = _synth().hd_synth_filterbank(C.byref(spec), out.ctypes.data)
     if rc != 0:
         raise ValueError("hd_synth_filterbank failed")
     if return_truth:
-        return out, {"pulses": pulses, "tones": tones, "bursts": bursts}
+        return out, truth
     return out
+
+
+def make_gulp_device(w: Workload, seed: int, d_out, gpu_id: int = 0, **kw):
+    """The same gulp rendered on the GPU into device memory at address d_out
+    (libheimdall_mi355x's hd_synth_filterbank_device); returns the truth."""
+    from . import lib, HdError
+    L = lib()
+    L.hd_synth_filterbank_device.argtypes = [C.POINTER(_Spec), C.c_void_p, C.c_int]
+    L.hd_synth_filterbank_device.restype = C.c_int
+    spec, keep, truth = gulp_spec(w, seed, **kw)
+    rc = L.hd_synth_filterbank_device(C.byref(spec), C.c_void_p(int(d_out)), gpu_id)
+    if rc != 0:
+        raise HdError(rc, "hd_synth_filterbank_device")
+    return truth
