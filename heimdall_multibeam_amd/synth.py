@@ -79,6 +79,7 @@ class Workload:
     burst_snr: float = 20.0
     mean: float = 128.0
     sigma: float = 16.0
+    min_tscrunch_width: int = 4096
 
 
 # The workloads the search is measured and checked on.
@@ -96,6 +97,21 @@ CONFIGS = {
     # the rest held at the noise median, 0-3 pulses per beam, DM 0-2000, boxcars 1..1024
     "batch_320ch": Workload("batch_320ch", 320, 8, 655.36e-6, 864.953125, -0.09375, 16384, 0.0,
                             2000.0, 1024, 6.0, 3, (0.0, 2000.0), (1.0, 64.0), (6.0, 40.0)),
+    # learned proposal training data: the full_1024ch setup with pulses of DM 0-2000,
+    # width 1-512 samples and S/N 4-30 (50 per gulp).  The proposal workloads search
+    # with -min_tscrunch_width 8192 (every boxcar at full resolution): with the
+    # default 4096 the 4096-sample boxcar's RMS is taken over a buffer half filled
+    # with the previous width's normalised values (a reference quirk kept by the
+    # search), which underestimates it and fills every DM range with spurious
+    # high-S/N candidates at that width
+    "learned_1024ch": Workload("learned_1024ch", 1024, 8, 64e-6, 1517.8359375, -0.328125, 262144,
+                               0.0, 2000.0, 4096, 6.0, 50, (0.0, 2000.0), (1.0, 512.0),
+                               (4.0, 30.0), 0.05, 10.0, 3, 20.0, 128.0, 16.0, 8192),
+    # gated multibeam: 4096 ch over 1200-1600 MHz, 64 us, 2-bit, gulp 131072, DM 0-3000,
+    # boxcars 1..4096, 0-5 pulses per beam (width 1-1024, S/N 6-40), 2% tone channels
+    "gated_4096ch": Workload("gated_4096ch", 4096, 2, 64e-6, 1599.951171875, -0.09765625, 131072,
+                             0.0, 3000.0, 4096, 6.0, 5, (0.0, 3000.0), (1.0, 1024.0),
+                             (6.0, 40.0), 0.02, 10.0, 0, 20.0, 1.5, 1.0, 8192),
 }
 
 # beams and per-beam ranges of the ragged batch
