@@ -121,11 +121,11 @@ struct hd_pipeline_t {
   std::vector<float> co_dms, co_kc;
   size_t co_max_delay = 0, co_wlen = 0;
   std::vector<int> co_live_cached;
-  DevBuf co_delays, co_pairs, co_xc, co_plane, co_pts, co_rms, co_brms, co_bf, co_gdev;
+  DevBuf co_delays, co_pairs, co_xc, co_plane, co_pts, co_rms, co_brms, co_bf, co_bw, co_gdev;
   DevBuf co_tiles, co_scores, co_toff, co_ntpos, co_ncols, co_flags, co_pos;
   std::vector<hd::GroupDev> co_hgd;
   int co_G = 0, co_rows = 0, co_units = 0;
-  size_t co_bpitch = 0;
+  size_t co_bpitch = 0, co_wpitch = 0;
   std::vector<uint32_t> co_toff_h, co_ntpos_h, co_ncols_h;
   std::vector<uint8_t> prop_flags;      // per gulp x unit of the last gated call
   int prop_units = 0;
@@ -732,10 +732,13 @@ hd_error coarse_pass(hd_pipeline pl, int G, size_t xpitch, size_t nbits, hipStre
   HD_CHECK(hd::launch_beam_rms(pl->co_plane.as<float>(), pl->co_pts.as<float>(), cg[0].sh, nc,
                                pl->co_rms.as<float>(), s, pl->co_brms.as<float>(), words, cga));
   const size_t bpitch = round_up(nout_max, 64);
+  const size_t wpitch = round_up(nout_max / 4 + 1, 64);
   HD_CHECK(pl->co_bf.ensure(rows * bpitch * sizeof(uint16_t)));
+  HD_CHECK(pl->co_bw.ensure(rows * wpitch * sizeof(uint16_t)));
   HD_CHECK(hd::launch_coarse_bf16(pl->co_plane.as<float>(), ppc, pl->co_pts.as<float>(),
                                   cga.prow, pl->co_rms.as<float>(), 1, cga.gd, G, nc,
-                                  pl->co_bf.as<uint16_t>(), bpitch, s));
+                                  pl->co_bf.as<uint16_t>(), bpitch, pl->co_bw.as<uint16_t>(),
+                                  wpitch, s));
   // tile grid
   const int units = (int)((pl->dm_list.size() + HD_PROPOSAL_UNIT - 1) / HD_PROPOSAL_UNIT);
   pl->co_toff_h.assign(G + 1, 0);
@@ -762,6 +765,7 @@ hd_error coarse_pass(hd_pipeline pl, int G, size_t xpitch, size_t nbits, hipStre
   pl->co_rows = nc;
   pl->co_units = units;
   pl->co_bpitch = bpitch;
+  pl->co_wpitch = wpitch;
   return HD_NO_ERROR;
 }
 
@@ -769,6 +773,8 @@ hd::TileGrid tile_grid(hd_pipeline pl) {
   hd::TileGrid tg;
   tg.bf = pl->co_bf.as<uint16_t>();
   tg.bpitch = pl->co_bpitch;
+  tg.bw = pl->co_bw.as<uint16_t>();
+  tg.wpitch = pl->co_wpitch;
   tg.n_rows = pl->co_rows;
   tg.ncols = pl->co_ncols.as<uint32_t>();
   tg.n_units = pl->co_units;
@@ -784,7 +790,7 @@ hd_error score_tiles(hd_pipeline pl, hipStream_t s) {
   if (!pl->net || pl->co_G == 0) return HD_INVALID_POINTER;
   const size_t total = pl->co_toff_h.back();
   const size_t B = hd::cnn_max_batch(pl->net);
-  HD_CHECK(pl->co_tiles.ensure(B * HD_PROPOSAL_TILE * HD_PROPOSAL_TILE * sizeof(uint16_t)));
+  HD_CHECK(pl->co_tiles.ensure(B * HD_PROPOSAL_TILE * HD_PROPOSAL_TILE * 2 * sizeof(uint16_t)));
   HD_CHECK(pl->co_scores.ensure(std::max<size_t>(total, 1) * sizeof(float)));
   const hd::TileGrid tg = tile_grid(pl);
   for (size_t first = 0; first < total; first += B) {
@@ -1460,7 +1466,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->o_ind, &pl->o_begin, &pl->o_end, &pl->o_filter, &pl->o_dm,
                       &pl->o_members, &pl->gdev, &pl->g_group, &pl->o_group, &pl->c_sind,
                       &pl->c_sfd, &pl->rdev, &pl->co_delays, &pl->co_pairs, &pl->co_xc,
-                      &pl->co_plane, &pl->co_pts, &pl->co_rms, &pl->co_brms, &pl->co_bf,
+                      &pl->co_plane, &pl->co_pts, &pl->co_rms, &pl->co_brms, &pl->co_bf, &pl->co_bw,
                       &pl->co_gdev, &pl->co_tiles, &pl->co_scores, &pl->co_toff, &pl->co_ntpos,
                       &pl->co_ncols, &pl->co_flags, &pl->co_pos};
     for (DevBuf* b : bufs) b->release();
@@ -1618,15 +1624,18 @@ extern "C" hd_error hd_proposal_extract(hd_pipeline pl, const unsigned* pos, hd_
   return HD_NO_ERROR;
 }
 
-extern "C" hd_size hd_proposal_get_plane(hd_pipeline pl, unsigned short* host, hd_size cap,
-                                         hd_size* rows, hd_size* cols) {
-  if (!pl || pl->co_G == 0) return 0;
-  const size_t r = (size_t)pl->co_rows, c = pl->co_ncols_h[0];
+extern "C" hd_size hd_proposal_get_plane(hd_pipeline pl, int channel, unsigned short* host,
+                                         hd_size cap, hd_size* rows, hd_size* cols) {
+  if (!pl || pl->co_G == 0 || channel < 0 || channel > 1) return 0;
+  const size_t r = (size_t)pl->co_rows;
+  const size_t c = channel ? pl->co_ncols_h[0] / 4 : pl->co_ncols_h[0];
+  const size_t pitch = channel ? pl->co_wpitch : pl->co_bpitch;
+  const void* src = channel ? pl->co_bw.p : pl->co_bf.p;
   if (rows) *rows = r;
   if (cols) *cols = c;
   if (host && cap >= r * c) {
     DeviceGuard dg(pl->device);
-    if (hipMemcpy2D(host, c * sizeof(uint16_t), pl->co_bf.p, pl->co_bpitch * sizeof(uint16_t),
+    if (hipMemcpy2D(host, c * sizeof(uint16_t), src, pitch * sizeof(uint16_t),
                     c * sizeof(uint16_t), r, hipMemcpyDeviceToHost) != hipSuccess)
       return 0;
   }

@@ -263,17 +263,22 @@ hipError_t launch_merge(const float* peak, const uint64_t* ind, const uint64_t* 
 // gd[g].nsamps is the group's coarse sample count.
 hipError_t launch_decimate(const uint8_t* x, size_t xpitch, int n_live, int nbits, uint8_t* xc,
                            size_t xpitch_c, const GroupDev* gd, int G, hipStream_t s);
-// Normalised coarse plane as bfloat16: bf[(g*n_rows + m)*bpitch + t].
+// Normalised coarse plane as bfloat16, bf[(g*n_rows + m)*bpitch + t], and
+// its wide plane bw[(g*n_rows + m)*wpitch + j] = (z[4j] + .. + z[4j+3]) / 2.
 hipError_t launch_coarse_bf16(const float* plane, size_t ppitch, const float* pts, size_t prow,
                               const float* rms, unsigned nbmax, const GroupDev* gd, int G,
-                              int n_rows, uint16_t* bf, size_t bpitch, hipStream_t s);
-// Tiles of the coarse plane.  Explicit: n tiles at (u, v) = pos[2i..2i+1] of
+                              int n_rows, uint16_t* bf, size_t bpitch, uint16_t* bw,
+                              size_t wpitch, hipStream_t s);
+// Two-channel tiles [64][64][2] of the coarse and wide planes.  Explicit: n
+// tiles at (u, v) = pos[2i..2i+1] of
 // group 0.  Enumerated: tiles first .. first+n-1 of the order (g, u, v),
 // tile_off[g] the first tile of group g (G+1 entries), n_tpos[g] its time
 // positions, n_units units per group.
 struct TileGrid {
   const uint16_t* bf;
   size_t bpitch;
+  const uint16_t* bw;         // wide plane
+  size_t wpitch;
   int n_rows;                 // coarse rows per group
   const uint32_t* ncols;      // coarse samples of each group's plane
   int n_units;

@@ -32,21 +32,23 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TILE = 64, CH = 32, HID = 64, FEAT = 16 * CH;
-constexpr int K1 = 16;          // layer-1 reduction (9 taps, padded)
+constexpr int CIN1 = 2;         // input channels: coarse plane, 4x wider time scale
+constexpr int KL1 = 9 * CIN1;   // layer-1 reduction (tap*2 + channel)
+constexpr int K1 = 32;          // ... padded to the MFMA step
 constexpr int KC = 9 * CH;      // 288
 
 // Flat float32 parameter offsets.
-constexpr size_t O_W1 = 0, O_B1 = O_W1 + CH * 9;
+constexpr size_t O_W1 = 0, O_B1 = O_W1 + CH * KL1;
 constexpr size_t O_W2 = O_B1 + CH, O_B2 = O_W2 + CH * KC;
 constexpr size_t O_W3 = O_B2 + CH, O_B3 = O_W3 + CH * KC;
 constexpr size_t O_W4 = O_B3 + CH, O_B4 = O_W4 + CH * KC;
 constexpr size_t O_F1W = O_B4 + CH, O_F1B = O_F1W + (size_t)HID * FEAT;
 constexpr size_t O_F2W = O_F1B + HID, O_F2B = O_F2W + HID;
 constexpr size_t NPARAM = O_F2B + 1;
-static_assert(NPARAM == 60961, "parameter count");
+static_assert(NPARAM == 61249, "parameter count");
 
 // bfloat16 weight copies (element offsets in one uint16 array).
-constexpr size_t Q_W1 = 0;                          // [32][16]
+constexpr size_t Q_W1 = 0;                          // [32][32]
 constexpr size_t Q_W2 = Q_W1 + CH * K1;             // [32][288] x3
 constexpr size_t Q_W3 = Q_W2 + CH * KC;
 constexpr size_t Q_W4 = Q_W3 + CH * KC;
@@ -125,7 +127,7 @@ hipError_t run_mm(const P& p, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Forward stages.
 
-// Layer 1: x [B][64][64] -> y1 [B][32][32][32].  k = tap (9 of 16 used).
+// Layer 1: x [B][64][64][2] -> y1 [B][32][32][32].  k = tap*2 + channel (18 of 32 used).
 struct Conv1Fwd {
   long M, N, K, kchunk;
   const uint16_t* x;
@@ -139,11 +141,13 @@ struct Conv1Fwd {
     bf16x4 v;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int tap = (int)k + e;
+      const int kk = (int)k + e;
+      const int tap = kk >> 1, ci = kk & 1;
       uint16_t h = 0;
       if (tap < 9) {
         const int iy = 2 * oy - 1 + tap / 3, ix = 2 * ox - 1 + tap % 3;
-        if (iy >= 0 && iy < TILE && ix >= 0 && ix < TILE) h = x[(b * TILE + iy) * TILE + ix];
+        if (iy >= 0 && iy < TILE && ix >= 0 && ix < TILE)
+          h = x[((b * TILE + iy) * TILE + ix) * CIN1 + ci];
       }
       v[e] = (short)h;
     }
@@ -362,14 +366,14 @@ struct ConvDgrad {
 
 // Weight gradients of conv layer l with the bias as column KL: M = 32 output
 // channels, N = KL + 1, K = pixels of the layer output; float32 partials.
-// CIN == 1 reads the input tiles, CIN == 32 an NHWC activation.
+// CIN == 2 reads the input tiles, CIN == 32 an NHWC activation.
 template <int CIN>
 struct ConvWgrad {
   long M, N, K, kchunk;
   int H, Ho;
   long NP;
   const uint16_t* d;     // [B][Ho][Ho][32]
-  const uint16_t* xin;   // CIN == 1: [B][64][64]; else [B][H][H][32]
+  const uint16_t* xin;   // CIN == 2: [B][64][64][2]; else [B][H][H][32]
   float* part;
   static constexpr int KL = 9 * CIN;
   __device__ bf16x4 A(long m, long k) const {
@@ -423,7 +427,7 @@ __global__ void k_quantize(const float* __restrict__ p, uint16_t* __restrict__ q
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < (size_t)CH * K1) {
     const int co = (int)(i / K1), t = (int)(i % K1);
-    q[Q_W1 + i] = t < 9 ? f2bf(p[O_W1 + co * 9 + t]) : (uint16_t)0;
+    q[Q_W1 + i] = t < KL1 ? f2bf(p[O_W1 + co * KL1 + t]) : (uint16_t)0;
   }
   if (i < (size_t)CH * KC) {
     const size_t ow[3] = {O_W2, O_W3, O_W4};
@@ -511,7 +515,7 @@ size_t part_words_for(size_t B) {
   for (int l = 0; l < 4; ++l) {
     const long P = (long)B * pix[l];
     const long splits = (P + WG_CHUNK - 1) / WG_CHUNK;
-    const long np = l == 0 ? 16 : 304;
+    const long np = l == 0 ? K1 : 304;
     w = std::max<size_t>(w, (size_t)(splits * CH * np));
   }
   const long s1 = ((long)B + F1_CHUNK - 1) / F1_CHUNK;
@@ -603,7 +607,7 @@ extern "C" hd_error hd_cnn_init_params(hd_cnn net, unsigned long long seed) {
     std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / fan_in));
     for (size_t i = 0; i < n; ++i) p[off + i] = (float)nd(rng);
   };
-  fill(O_W1, CH * 9, 9);
+  fill(O_W1, CH * KL1, KL1);
   fill(O_W2, CH * KC, KC);
   fill(O_W3, CH * KC, KC);
   fill(O_W4, CH * KC, KC);
@@ -741,9 +745,9 @@ extern "C" hd_error hd_cnn_backward(hd_cnn net, const hd_float* d_labels,
   }
   {  // conv1 weights (input tiles)
     const long P = n * 1024;
-    ConvWgrad<1> pw{CH, 10, P, WG_CHUNK, TILE, 32, 16, net->d1, net->x, net->part};
+    ConvWgrad<CIN1> pw{CH, KL1 + 1, P, WG_CHUNK, TILE, 32, K1, net->d1, net->x, net->part};
     CK(run_mm<1, 1>(pw, s));
-    CK(reduce((P + WG_CHUNK - 1) / WG_CHUNK, CH, 16, 9, g + O_W1, g + O_B1));
+    CK(reduce((P + WG_CHUNK - 1) / WG_CHUNK, CH, K1, KL1, g + O_W1, g + O_B1));
   }
   if (loss) CK(hipMemcpyAsync(loss, net->loss, sizeof(float), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));

@@ -5,9 +5,10 @@
 // time on every 8th trial: the unpacked channel rows are summed over 16
 // samples (k_decimate), dedispersed by k_dedisperse16 with the coarse delay
 // table, baseline-removed and RMS-normalised by the search's kernels, and
-// stored as bfloat16 (k_coarse_bf16).  Tiles are 64x64 windows of it
-// (k_extract_tiles); a unit of 128 trials is searched when any of its tiles
-// scores above the threshold (k_unit_flags).
+// stored as bfloat16 (k_coarse_bf16) together with a 4x wider time scale
+// (the mean of 4 coarse samples, times 2).  Tiles are 64x64 windows of both,
+// centred alike (k_extract_tiles); a unit of 128 trials is searched when any
+// of its tiles scores above the threshold (k_unit_flags).
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
 #include "hd_series.hpp"
@@ -21,6 +22,8 @@ constexpr int TROW0 = 24;       // coarse rows above a tile's centre band
 constexpr int TCOL0 = 16;       // coarse samples before its centre span
 constexpr int UROWS = 16;       // coarse rows per unit
 constexpr int VSTEP = 32;       // coarse samples per time position
+constexpr int WIDE = 4;         // coarse samples per wide sample
+constexpr int WCOL0 = 28;       // wide samples before a tile centre's first (8v - 28)
 
 // One output byte per thread: 16 input bytes (one 16-byte load, aligned since
 // both pitches are multiples of 256 and the group's slot starts on a row).
@@ -54,7 +57,8 @@ __global__ __launch_bounds__(256) void k_coarse_bf16(const float* __restrict__ p
                                                      unsigned nbmax,
                                                      const GroupDev* __restrict__ gd,
                                                      int n_rows, uint16_t* __restrict__ bf,
-                                                     size_t bpitch) {
+                                                     size_t bpitch, uint16_t* __restrict__ bw,
+                                                     size_t wpitch) {
   const int g = blockIdx.z;
   const size_t row = (size_t)g * n_rows + blockIdx.y;
   const SeriesShape sh = gd[g].sh;
@@ -62,14 +66,24 @@ __global__ __launch_bounds__(256) void k_coarse_bf16(const float* __restrict__ p
   const float* R = rms + row * nbmax;
   const float* xr = plane + row * ppitch;
   uint16_t* o = bf + row * bpitch;
+  uint16_t* ow = bw + row * wpitch;
+  auto z_at = [&](size_t t) {
+    const float z = normalised_at(xr[t], P, R, sh, t);
+    return isfinite(z) ? z : 0.f;
+  };
   for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < bpitch;
-       t += (size_t)gridDim.x * 256) {
+       t += (size_t)gridDim.x * 256)
+    o[t] = t < sh.nout ? f2bf(z_at(t)) : (uint16_t)0;
+  const size_t nw = sh.nout / WIDE;
+  for (size_t j = (size_t)blockIdx.x * 256 + threadIdx.x; j < wpitch;
+       j += (size_t)gridDim.x * 256) {
     uint16_t h = 0;
-    if (t < sh.nout) {
-      const float z = normalised_at(xr[t], P, R, sh, t);
-      h = isfinite(z) ? f2bf(z) : (uint16_t)0;
+    if (j < nw) {
+      float s = 0.f;
+      for (int i = 0; i < WIDE; ++i) s += z_at(WIDE * j + i);
+      h = f2bf(0.5f * s);
     }
-    o[t] = h;
+    ow[j] = h;
   }
 }
 
@@ -96,18 +110,23 @@ __global__ __launch_bounds__(256) void k_extract_tiles(TileGrid tg, const uint32
     v = (int)(k % nv);
   }
   const long ncols = (long)tg.ncols[g];
+  const long nwide = ncols / WIDE;
   const uint16_t* base = tg.bf + (size_t)g * tg.n_rows * tg.bpitch;
-  uint16_t* out = tiles + i * (size_t)(TS * TS);
+  const uint16_t* wbase = tg.bw + (size_t)g * tg.n_rows * tg.wpitch;
+  uint32_t* out = reinterpret_cast<uint32_t*>(tiles + i * (size_t)(TS * TS * 2));
   const int r0 = u * UROWS - TROW0;
   const long c0 = (long)v * VSTEP - TCOL0;
+  const long w0 = (long)v * (VSTEP / WIDE) - WCOL0;
   for (int e = threadIdx.x; e < TS * TS; e += 256) {
     const int y = e / TS, xx = e % TS;
     const int row = r0 + y;
-    const long col = c0 + xx;
-    uint16_t h = 0;
-    if (row >= 0 && row < tg.n_rows && col >= 0 && col < ncols)
-      h = base[(size_t)row * tg.bpitch + col];
-    out[e] = h;
+    const long col = c0 + xx, wc = w0 + xx;
+    uint32_t h0 = 0, h1 = 0;
+    if (row >= 0 && row < tg.n_rows) {
+      if (col >= 0 && col < ncols) h0 = base[(size_t)row * tg.bpitch + col];
+      if (wc >= 0 && wc < nwide) h1 = wbase[(size_t)row * tg.wpitch + wc];
+    }
+    out[e] = h0 | (h1 << 16);
   }
 }
 
@@ -141,12 +160,13 @@ hipError_t launch_decimate(const uint8_t* x, size_t xpitch, int n_live, int nbit
 
 hipError_t launch_coarse_bf16(const float* plane, size_t ppitch, const float* pts, size_t prow,
                               const float* rms, unsigned nbmax, const GroupDev* gd, int G,
-                              int n_rows, uint16_t* bf, size_t bpitch, hipStream_t s) {
+                              int n_rows, uint16_t* bf, size_t bpitch, uint16_t* bw,
+                              size_t wpitch, hipStream_t s) {
   if (G <= 0 || n_rows <= 0) return hipSuccess;
   size_t gx = (bpitch + 255) / 256;
   if (gx > 64) gx = 64;
   hipLaunchKernelGGL(k_coarse_bf16, dim3((unsigned)gx, n_rows, G), dim3(256), 0, s, plane, ppitch,
-                     pts, prow, rms, nbmax, gd, n_rows, bf, bpitch);
+                     pts, prow, rms, nbmax, gd, n_rows, bf, bpitch, bw, wpitch);
   return hipGetLastError();
 }
 

@@ -54,7 +54,7 @@ def lib():
     L.hd_proposal_coarse.restype = C.c_int
     L.hd_proposal_extract.argtypes = [vp, C.POINTER(C.c_uint), sz, vp]
     L.hd_proposal_extract.restype = C.c_int
-    L.hd_proposal_get_plane.argtypes = [vp, C.POINTER(C.c_uint16), sz, C.POINTER(sz),
+    L.hd_proposal_get_plane.argtypes = [vp, C.c_int, C.POINTER(C.c_uint16), sz, C.POINTER(sz),
                                         C.POINTER(sz)]
     L.hd_proposal_get_plane.restype = sz
     L.hd_proposal_scores.argtypes = [vp, fp, sz]
@@ -110,7 +110,7 @@ class Cnn:
         return out
 
     def forward(self, d_tiles, n):
-        """Logits of n device tiles (address of bfloat16 [n][64][64]) -> numpy."""
+        """Logits of n device tiles (address of bfloat16 [n][64][64][2]) -> numpy."""
         _check(lib().hd_cnn_forward(self.handle, C.c_void_p(int(d_tiles)), n, None),
                "hd_cnn_forward")
         out = np.zeros(n, np.float32)
@@ -166,15 +166,17 @@ def extract(pipeline, positions, d_tiles):
     return n
 
 
-def plane(pipeline):
-    """The last coarse plane as bfloat16 bits [rows, cols] (uint16)."""
+def plane(pipeline, channel=0):
+    """The last coarse plane (channel 0) or its wide plane (channel 1) as
+    bfloat16 bits [rows, cols] (uint16)."""
     L = lib()
     rows, cols = C.c_size_t(0), C.c_size_t(0)
-    n = L.hd_proposal_get_plane(pipeline.handle, None, 0, C.byref(rows), C.byref(cols))
+    n = L.hd_proposal_get_plane(pipeline.handle, channel, None, 0, C.byref(rows), C.byref(cols))
     out = np.zeros(n, np.uint16)
     if n:
-        L.hd_proposal_get_plane(pipeline.handle, out.ctypes.data_as(C.POINTER(C.c_uint16)), n,
-                                C.byref(rows), C.byref(cols))
+        L.hd_proposal_get_plane(pipeline.handle, channel,
+                                out.ctypes.data_as(C.POINTER(C.c_uint16)), n, C.byref(rows),
+                                C.byref(cols))
     return out.reshape(rows.value, cols.value)
 
 

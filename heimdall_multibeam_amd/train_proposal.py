@@ -69,10 +69,15 @@ class Setup:
         elif kind == "rfi":
             wk = w if w.tone_frac > 0 or w.n_bursts > 0 else dataclasses.replace(w, tone_frac=0.02)
             npul = 0
+        elif w.name == "learned_1024ch":
+            # pulse gulps carry no RFI: the RFI-only gulps hold it (with the
+            # tones of the full_1024ch setup the search finds almost none of
+            # the injected pulses, and their tiles would be labelled empty)
+            wk = dataclasses.replace(w, tone_frac=0.0, n_bursts=0)
+            npul = None
         else:
             wk = w
-            npul = None if w.name == "learned_1024ch" else int(
-                np.random.default_rng(seed).integers(0, w.n_pulses + 1))
+            npul = int(np.random.default_rng(seed).integers(0, w.n_pulses + 1))
         truth = synth.make_gulp_device(wk, seed, self.fb.address, dm_trials=self.dms,
                                        n_pulses=npul)
         self.pl.execute(self.fb.address, w.nsamps, w.nbits, device=True)
@@ -89,15 +94,34 @@ def cand_cells(c, nu, nv, snr_min):
     return u[ok], v[ok]
 
 
+def positive_cells(c, nu, nv, snr_min):
+    """Tiles whose window holds a candidate peak: its unit, and every time
+    position whose 64 coarse samples ([512v - 256, 512v + 768) in samples)
+    contain the peak sample."""
+    keep = c["peak"] >= snr_min
+    pos = set()
+    span = P.TIME_DECIM * P.TSTEP
+    for dm, ind in zip(c["dm"][keep], c["ind"][keep]):
+        u = int(dm) // P.UNIT
+        if u >= nu:
+            continue
+        lo = max(0, -(-(int(ind) - 3 * span // 2 + 1) // span))
+        hi = min(nv - 1, (int(ind) + span // 2) // span)
+        for v in range(lo, hi + 1):
+            pos.add((u, v))
+    return pos
+
+
 def choose_tiles(rng, c, nu, nv, snr_min):
-    """Positives (candidate peaks) plus their neighbours and random negatives."""
-    u, v = cand_cells(c, nu, nv, snr_min)
-    pos = set(zip(u.tolist(), v.tolist()))
+    """Positives (tiles holding a candidate peak), the tiles of the same time
+    positions in the neighbouring units, and random negatives.  Tiles next to
+    a positive in time are not used as negatives: they hold the same pulse."""
+    pos = positive_cells(c, nu, nv, snr_min)
     neg = set()
     for (a, b) in pos:
-        for da, db in ((-1, 0), (1, 0), (0, -1), (0, 1)):
-            q = (a + da, b + db)
-            if 0 <= q[0] < nu and 0 <= q[1] < nv and q not in pos:
+        for da in (-1, 1):
+            q = (a + da, b)
+            if 0 <= q[0] < nu and q not in pos:
                 neg.add(q)
     n_rand = 2 * len(pos) + 8
     for _ in range(n_rand):
@@ -173,7 +197,7 @@ def collect(st, seeds, kinds, pick, dev, what):
     the (u, v) tiles and labels to keep.  Returns (tiles tensor, labels)."""
     import torch
     from .hip import copy_device
-    buf = DeviceBuffer(4096 * 64 * 64 * 2)
+    buf = DeviceBuffer(4096 * 64 * 64 * 2 * 2)
     tiles, labels = [], []
     n_pos = n_tiles = n_cand = 0
     for i, (sd, kd) in enumerate(zip(seeds, kinds)):
@@ -183,8 +207,8 @@ def collect(st, seeds, kinds, pick, dev, what):
         for k0 in range(0, len(pos), 4096):
             chunk = pos[k0:k0 + 4096]
             P.extract(st.pl, chunk, buf.address)
-            t = torch.empty((len(chunk), 64, 64), dtype=torch.int16, device=dev)
-            copy_device(t.data_ptr(), buf.address, len(chunk) * 64 * 64 * 2)
+            t = torch.empty((len(chunk), 64, 64, 2), dtype=torch.int16, device=dev)
+            copy_device(t.data_ptr(), buf.address, len(chunk) * 64 * 64 * 2 * 2)
             tiles.append(t)
         labels.append(np.asarray(lab, np.float32))
         n_pos += int(sum(lab))
@@ -265,9 +289,11 @@ def main(argv=None):
 
         def pick(c, nu, nv, st=st, extra=extra):
             pos, lab = choose_tiles(rng, c, nu, nv, st.w.detect_thresh)
+            have = set(pos)
             for _ in range(extra):
                 q = (int(rng.integers(nu)), int(rng.integers(nv)))
-                if q not in pos:
+                if q not in have:
+                    have.add(q)
                     pos.append(q)
                     lab.append(0.0)
             return pos, lab
@@ -291,9 +317,9 @@ def main(argv=None):
 
         def pick_hard(c, nu, nv, st=st):
             sc = P.scores(st.pl, nu, nv)
-            u, v = cand_cells(c, nu, nv, st.w.detect_thresh)
             mask = np.zeros((nu, nv), bool)
-            mask[u, v] = True
+            for (u, v) in positive_cells(c, nu, nv, st.w.detect_thresh):
+                mask[u, max(v - 1, 0):v + 2] = True   # and the same pulse's neighbours
             sc = np.where(mask, -np.inf, sc)
             flat = np.argsort(sc, axis=None)[::-1][:a.mine_per]
             pos = [(int(f // nv), int(f % nv)) for f in flat if np.isfinite(sc.flat[f])]

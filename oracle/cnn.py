@@ -5,7 +5,7 @@ checker of the HIP forward and backward kernels; the product never imports it.
 
 The reference has no learned stage, so nothing here follows a reference file.
 The network and its quantisation points are defined by the product's kernels:
-  input  x: 64x64 tile, bfloat16
+  input  x: 64x64 tile of 2 channels (coarse plane, wide plane), bfloat16
   conv_l (l = 1..4): 3x3, pad 1, stride 2, 32 output channels, ReLU;
          64 -> 32 -> 16 -> 8 -> 4 pixels, activations stored as bfloat16 (NHWC)
   fc1: 512 -> 64, ReLU, stored as bfloat16;  fc2: 64 -> 1 logit (float32)
@@ -18,7 +18,7 @@ as bfloat16, and accumulates weight gradients in float32.
 finite-difference check of the backward pass uses that mode.
 
 Flat parameter layout (float32), in order:
-  w1[32][9]  b1[32]  w2[32][9][32]  b2[32]  w3 ..  b3  w4 ..  b4
+  w1[32][9][2]  b1[32]  w2[32][9][32]  b2[32]  w3 ..  b3  w4 ..  b4
   fc1_w[64][512]  fc1_b[64]  fc2_w[64]  fc2_b[1]
 with conv weights indexed [out channel][ky*3 + kx][in channel] and the fc1
 input flattened as (y*4 + x)*32 + channel.
@@ -28,7 +28,8 @@ import numpy as np
 TILE = 64
 C = 32
 HID = 64
-SHAPES = [("w1", (C, 9)), ("b1", (C,)),
+CIN = 2
+SHAPES = [("w1", (C, 9, CIN)), ("b1", (C,)),
           ("w2", (C, 9, C)), ("b2", (C,)),
           ("w3", (C, 9, C)), ("b3", (C,)),
           ("w4", (C, 9, C)), ("b4", (C,)),
@@ -92,7 +93,7 @@ def _col2im(g, H, W):
 
 
 def forward(p, x, quantize=True):
-    """p: flat parameters; x: [B,64,64] tiles.  Returns (logits [B], cache)."""
+    """p: flat parameters; x: [B,64,64,2] tiles.  Returns (logits [B], cache)."""
     P = unflatten(np.asarray(p, np.float64 if not quantize else np.float32))
     if quantize:
         q = bf16
@@ -100,13 +101,10 @@ def forward(p, x, quantize=True):
     else:
         q = lambda a: np.asarray(a, np.float64)
         dt = np.float64
-    a = q(np.asarray(x, np.float32)).astype(dt)[..., None]     # [B,64,64,1]
+    a = q(np.asarray(x, np.float32)).astype(dt)                 # [B,64,64,2]
     acts = [a]
     for l in range(1, 5):
-        w = P["w%d" % l]
-        if l == 1:
-            w = w[:, :, None]
-        wq = q(w).astype(dt)                                    # [32,9,Ci]
+        wq = q(P["w%d" % l]).astype(dt)                         # [32,9,Ci]
         cols = _im2col(acts[-1])                                 # [B,Ho,Wo,9,Ci]
         v = np.tensordot(cols, wq, axes=([3, 4], [1, 2]))        # [B,Ho,Wo,32]
         if quantize:
@@ -165,7 +163,7 @@ def backward(cache, labels, weights):
         x_in = acts[l - 1]
         cols = _im2col(x_in)                                     # [B,Ho,Wo,9,Ci]
         gw = np.tensordot(d, cols, axes=([0, 1, 2], [0, 1, 2]))  # [32,9,Ci]
-        G["w%d" % l] = gw[:, :, 0] if l == 1 else gw
+        G["w%d" % l] = gw
         G["b%d" % l] = d.sum((0, 1, 2))
         if l == 1:
             break

@@ -32,7 +32,7 @@ def _params(seed, P):
     rng = np.random.default_rng(seed)
     d = {}
     for name, shp in cnn.SHAPES:
-        fan_in = {"w1": 9, "fc1_w": 512, "fc2_w": 64}.get(name, 288)
+        fan_in = {"w1": 18, "fc1_w": 512, "fc2_w": 64}.get(name, 288)
         if name.startswith("b") or name.endswith("_b"):
             d[name] = rng.normal(0, 0.05, shp)
         else:
@@ -45,10 +45,11 @@ def _params(seed, P):
 def _tiles(seed, n):
     from oracle import cnn
     rng = np.random.default_rng(seed)
-    x = rng.normal(0, 1, (n, 64, 64)).astype(np.float32)
+    x = rng.normal(0, 1, (n, 64, 64, 2)).astype(np.float32)
     for b in range(0, n, 2):
         y, t = rng.integers(8, 56, 2)
-        x[b, y - 3:y + 3, t - 2:t + 2] += 3.0
+        x[b, y - 3:y + 3, t - 2:t + 2, 0] += 3.0
+        x[b, y - 3:y + 3, t // 4 + 20:t // 4 + 22, 1] += 2.0
     return cnn.bf16(x), cnn.bf16_bits(x)
 
 
@@ -145,11 +146,11 @@ def test_training_reduces_loss(P):
     from oracle import cnn
     rng = np.random.default_rng(41)
     n = 256
-    x = rng.normal(0, 1, (n, 64, 64)).astype(np.float32)
+    x = rng.normal(0, 1, (n, 64, 64, 2)).astype(np.float32)
     lab = (np.arange(n) % 2).astype(np.float32)
     for b in np.nonzero(lab)[0]:
         y, t = rng.integers(4, 60, 2)
-        x[b, y - 2:y + 2, t - 2:t + 2] += 2.5
+        x[b, y - 2:y + 2, t - 2:t + 2, 0] += 2.5
     xb = cnn.bf16_bits(x)
     net = P.Cnn(max_batch=n, seed=5)
     dx, dl, dw = _dev(xb), _dev(lab), _dev(np.ones(n, np.float32))

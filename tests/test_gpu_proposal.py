@@ -2,8 +2,9 @@
 
   * coarse plane: bit-exact against the oracle's own stages run on the
     16-sample sums of the channel data (dedisperse with the coarse delay
-    table, remove_baseline, get_rms, divide) and rounded to bfloat16;
-  * tiles: numpy windows of that plane with zero padding;
+    table, remove_baseline, get_rms, divide) and rounded to bfloat16, and
+    its wide plane (float32 sums of 4 normalised samples, halved);
+  * tiles: numpy windows of both planes with zero padding;
   * scores: the oracle network on the extracted tiles (logit tolerance as in
     test_gpu_cnn);
   * gated search: with every unit proposed it equals the full search bit for
@@ -62,7 +63,13 @@ def oracle_coarse(O, w, fb, nbits, dms):
         z = O.remove_baseline(r[:nout].copy(), ns, nout, smooth, 1)
         rms = O.get_rms_multibeam(z, ns, nout, 1)
         rows.append((z / rms[0]).astype(np.float32))
-    return cnn.bf16_bits(np.array(rows)), nout
+    z = np.array(rows)
+    nw = nout // 4
+    s = np.zeros((z.shape[0], nw), np.float32)
+    for i in range(4):                      # float32, in index order
+        s = (s + z[:, i:4 * nw:4]).astype(np.float32)
+    wide = (s * np.float32(0.5)).astype(np.float32)
+    return cnn.bf16_bits(z), nout, cnn.bf16_bits(wide)
 
 
 def _device(a):
@@ -85,26 +92,28 @@ def test_coarse_plane_matches_oracle(H, oracle, nbits):
     d = _device(fb)
     nu, nv = P.coarse(pl, d.address, w.nsamps, nbits)
     got = P.plane(pl)
-    want, nout = oracle_coarse(oracle, w, fb, nbits, pl.dm_list())
+    want, nout, wide = oracle_coarse(oracle, w, fb, nbits, pl.dm_list())
     print("coarse plane", got.shape, "units", nu, "tpos", nv)
     assert got.shape == want.shape
     np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(P.plane(pl, 1), wide)
     assert nu == (len(pl.dm_list()) + 127) // 128
     assert nv == (nout + 31) // 32
     pl.close()
 
 
-def _window(plane, u, v):
-    t = np.zeros((64, 64), np.uint16)
-    R, Cc = plane.shape
-    for y in range(64):
-        r = 16 * u - 24 + y
-        if not 0 <= r < R:
-            continue
-        c0 = 32 * v - 16
-        lo, hi = max(c0, 0), min(c0 + 64, Cc)
-        if lo < hi:
-            t[y, lo - c0:hi - c0] = plane[r, lo:hi]
+def _window(plane, wide, u, v):
+    t = np.zeros((64, 64, 2), np.uint16)
+    R = plane.shape[0]
+    for ch, src, c0 in ((0, plane, 32 * v - 16), (1, wide, 8 * v - 28)):
+        Cc = src.shape[1]
+        for y in range(64):
+            r = 16 * u - 24 + y
+            if not 0 <= r < R:
+                continue
+            lo, hi = max(c0, 0), min(c0 + 64, Cc)
+            if lo < hi:
+                t[y, lo - c0:hi - c0, ch] = src[r, lo:hi]
     return t
 
 
@@ -118,27 +127,27 @@ def test_tiles_and_scores(H):
     pl.set_output(False)
     d = _device(fb)
     nu, nv = P.coarse(pl, d.address, w.nsamps, 8)
-    plane = P.plane(pl)
+    plane, wide = P.plane(pl), P.plane(pl, 1)
     rng = np.random.default_rng(5)
     pos = [(0, 0), (nu - 1, nv - 1), (0, nv - 1), (nu - 1, 0)]
     pos += [(int(rng.integers(nu)), int(rng.integers(nv))) for _ in range(28)]
-    tiles = DeviceBuffer(len(pos) * 64 * 64 * 2)
+    tiles = DeviceBuffer(len(pos) * 64 * 64 * 2 * 2)
     P.extract(pl, pos, tiles.address)
-    got = tiles.download().view(np.uint16).reshape(len(pos), 64, 64)
+    got = tiles.download().view(np.uint16).reshape(len(pos), 64, 64, 2)
     for k, (u, v) in enumerate(pos):
-        np.testing.assert_array_equal(got[k], _window(plane, u, v), err_msg=str((u, v)))
+        np.testing.assert_array_equal(got[k], _window(plane, wide, u, v), err_msg=str((u, v)))
     # scores of every tile with a random network against the oracle
     prng = np.random.default_rng(6)
     p = np.concatenate([prng.normal(0, np.sqrt(2.0 / fi), n) if not bias else np.zeros(n)
                         for (name, shp) in cnn.SHAPES
                         for n, fi, bias in [(int(np.prod(shp)),
-                                             {"w1": 9, "fc1_w": 512, "fc2_w": 64}.get(name, 288),
+                                             {"w1": 18, "fc1_w": 512, "fc2_w": 64}.get(name, 288),
                                              name.startswith("b") or name.endswith("_b"))]])
     p = p.astype(np.float32)
     P.set_proposal(pl, p, 0.0)
     sc = P.scores(pl, nu, nv)
     allpos = [(u, v) for u in range(nu) for v in range(nv)]
-    x = cnn.from_bits(np.stack([_window(plane, u, v) for u, v in allpos]))
+    x = cnn.from_bits(np.stack([_window(plane, wide, u, v) for u, v in allpos]))
     zo, _ = cnn.forward(p, x, quantize=True)
     err = np.abs(sc.reshape(-1) - zo)
     print("scores", sc.shape, "max err", err.max())
@@ -158,7 +167,7 @@ def _random_params(seed):
     out = []
     for name, shp in cnn.SHAPES:
         n = int(np.prod(shp))
-        fi = {"w1": 9, "fc1_w": 512, "fc2_w": 64}.get(name, 288)
+        fi = {"w1": 18, "fc1_w": 512, "fc2_w": 64}.get(name, 288)
         out.append(np.zeros(n) if (name.startswith("b") or name.endswith("_b"))
                    else rng.normal(0, np.sqrt(2.0 / fi), n))
     return np.concatenate(out).astype(np.float32)
