@@ -179,15 +179,21 @@ def batch_main(a):
 
 
 def proposal_weights():
-    """The trained proposal network shipped with the package (params, threshold)."""
+    """The trained proposal network shipped with the package: params and the
+    operating points [(held-out recall target, logit threshold)], the
+    default (highest recall) first."""
     path = os.path.join(HERE, "heimdall_multibeam_amd", "data", "proposal.npz")
     if not os.path.exists(path):
         return None, None
     z = np.load(path, allow_pickle=False)
-    return z["params"].astype(np.float32), float(z["threshold"])
+    ops = [(None, float(z["threshold"]))]
+    if "thresholds" in z.files:
+        ops = sorted(zip([float(r) for r in z["recall_targets"]],
+                         [float(t) for t in z["thresholds"]]), reverse=True)
+    return z["params"].astype(np.float32), ops
 
 
-def gated_eval(a, nbeams=64, per_call=16, steps=2, warmup=1):
+def gated_eval(a, nbeams=64, per_call=16, steps=1, warmup=1):
     """Batched full search vs proposal-gated search on the gated_4096ch beams
     (64 beams of 4096 channels x 2-bit, 131072 samples, 0-5 pulses each),
     rendered on the GPU, inputs resident in HBM.  One step searches every
@@ -196,7 +202,7 @@ def gated_eval(a, nbeams=64, per_call=16, steps=2, warmup=1):
     same peak (beam, sample, trial, width, S/N bits)."""
     import heimdall_multibeam_amd as H
     from heimdall_multibeam_amd import hip, proposal as P, synth
-    params, thr = proposal_weights()
+    params, ops = proposal_weights()
     if params is None:
         return {"error": "no trained network (heimdall_multibeam_amd/data/proposal.npz)"}
     w = synth.CONFIGS["gated_4096ch"]
@@ -219,8 +225,8 @@ def gated_eval(a, nbeams=64, per_call=16, steps=2, warmup=1):
               for b in range(c, min(c + per_call, nbeams))] for c in range(0, nbeams, per_call)]
     beam_s = nbeams * (w.nsamps - w.boxcar_max - max_delay) * w.dt
 
-    def run(gated):
-        if gated:
+    def run(thr):
+        if thr is not None:
             P.set_proposal(pl, params, thr)
         else:
             P.set_proposal(pl, None, 0.0)
@@ -229,24 +235,24 @@ def gated_eval(a, nbeams=64, per_call=16, steps=2, warmup=1):
             pl.execute_batch(c, w.nbits, device=True)
             cd = pl.candidates()
             gi = pl.candidate_gulps()
-            fl = P.last_proposal(pl) if gated else None
+            fl = P.last_proposal(pl) if thr is not None else None
             res.append((cd, gi, fl, pl.timings()))
         return res
 
-    out = {}
-    for gated in (False, True):
+    def timed(thr):
         for _ in range(warmup):
-            run(gated)
+            run(thr)
         hip.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            res = run(gated)
+            res = run(thr)
         hip.synchronize()
         dt = (time.perf_counter() - t0) / steps
-        out["gated" if gated else "full"] = (dt, res)
-        print("%s search: %.1f ms per %d beams" % ("gated" if gated else "full", dt * 1e3,
-                                                  nbeams), file=sys.stderr, flush=True)
-    P.set_proposal(pl, None, 0.0)
+        print("%s search: %.1f ms per %d beams" % ("full" if thr is None else "gated (%.3f)" % thr,
+                                                  dt * 1e3, nbeams), file=sys.stderr, flush=True)
+        return dt, res
+
+    full = timed(None)
 
     def cands(res, base_calls):
         keys = set()
@@ -260,27 +266,37 @@ def gated_eval(a, nbeams=64, per_call=16, steps=2, warmup=1):
                 allc.append((key, float(cd["peak"][k])))
         return keys, allc
 
-    fk, fall = cands(out["full"][1], calls)
-    gk, _ = cands(out["gated"][1], calls)
+    fk, fall = cands(full[1], calls)
     strong = [k for k, pk in fall if pk >= 8.0]
-    rec = float(np.mean([k in gk for k in strong])) if strong else 1.0
-    flags = np.concatenate([r[2] for r in out["gated"][1]])
-    unit_hits = [bool(flags[k[0], k[2] // P.UNIT]) for k in strong]
-    t_full, t_gated = out["full"][0], out["gated"][0]
-    stage = {}
-    for k in H.Pipeline.STAGES:
-        stage[k] = round(sum(r[3].get(k, 0.0) for r in out["gated"][1]), 3)
+    t_full = full[0]
     res = {"workload": "gated_4096ch: %d beams, %d ch x %d-bit, tsamp %g us, gulp %d, DM %g-%g "
                        "(%d trials), boxcars 1..%d, min_tscrunch_width %d"
                        % (nbeams, w.nchans, w.nbits, w.dt * 1e6, w.nsamps, w.dm_min, w.dm_max,
                           len(dms), w.boxcar_max, w.min_tscrunch_width),
-           "full_beam_s_per_s": round(beam_s / t_full, 3),
-           "gated_beam_s_per_s": round(beam_s / t_gated, 3),
-           "speedup": round(t_full / t_gated, 3),
-           "recall_snr8": round(rec, 4), "candidates_snr8": len(strong),
-           "unit_recall_snr8": round(float(np.mean(unit_hits)) if strong else 1.0, 4),
-           "units_searched_fraction": round(float(flags.mean()), 4),
-           "threshold": thr, "gated_stage_ms": stage}
+           "full_beam_s_per_s": round(beam_s / t_full, 3), "candidates_snr8": len(strong),
+           "full_stage_ms": {k: round(sum(r[3].get(k, 0.0) for r in full[1]), 3)
+                             for k in H.Pipeline.STAGES},
+           "operating_points": []}
+    for target, thr in ops:
+        t_gated, gres = timed(thr)
+        gk, _ = cands(gres, calls)
+        rec = float(np.mean([k in gk for k in strong])) if strong else 1.0
+        flags = np.concatenate([r[2] for r in gres])
+        unit_hits = [bool(flags[k[0], k[2] // P.UNIT]) for k in strong]
+        res["operating_points"].append({
+            "heldout_recall_target": target, "threshold": thr,
+            "gated_beam_s_per_s": round(beam_s / t_gated, 3),
+            "speedup": round(t_full / t_gated, 3),
+            "recall_snr8": round(rec, 4),
+            "unit_recall_snr8": round(float(np.mean(unit_hits)) if strong else 1.0, 4),
+            "units_searched_fraction": round(float(flags.mean()), 4),
+            "gated_stage_ms": {k: round(sum(r[3].get(k, 0.0) for r in gres), 3)
+                               for k in H.Pipeline.STAGES}})
+    P.set_proposal(pl, None, 0.0)
+    op0 = res["operating_points"][0]
+    res.update({"gated_beam_s_per_s": op0["gated_beam_s_per_s"], "speedup": op0["speedup"],
+                "recall_snr8": op0["recall_snr8"],
+                "units_searched_fraction": op0["units_searched_fraction"]})
     pl.close()
     buf.free()
     return res

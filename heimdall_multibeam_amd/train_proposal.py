@@ -264,6 +264,9 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--target-recall", type=float, default=0.995)
     ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--evaluate", default=None,
+                    help="skip training: evaluate the network in this .npz on the held-out "
+                         "gulps and write <out> with its thresholds")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     import torch
@@ -298,10 +301,16 @@ def main(argv=None):
                     lab.append(0.0)
             return pos, lab
 
+        if a.evaluate:
+            stats[name] = {"heldout_gulps": n_val, "n_dm": int(len(st.dms))}
+            continue
         t, l, stt = collect(st, seeds[:n_tr], kinds[:n_tr], pick, dev, "train")
         tiles_all += t
         labels_all += l
         stats[name] = dict(stt, heldout_gulps=n_val, n_dm=int(len(st.dms)))
+    if a.evaluate:
+        z = np.load(a.evaluate, allow_pickle=False)
+        return finish(a, z["params"], setups, heldout, stats, None)
     X = torch.cat(tiles_all)
     Y = torch.from_numpy(np.concatenate(labels_all)).to(dev)
     tiles_all = []
@@ -332,18 +341,29 @@ def main(argv=None):
         pos_frac = train(net, X, Y, a.epochs2, a.lr * 0.3, a.batch, a.seed + 1)
     params = net.params()
     del X, Y
-    # held-out evaluation and thresholds
-    metrics = {"tiles": None, "positive_fraction": pos_frac, "epochs": [a.epochs, a.epochs2],
+    return finish(a, params, setups, heldout, stats, pos_frac)
+
+
+RECALL_TARGETS = (0.95, 0.98, 0.99, 0.995)
+
+
+def finish(a, params, setups, heldout, stats, pos_frac):
+    """Held-out evaluation, thresholds per recall target, and the outputs."""
+    metrics = {"positive_fraction": pos_frac, "epochs": [a.epochs, a.epochs2],
                "target_recall": a.target_recall, "setups": stats, "heldout": {}}
     thr_deploy = None
+    targets = None
     for name, (seeds, kinds) in heldout.items():
         ev = evaluate(setups[name], seeds, kinds, params)
         thr_n, rec_n, frac_n = threshold_for(ev, "net", a.target_recall)
         thr_h, rec_h, frac_h = threshold_for(ev, "heur", a.target_recall)
         curve = {}
-        for r in (0.95, 0.98, 0.99, 0.995):
-            curve[str(r)] = {"network": threshold_for(ev, "net", r)[2],
+        thr_r = []
+        for r in RECALL_TARGETS:
+            tn = threshold_for(ev, "net", r)
+            curve[str(r)] = {"network": tn[2], "network_threshold": tn[0],
                              "coarse_boxcar": threshold_for(ev, "heur", r)[2]}
+            thr_r.append(tn[0])
         metrics["heldout"][name] = {
             "gulps": len(seeds), "candidates_snr8": int(sum(len(e["units"]) for e in ev)),
             "ideal_unit_fraction": ideal_fraction(ev),
@@ -353,9 +373,11 @@ def main(argv=None):
         log(name, json.dumps(metrics["heldout"][name]))
         if name == "gated_4096ch" or thr_deploy is None:
             thr_deploy = thr_n
+            targets = np.array(thr_r, np.float32)
     metrics["threshold"] = thr_deploy
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
-    np.savez(a.out + ".npz", params=params, threshold=np.float32(thr_deploy))
+    np.savez(a.out + ".npz", params=params, threshold=np.float32(thr_deploy),
+             recall_targets=np.array(RECALL_TARGETS, np.float32), thresholds=targets)
     with open(a.out + ".json", "w") as f:
         json.dump(metrics, f, indent=1)
     log("wrote", a.out + ".npz", "threshold %.4f" % thr_deploy)
