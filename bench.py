@@ -149,6 +149,8 @@ def batch_main(a):
                    "giants": n_giants,
                    "parallelism": "1 GPU"},
         "stage_ms": {k: round(float(np.mean(v)), 4) for k, v in stage.items() if v},
+        "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
+                        "chosen_by": "autotuner (first call of the shape)"},
         "roofline": {"bound": "hbm", "achieved": round(alg / t_dd / 1e9, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / t_dd / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
@@ -392,6 +394,8 @@ def main():
                    "giants": n_giants,
                    "parallelism": "1 GPU"},
         "stage_ms": stages_ms,
+        "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
+                        "chosen_by": "autotuner (first call of the shape)"},
         "roofline": roof,
         "cpu_baseline": None,
     }

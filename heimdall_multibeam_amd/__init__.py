@@ -120,6 +120,10 @@ def lib():
     L.hd_get_last_candidate_gulps.restype = sz
     L.hd_set_cleaning.argtypes = [vp, C.c_int]
     L.hd_set_cleaning.restype = None
+    L.hd_get_dedisp_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.hd_get_dedisp_tile.restype = None
+    L.hd_set_dedisp_tile.argtypes = [vp, C.c_int, C.c_int]
+    L.hd_set_dedisp_tile.restype = C.c_int
     L.hd_clean_filterbank_rfi.argtypes = [sz, C.c_float, C.c_float, vp, sz, sz, vp,
                                           C.POINTER(C.c_int), C.c_float, C.c_float, C.c_float,
                                           C.c_float, sz, sz, sz, vp]
@@ -196,6 +200,18 @@ class Pipeline:
 
     def set_output(self, enabled):
         lib().hd_set_output_enabled(self.handle, 1 if enabled else 0)
+
+    def dedisp_tile(self):
+        """(trials per block, XCD-aware order) of the last call's dedispersion."""
+        db, rm = C.c_int(0), C.c_int(0)
+        lib().hd_get_dedisp_tile(self.handle, C.byref(db), C.byref(rm))
+        return db.value, rm.value
+
+    def set_dedisp_tile(self, trials_per_block, xcd_remap=1):
+        """Fix the dedispersion tile (8, 16 or 32 trials per block), or 0 for the autotuner."""
+        err = lib().hd_set_dedisp_tile(self.handle, trials_per_block, xcd_remap)
+        if err != HD_NO_ERROR:
+            raise HdError(err, "hd_set_dedisp_tile")
 
     def set_cleaning(self, enabled):
         """Zero-DM RFI cleaning before each search (the reference's CLEANING build)."""

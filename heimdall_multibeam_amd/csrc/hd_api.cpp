@@ -17,6 +17,8 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <map>
+#include <tuple>
 #include <netdb.h>
 #include <sstream>
 #include <string>
@@ -100,7 +102,7 @@ struct hd_pipeline_t {
   std::vector<float> kc;
   size_t max_delay = 0;
   // device buffers
-  DevBuf fb, x, live, delays, chunk_pairs, plane, pts, rms, brms, S, partial, wrms, winv;
+  DevBuf fb, x, live, delays, plane, pts, rms, brms, S, partial, wrms, winv;
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
   DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm, g_best;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
@@ -134,7 +136,15 @@ struct hd_pipeline_t {
   // delay table state for the current live-channel set
   std::vector<int> live_list;
   std::vector<int> killmask_cached;
-  size_t wlen = 0;  // staged dwords per copy of the widest DM block
+  size_t wlen = 0;  // staged dwords per copy of the widest DM block (any tile height)
+  // dedispersion tile heights the autotuner chooses from, with their staging
+  // windows; the choice per launch shape (rows, groups, live channels, pitch)
+  DevBuf pairs_db[3];
+  int wlen_db[3] = {0, 0, 0};
+  std::map<std::tuple<int, int, int, size_t>, std::pair<int, int>> tune;
+  hipEvent_t tev[2] = {};
+  int last_db = 16, last_remap = 1;
+  int fixed_db = 0, fixed_remap = 1;    // hd_set_dedisp_tile (0: autotune)
   // results of the last call
   size_t n_giants = 0;
   std::vector<float> r_peak, r_dms;
@@ -402,8 +412,8 @@ struct DeviceGuard {  // binds the pipeline's device, restores the caller's
 // the first trial's delay rounded down to a dword (and per chunk of channels
 // staged together).  d: n_rows x nl delays; returns the pairs and the widest
 // window.
-std::vector<int> staging_pairs(const std::vector<int>& d, size_t n_rows, size_t nl, int* hw_out) {
-  const int DB = hd::dedisp_tile_dms();
+std::vector<int> staging_pairs(const std::vector<int>& d, size_t n_rows, size_t nl, int DB,
+                               int* hw_out) {
   const int CC = hd::dedisp_chunk_channels();
   const size_t nblk = (n_rows + DB - 1) / DB;
   const size_t nchunk = (nl + CC - 1) / CC;
@@ -444,16 +454,21 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   for (size_t m = 0; m < n_rows; ++m)
     for (size_t l = 0; l < nl; ++l)
       d[m * nl + l] = hdplan::delay(pl->dm_list[std::min(m, n_dm - 1)], pl->kc[pl->live_list[l]]);
-  int hw_max = 2;
-  const std::vector<int> pairs = staging_pairs(d, n_rows, nl, &hw_max);
-  pl->wlen = (size_t)hw_max;
   HD_CHECK(pl->live.ensure(std::max<size_t>(nl, 1) * sizeof(int)));
   HD_CHECK(pl->delays.ensure(d.size() * sizeof(int)));
-  HD_CHECK(pl->chunk_pairs.ensure(pairs.size() * sizeof(int)));
   HD_CHECK(hipMemcpy(pl->live.p, pl->live_list.data(), nl * sizeof(int), hipMemcpyHostToDevice));
   HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
-  HD_CHECK(hipMemcpy(pl->chunk_pairs.p, pairs.data(), pairs.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
+  pl->wlen = 0;
+  for (int k = 0; k < 3; ++k) {
+    int hw = 2;
+    const std::vector<int> pairs = staging_pairs(d, n_rows, nl, 8 << k, &hw);
+    pl->wlen_db[k] = hw;
+    pl->wlen = std::max(pl->wlen, (size_t)hw);
+    HD_CHECK(pl->pairs_db[k].ensure(pairs.size() * sizeof(int)));
+    HD_CHECK(hipMemcpy(pl->pairs_db[k].p, pairs.data(), pairs.size() * sizeof(int),
+                       hipMemcpyHostToDevice));
+  }
+  pl->tune.clear();
   return HD_NO_ERROR;
 }
 
@@ -480,7 +495,7 @@ hd_error build_coarse(hd_pipeline pl) {
     for (size_t l = 0; l < nl; ++l)
       d[m * nl + l] = hdplan::delay(pl->co_dms[m], pl->co_kc[pl->live_list[l]]);
   int hw_max = 2;
-  const std::vector<int> pairs = staging_pairs(d, nc, nl, &hw_max);
+  const std::vector<int> pairs = staging_pairs(d, nc, nl, hd::dedisp_tile_dms(), &hw_max);
   pl->co_wlen = (size_t)hw_max;
   HD_CHECK(pl->co_delays.ensure(d.size() * sizeof(int)));
   HD_CHECK(pl->co_pairs.ensure(pairs.size() * sizeof(int)));
@@ -664,6 +679,75 @@ struct GulpIn {
   const hd_byte* d_fb;
   size_t nsamps, nbeams, first_idx, beam;
 };
+
+// Dedispersion autotuner.  Every tile height (8, 16 or 32 trials per block)
+// and block order (XCD-aware or linear) computes the same exact sums, so the
+// choice only moves time.  The first launch of a shape (rows per group,
+// groups, live channels, row pitch) times each candidate on the real input
+// with HIP events (one warm-up launch and one timed launch each) and keeps
+// the fastest; HD_DEDISP_TILE="db,remap" fixes the choice instead.
+hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
+                            hipStream_t s) {
+  static const std::pair<int, int> forced = [] {
+    const char* e = getenv("HD_DEDISP_TILE");
+    int db = 0, rm = 1;
+    if (e && std::sscanf(e, "%d,%d", &db, &rm) >= 1 && (db == 8 || db == 16 || db == 32))
+      return std::make_pair(db, rm ? 1 : 0);
+    return std::make_pair(0, 1);
+  }();
+  if (pl->fixed_db) {
+    dg.db = pl->fixed_db;
+    dg.remap = pl->fixed_remap;
+  } else if (forced.first) {
+    dg.db = forced.first;
+    dg.remap = forced.second;
+  } else {
+    const auto key = std::make_tuple(dg.n_dm, gr.G, dg.n_live, dg.ppitch);
+    auto it = pl->tune.find(key);
+    if (it == pl->tune.end()) {
+      if (!pl->tev[0]) {
+        HD_CHECK(hipEventCreate(&pl->tev[0]));
+        HD_CHECK(hipEventCreate(&pl->tev[1]));
+      }
+      float best = 1e30f;
+      std::pair<int, int> choice{16, 1};
+      for (int k = 0; k < 3; ++k)
+        for (int rm = 0; rm < 2; ++rm) {
+          hd::DedispGeom t = dg;
+          t.db = 8 << k;
+          t.remap = rm;
+          t.wlen = pl->wlen_db[k];
+          if ((size_t)t.wlen * 2 * hd::dedisp_chunk_channels() * 4 > 160 * 1024) continue;
+          auto launch = [&]() {
+            return hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
+                                         pl->pairs_db[k].as<int>(), t, pl->plane.as<float>(), s,
+                                         gr);
+          };
+          if (launch() != hipSuccess) continue;
+          HD_CHECK(hipEventRecord(pl->tev[0], s));
+          HD_CHECK(launch());
+          HD_CHECK(hipEventRecord(pl->tev[1], s));
+          HD_CHECK(hipEventSynchronize(pl->tev[1]));
+          float ms = 0.f;
+          HD_CHECK(hipEventElapsedTime(&ms, pl->tev[0], pl->tev[1]));
+          if (ms < best) {
+            best = ms;
+            choice = {t.db, rm};
+          }
+        }
+      it = pl->tune.emplace(key, choice).first;
+      if (pl->params.verbosity >= 2)
+        std::cout << "dedispersion tile: " << choice.first << " trials per block, "
+                  << (choice.second ? "XCD-aware" : "linear") << " order (" << best << " ms)"
+                  << std::endl;
+    }
+    dg.db = it->second.first;
+    dg.remap = it->second.second;
+  }
+  pl->last_db = dg.db;
+  pl->last_remap = dg.remap;
+  return HD_NO_ERROR;
+}
 
 // The coarse plane of the G gulps of pl->hgd, whose unpacked rows are slots
 // 0..G-1 of pl->x (pitch xpitch): decimation, coarse dedispersion, the
@@ -987,8 +1071,14 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   // dedisperse
   HD_CHECK(pl->plane.ensure(rows * ppitch * sizeof(float)));
   hd::DedispGeom dg{R, n_live, sh0.nout, xpitch, ppitch, (int)pl->wlen};
-  HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
-                                 pl->chunk_pairs.as<int>(), dg, pl->plane.as<float>(), s, gr));
+  {
+    hd_error te = choose_dedisp_tile(pl, dg, gr, s);
+    if (te != HD_NO_ERROR) return te;
+    const int k = dg.db == 8 ? 0 : (dg.db == 16 ? 1 : 2);
+    dg.wlen = pl->wlen_db[k];
+    HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
+                                   pl->pairs_db[k].as<int>(), dg, pl->plane.as<float>(), s, gr));
+  }
   record(pl, 2);
   // baseline points and per-beam RMS (normalisation is fused into the scan)
   HD_CHECK(pl->pts.ensure((rows * gr.prow + 64) * sizeof(float)));
@@ -1456,7 +1546,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
   {
     DeviceGuard dg(pl->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    DevBuf* bufs[] = {&pl->clean_out, &pl->clean_work, &pl->fb, &pl->x, &pl->live, &pl->delays, &pl->chunk_pairs, &pl->plane, &pl->pts, &pl->rms, &pl->brms,
+    DevBuf* bufs[] = {&pl->clean_out, &pl->clean_work, &pl->fb, &pl->x, &pl->live, &pl->delays, &pl->plane, &pl->pts, &pl->rms, &pl->brms,
                       &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
                       &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->okey, &pl->oval,
                       &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,
@@ -1470,6 +1560,9 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->co_gdev, &pl->co_tiles, &pl->co_scores, &pl->co_toff, &pl->co_ntpos,
                       &pl->co_ncols, &pl->co_flags, &pl->co_pos};
     for (DevBuf* b : bufs) b->release();
+    for (DevBuf& b : pl->pairs_db) b.release();
+    for (hipEvent_t& e : pl->tev)
+      if (e) (void)hipEventDestroy(e);
     if (pl->net) hd_cnn_destroy(pl->net);
     for (hipEvent_t& ev : pl->ev)
       if (ev) (void)hipEventDestroy(ev);
@@ -1663,4 +1756,20 @@ extern "C" hd_size hd_get_last_proposal(hd_pipeline pl, unsigned char* flags, hd
   const size_t n = pl->prop_flags.size();
   if (flags) std::memcpy(flags, pl->prop_flags.data(), std::min<size_t>(n, cap));
   return n;
+}
+
+extern "C" void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap) {
+  if (!pl) return;
+  if (trials_per_block) *trials_per_block = pl->last_db;
+  if (xcd_remap) *xcd_remap = pl->last_remap;
+}
+
+extern "C" hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap) {
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (trials_per_block != 0 && trials_per_block != 8 && trials_per_block != 16 &&
+      trials_per_block != 32)
+    return HD_INVALID_STRIDE;
+  pl->fixed_db = trials_per_block;
+  pl->fixed_remap = xcd_remap ? 1 : 0;
+  return HD_NO_ERROR;
 }

@@ -140,12 +140,14 @@ struct DedispGeom {
   size_t xpitch;       // bytes per unpacked channel row
   size_t ppitch;       // floats per output row
   int wlen;            // LDS dwords per staged copy (the largest block's need)
+  int db = 16;         // trials per block (8, 16 or 32; the staging pairs are planned for it)
+  int remap = 1;       // XCD-aware block order
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,
                              const GroupArgs& ga = GroupArgs{});
 int dedisp_tile_time();     // outputs per block in time
-int dedisp_tile_dms();      // DMs per block
+int dedisp_tile_dms();      // default DMs per block
 int dedisp_window_dwords(int spread);  // staged dwords per copy for a delay span
 int dedisp_chunk_channels();           // channels staged per step
 

@@ -315,15 +315,8 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
 }  // namespace
 
 int dedisp_tile_time() { return D2_TB; }
-// Trials per block: 16 (default) or 32 (HD_DEDISP_TRIALS=32); the host plans
-// its per-block staging windows with the same value.
-int dedisp_tile_dms() {
-  static const int db = [] {
-    const char* e = getenv("HD_DEDISP_TRIALS");
-    return (e && atoi(e) == 32) ? 32 : D2_DB;
-  }();
-  return db;
-}
+// Default trials per block (the pipeline's autotuner may choose 8 or 32).
+int dedisp_tile_dms() { return D2_DB; }
 int dedisp_chunk_channels() { return D2_CC; }
 int dedisp_window_dwords(int spread) {
   // E/O copies: dword (o>>1) + 511 with o <= spread + 3, rounded to 64
@@ -351,28 +344,28 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
                              const DedispGeom& g, float* plane, hipStream_t s,
                              const GroupArgs& ga) {
   // DM blocks fastest: blocks resident together share one time tile's input
-  const int db = dedisp_tile_dms();
+  const int db = g.db;
+  if (db != 8 && db != 16 && db != 32) return hipErrorInvalidValue;
   dim3 grid((unsigned)((g.n_dm + db - 1) / db), (unsigned)(g.ppitch / D2_TB),
             (unsigned)ga.G);
   const size_t lds = (size_t)D2_CC * 2 * (size_t)g.wlen * sizeof(uint32_t);
   if (lds > 160 * 1024) return hipErrorInvalidValue;  // delay span of a DM block too wide
+  const void* fn = db == 32 ? reinterpret_cast<const void*>(&k_dedisperse16<32>)
+                 : db == 8  ? reinterpret_cast<const void*>(&k_dedisperse16<8>)
+                            : reinterpret_cast<const void*>(&k_dedisperse16<16>);
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(
-        db == 32 ? reinterpret_cast<const void*>(&k_dedisperse16<32>)
-                 : reinterpret_cast<const void*>(&k_dedisperse16<D2_DB>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  static const int remap = [] {
-    const char* e = getenv("HD_DEDISP_XCD_REMAP");
-    return e ? atoi(e) : 1;
-  }();
   if (db == 32)
     hipLaunchKernelGGL(k_dedisperse16<32>, grid, dim3(D2_BS), lds, s, x, d_delays,
-                       d_chunk_pairs, g, g.wlen, remap, plane, ga.gd);
+                       d_chunk_pairs, g, g.wlen, g.remap, plane, ga.gd);
+  else if (db == 8)
+    hipLaunchKernelGGL(k_dedisperse16<8>, grid, dim3(D2_BS), lds, s, x, d_delays,
+                       d_chunk_pairs, g, g.wlen, g.remap, plane, ga.gd);
   else
-    hipLaunchKernelGGL(k_dedisperse16<D2_DB>, grid, dim3(D2_BS), lds, s, x, d_delays,
-                       d_chunk_pairs, g, g.wlen, remap, plane, ga.gd);
+    hipLaunchKernelGGL(k_dedisperse16<16>, grid, dim3(D2_BS), lds, s, x, d_delays,
+                       d_chunk_pairs, g, g.wlen, g.remap, plane, ga.gd);
   return hipGetLastError();
 }
 

@@ -120,3 +120,26 @@ def test_first_idx_and_beam_offset(oracle):
     gpu, r = _run_both(oracle, TINY, 4, first_idx=123456, gpu_kw=dict(beam=2),
                        orc_kw=dict(beam=2))
     _compare(gpu, r, oracle, TINY)
+
+
+@pytest.mark.parametrize("db", [8, 16, 32])
+def test_every_dedisp_tile_gives_the_same_search(oracle, db):
+    """The autotuner's candidates (tile height, block order) all compute the
+    same exact sums: fixed to each, the search matches the oracle bit for bit."""
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import synth
+    w = SMALL_128
+    fb = synth.make_gulp(w, 31)
+    p = oracle_params_for(oracle, w)
+    r = oracle.execute(p, fb, w.nsamps, w.nbits, 0, 1)
+    for remap in (0, 1):
+        print("case tile", db, "remap", remap, flush=True)
+        pl = H.Pipeline(params_for(w))
+        pl.set_output(False)
+        pl.set_dedisp_tile(db, remap)
+        err, processed = pl.execute(fb, w.nsamps, w.nbits)
+        assert pl.dedisp_tile() == (db, remap)
+        gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
+                   text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
+        pl.close()
+        _compare(gpu, r, oracle, w)
