@@ -684,8 +684,10 @@ struct GulpIn {
 // and block order (XCD-aware or linear) computes the same exact sums, so the
 // choice only moves time.  The first launch of a shape (rows per group,
 // groups, live channels, row pitch) times each candidate on the real input
-// with HIP events (one warm-up launch and one timed launch each) and keeps
-// the fastest; HD_DEDISP_TILE="db,remap" fixes the choice instead.
+// with HIP events (one warm-up launch and one timed launch each, over a
+// slice of the time tiles, into the plane buffer that the real launch then
+// overwrites) and keeps the fastest; HD_DEDISP_TILE="db,remap" fixes the
+// choice instead.
 hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
                             hipStream_t s) {
   static const std::pair<int, int> forced = [] {
@@ -713,10 +715,13 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
       std::pair<int, int> choice{16, 1};
       for (int k = 0; k < 3; ++k)
         for (int rm = 0; rm < 2; ++rm) {
+          // timed on the first (at most) 48 time tiles of every group: the
+          // tiles are alike, and large shapes would otherwise spend seconds here
           hd::DedispGeom t = dg;
           t.db = 8 << k;
           t.remap = rm;
           t.wlen = pl->wlen_db[k];
+          t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
           if ((size_t)t.wlen * 2 * hd::dedisp_chunk_channels() * 4 > 160 * 1024) continue;
           auto launch = [&]() {
             return hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
