@@ -245,13 +245,19 @@ constexpr int TH_CHUNK = FB_BS * TH_PER;  // 4096 prefix-sum positions per block
 // blockIdx: (chunk, trial, group); rows g*n_dm + m.  Chunk keys are
 // (row*nw + width)*nchunks_per_w + chunk, so the compacted samples come out
 // in (group, trial, width, time) order.
+//
+// Thread t evaluates outputs p = q*FB_BS + t (q < TH_PER) of the chunk, so the
+// lanes of a wave read consecutive LDS words (no bank conflicts; with ts == 1
+// the q*FB_BS offsets are immediates).  Hits are written in time order:
+// q-major, then wave, then lane (one ballot per q and wave, wave counts
+// prefixed in LDS).
 __global__ __launch_bounds__(FB_BS) void k_threshold(
     const float* __restrict__ Sall, size_t spitch, const GroupDev* __restrict__ gd,
     WidthTable wt, unsigned amax, unsigned bmax, const float* __restrict__ winv_all,
     float thresh, GiantBuffers gb) {
-  // T holds prefix sums with one pad word after every 16, so the 16-apart
-  // positions that a wave's lanes read fall in different LDS banks.
-  extern __shared__ float T[];  // (TH_CHUNK + amax + bmax) * 17/16
+  extern __shared__ float T[];  // TH_CHUNK + amax + bmax prefix sums
+  __shared__ unsigned wcnt[TH_PER][FB_BS / 64];
+  __shared__ unsigned sbase;
   const size_t c = blockIdx.x;
   const size_t row = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
   const size_t stride = gd[blockIdx.z].sh.stride;
@@ -262,8 +268,7 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const size_t p0 = c * TH_CHUNK;                 // first position of the chunk
   {
     // chunks past every width's output (shorter groups of a batch) only
-    // record empty counts
-    // (and padding rows of a gated group)
+    // record empty counts (and padding rows of a gated group)
     bool any = false;
     if (blockIdx.y < gd[blockIdx.z].rows_valid)
       for (int j = 0; j < wt.nw; ++j)
@@ -275,13 +280,16 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     }
   }
   const size_t tbase = p0 + wt.off - bmax;        // S index held in T[0]
-  const size_t tlen = TH_CHUNK + amax + bmax;
-  for (size_t t = threadIdx.x; t < tlen; t += FB_BS) {
+  const unsigned tlen = (unsigned)(TH_CHUNK + amax + bmax);
+  for (unsigned t = threadIdx.x; t < tlen; t += FB_BS) {
     const size_t si = tbase + t;
-    T[t + (t >> 4)] = si < s_len ? S[si] : 0.f;
+    T[t] = si < s_len ? S[si] : 0.f;
   }
   __syncthreads();
-  __shared__ unsigned sbase;
+  const unsigned tid = threadIdx.x;
+  const int lane = (int)(tid & 63), wave = (int)(tid >> 6);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;  // lanes under this one
+  const unsigned tmax = tlen - 1;
   for (int j = 0; j < wt.nw; ++j) {
     const WidthInfo wi = wt.wi[j];
     const float inv = winv_all[row * wt.nw + j];
@@ -292,50 +300,54 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     if (i1 > n_w) i1 = n_w;
     const size_t key = (row * wt.nw + j) * gb.nchunks_per_w + c;
     if (i0 >= i1) {
-      if (threadIdx.x == 0) gb.ccount[key] = 0;
+      if (tid == 0) gb.ccount[key] = 0;
       continue;
     }
-    const size_t nout = i1 - i0;
-    const unsigned per = (unsigned)((nout + FB_BS - 1) / FB_BS);  // <= TH_PER
-    const size_t ib = i0 + (size_t)threadIdx.x * per;
+    const unsigned nout = (unsigned)(i1 - i0);
     // find_giants.cu:55-58 keeps i with (i mod stride) < bc692.  When the
     // chunk's outputs all lie in one beam period that is a single bound.
     const size_t beam0 = (size_t)(unsigned)i0 / stride;
     const bool one_period = (size_t)(unsigned)(i1 - 1) / stride == beam0 &&
                             i1 - 1 <= 0xffffffffull;
     const size_t lim = beam0 * stride + bc692;
-    // T indices (32-bit) of position ib*ts: ahead and behind reads
-    const unsigned r0 = (unsigned)(ib * wi.ts - p0 + bmax);
-    const unsigned ra0 = r0 + wi.ahead, rb0 = r0 - wi.behind;
+    const unsigned rbase = (unsigned)(i0 * wi.ts - p0 + bmax);   // T index of output i0
     float vals[TH_PER];
     unsigned mask = 0;
-    // valid outputs of this thread: q < nq
-    const unsigned nq = ib >= i1 ? 0u : (unsigned)min((size_t)per, i1 - ib);
     if (one_period) {
-      // every valid output is in one beam period: keep i < lim
-      const unsigned nk = ib >= lim ? 0u : (unsigned)min((size_t)nq, lim - ib);
-      // outputs past nk are not kept; their reads are clamped into the tile
-      const unsigned tmax = (unsigned)tlen - 1;
+      // kept: outputs p < pk (i < min(i1, lim)); the others are evaluated
+      // from in-tile reads and dropped
+      const size_t kend = i1 < lim ? i1 : lim;
+      const unsigned pk = kend > i0 ? (unsigned)(kend - i0) : 0u;
+      if (wi.ts == 1) {
+        // p*ts + ahead + rbase <= TH_CHUNK - 1 + amax + bmax: always in the tile
+        const float* Ta = T + rbase + wi.ahead + tid;
+        const float* Tb = T + rbase - wi.behind + tid;
 #pragma unroll
-      for (int q = 0; q < TH_PER; ++q) {
-        const unsigned ra = min(ra0 + (unsigned)q * wi.ts, tmax);
-        const unsigned rb = min(rb0 + (unsigned)q * wi.ts, tmax);
-        const float a = T[ra + (ra >> 4)];
-        const float b = T[rb + (rb >> 4)];
-        const float v = (a - b) * inv;               // get_rms.cu:194-197
-        vals[q] = v;
-        if ((unsigned)q < nk && v > thresh) mask |= 1u << q;
+        for (int q = 0; q < TH_PER; ++q) {
+          const float v = (Ta[q * FB_BS] - Tb[q * FB_BS]) * inv;   // get_rms.cu:194-197
+          vals[q] = v;
+          if ((unsigned)q * FB_BS + tid < pk && v > thresh) mask |= 1u << q;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < TH_PER; ++q) {
+          const unsigned pp = (unsigned)q * FB_BS + tid;
+          const unsigned ra = min(rbase + wi.ahead + pp * wi.ts, tmax);
+          const unsigned rb = min(rbase - wi.behind + pp * wi.ts, tmax);
+          const float v = (T[ra] - T[rb]) * inv;
+          vals[q] = v;
+          if (pp < pk && v > thresh) mask |= 1u << q;
+        }
       }
     } else {
 #pragma unroll
       for (int q = 0; q < TH_PER; ++q) {
         vals[q] = 0.f;
-        const size_t i = ib + q;
-        if ((unsigned)q < nq) {
-          const unsigned ra = ra0 + (unsigned)q * wi.ts, rb = rb0 + (unsigned)q * wi.ts;
-          const float a = T[ra + (ra >> 4)];
-          const float b = T[rb + (rb >> 4)];
-          const float v = (a - b) * inv;
+        const unsigned pp = (unsigned)q * FB_BS + tid;
+        if (pp < nout) {
+          const size_t i = i0 + pp;
+          const unsigned ra = rbase + wi.ahead + pp * wi.ts, rb = rbase - wi.behind + pp * wi.ts;
+          const float v = (T[ra] - T[rb]) * inv;
           vals[q] = v;
           const bool in_beam = (size_t)(unsigned)i % stride < bc692;
           if (in_beam && v > thresh) mask |= 1u << q;
@@ -344,27 +356,41 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     }
     const int any = __syncthreads_or(mask != 0);
     if (!any) {
-      if (threadIdx.x == 0) gb.ccount[key] = 0;
+      if (tid == 0) gb.ccount[key] = 0;
       continue;
     }
-    unsigned total;
-    const unsigned rank = block_exclusive_scan<FB_BS>(__popc(mask), &total);
-    if (threadIdx.x == 0) {
-      sbase = atomicAdd(gb.counter, total);
-      gb.ccount[key] = total;
+    uint64_t bq[TH_PER];
+#pragma unroll
+    for (int q = 0; q < TH_PER; ++q) bq[q] = __ballot((mask >> q) & 1u);
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < TH_PER; ++q) wcnt[q][wave] = (unsigned)__popcll(bq[q]);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned run = 0;
+      for (int q = 0; q < TH_PER; ++q)
+        for (int w = 0; w < FB_BS / 64; ++w) {
+          const unsigned n = wcnt[q][w];
+          wcnt[q][w] = run;
+          run += n;
+        }
+      sbase = atomicAdd(gb.counter, run);
+      gb.ccount[key] = run;
       gb.coff[key] = sbase;
     }
     __syncthreads();
     const size_t base = sbase;
-    if (mask && base + total <= gb.scap) {  // overflow: detected by the host
-      unsigned rr = rank;
+    if (mask) {
       const uint64_t dmw = (uint64_t)row * wt.nw + j;
 #pragma unroll
       for (int q = 0; q < TH_PER; ++q) {
         if (mask & (1u << q)) {
-          gb.skey[base + rr] = (dmw << 32) | (uint64_t)(ib + q);
-          gb.sval[base + rr] = vals[q];
-          ++rr;
+          const size_t pos = base + wcnt[q][wave] + (unsigned)__popcll(bq[q] & below);
+          if (pos < gb.scap) {  // overflow: detected by the host from the counter
+            gb.skey[pos] = (dmw << 32) | (uint64_t)(i0 + (unsigned)q * FB_BS + tid);
+            gb.sval[pos] = vals[q];
+          }
         }
       }
     }
@@ -553,8 +579,8 @@ hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
     bmax = wt.wi[j].behind > bmax ? wt.wi[j].behind : bmax;
   }
   const size_t tlen = (size_t)TH_CHUNK + amax + bmax;
-  const size_t lds = (tlen + tlen / 16 + 1) * sizeof(float);
-  if (lds > 65536) return hipErrorInvalidValue;  // boxcar_max beyond ~11000
+  const size_t lds = tlen * sizeof(float);
+  if (lds > 65536) return hipErrorInvalidValue;  // boxcar_max beyond ~12000
   dim3 grid((unsigned)gb.nchunks_per_w, n_dm, ga.G);
   hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), lds, s, S, spitch, ga.gd, wt, amax, bmax,
                      winv, thresh, gb);
