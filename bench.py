@@ -152,6 +152,7 @@ def batch_main(a):
                    "parallelism": "1 GPU"},
         "stage_ms": {k: round(float(np.mean(v)), 4) for k, v in stage.items() if v},
         "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
+                        "byte_lanes": pl.dedisp_tile()[2],
                         "chosen_by": "autotuner (first call of the shape)"},
         "roofline": {"bound": "hbm", "achieved": round(alg / t_dd / 1e9, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -281,7 +282,8 @@ def gated_eval(a, nbeams=64, per_call=16, steps=1, warmup=1, max_ops=None):
                        % (nbeams, w.nchans, w.nbits, w.dt * 1e6, w.nsamps, w.dm_min, w.dm_max,
                           len(dms), w.boxcar_max, w.min_tscrunch_width),
            "full_beam_s_per_s": round(beam_s / t_full, 3), "candidates_snr8": len(strong),
-           "dedisp_tile": {"trials_per_block": full_tile[0], "xcd_remap": full_tile[1]},
+           "dedisp_tile": {"trials_per_block": full_tile[0], "xcd_remap": full_tile[1],
+                           "byte_lanes": full_tile[2]},
            "full_stage_ms": {k: round(sum(r[3].get(k, 0.0) for r in full[1]), 3)
                              for k in H.Pipeline.STAGES},
            "operating_points": []}
@@ -423,6 +425,7 @@ def main():
                    "parallelism": "1 GPU"},
         "stage_ms": stages_ms,
         "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
+                        "byte_lanes": pl.dedisp_tile()[2],
                         "chosen_by": "autotuner (first call of the shape)"},
         "roofline": roof,
         "cpu_baseline": None,

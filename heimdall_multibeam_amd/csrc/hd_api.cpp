@@ -141,10 +141,11 @@ struct hd_pipeline_t {
   // windows; the choice per launch shape (rows, groups, live channels, pitch)
   DevBuf pairs_db[3];
   int wlen_db[3] = {0, 0, 0};
-  std::map<std::tuple<int, int, int, size_t>, std::pair<int, int>> tune;
+  int wlen8_db[3] = {0, 0, 0};          // 8-bit-lane kernel (sub-byte samples)
+  std::map<std::tuple<int, int, int, size_t, int>, std::tuple<int, int, int>> tune;
   hipEvent_t tev[2] = {};
-  int last_db = 16, last_remap = 1;
-  int fixed_db = 0, fixed_remap = 1;    // hd_set_dedisp_tile (0: autotune)
+  int last_db = 16, last_remap = 1, last_lanes8 = 0;
+  int fixed_db = 0, fixed_remap = 1, fixed_lanes8 = 0;  // hd_set_dedisp_tile (0: autotune)
   // results of the last call
   size_t n_giants = 0;
   std::vector<float> r_peak, r_dms;
@@ -413,12 +414,12 @@ struct DeviceGuard {  // binds the pipeline's device, restores the caller's
 // staged together).  d: n_rows x nl delays; returns the pairs and the widest
 // window.
 std::vector<int> staging_pairs(const std::vector<int>& d, size_t n_rows, size_t nl, int DB,
-                               int* hw_out) {
+                               int* hw_out, int* spread_out = nullptr) {
   const int CC = hd::dedisp_chunk_channels();
   const size_t nblk = (n_rows + DB - 1) / DB;
   const size_t nchunk = (nl + CC - 1) / CC;
   std::vector<int> pairs(std::max<size_t>(nblk * nchunk, 1));
-  int hw_max = 2;
+  int hw_max = 2, spread_max = 0;
   for (size_t b = 0; b < nblk; ++b) {
     const size_t m0 = b * DB;
     const size_t ml = std::min(m0 + DB, n_rows) - 1;
@@ -429,9 +430,11 @@ std::vector<int> staging_pairs(const std::vector<int>& d, size_t n_rows, size_t 
       const int hw = hd::dedisp_window_dwords(spread);
       pairs[b * nchunk + k] = hw / 2;
       hw_max = std::max(hw_max, hw);
+      spread_max = std::max(spread_max, spread);
     }
   }
   *hw_out = hw_max;
+  if (spread_out) *spread_out = spread_max;
   return pairs;
 }
 
@@ -460,9 +463,10 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
   pl->wlen = 0;
   for (int k = 0; k < 3; ++k) {
-    int hw = 2;
-    const std::vector<int> pairs = staging_pairs(d, n_rows, nl, 8 << k, &hw);
+    int hw = 2, spread = 0;
+    const std::vector<int> pairs = staging_pairs(d, n_rows, nl, 8 << k, &hw, &spread);
     pl->wlen_db[k] = hw;
+    pl->wlen8_db[k] = hd::dedisp8_window_dwords(spread);
     pl->wlen = std::max(pl->wlen, (size_t)hw);
     HD_CHECK(pl->pairs_db[k].ensure(pairs.size() * sizeof(int)));
     HD_CHECK(hipMemcpy(pl->pairs_db[k].p, pairs.data(), pairs.size() * sizeof(int),
@@ -680,31 +684,50 @@ struct GulpIn {
   size_t nsamps, nbeams, first_idx, beam;
 };
 
-// Dedispersion autotuner.  Every tile height (8, 16 or 32 trials per block)
-// and block order (XCD-aware or linear) computes the same exact sums, so the
-// choice only moves time.  The first launch of a shape (rows per group,
-// groups, live channels, row pitch) times each candidate on the real input
+// Dedispersion autotuner.  Every tile height (8, 16 or 32 trials per block),
+// block order (XCD-aware or linear) and, for sub-byte samples, lane width
+// (16-bit pairs or 8-bit quads) computes the same exact sums, so the choice
+// only moves time.  The first launch of a shape (rows per group, groups, live
+// channels, row pitch, sample bits) times each candidate on the real input
 // with HIP events (one warm-up launch and one timed launch each, over a
 // slice of the time tiles, into the plane buffer that the real launch then
-// overwrites) and keeps the fastest; HD_DEDISP_TILE="db,remap" fixes the
-// choice instead.
+// overwrites) and keeps the fastest; HD_DEDISP_TILE="db,remap[,lanes8]" fixes
+// the choice instead.  Sets dg's tile fields and returns the staging pairs.
+int widen8_for(size_t nbits) {
+  return nbits <= 4 ? 255 / ((1 << nbits) - 1) : 0;  // channels per 8-bit lane
+}
+
+void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes8,
+                size_t nbits) {
+  const int k = db == 8 ? 0 : (db == 16 ? 1 : 2);
+  dg.db = db;
+  dg.remap = remap;
+  dg.wlen = pl->wlen_db[k];
+  dg.widen8 = lanes8 ? widen8_for(nbits) : 0;
+  dg.wlen8 = lanes8 ? pl->wlen8_db[k] : 0;
+}
+
 hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
-                            hipStream_t s) {
-  static const std::pair<int, int> forced = [] {
+                            size_t nbits, hipStream_t s, const int** pairs) {
+  static const std::tuple<int, int, int> forced = [] {
     const char* e = getenv("HD_DEDISP_TILE");
-    int db = 0, rm = 1;
-    if (e && std::sscanf(e, "%d,%d", &db, &rm) >= 1 && (db == 8 || db == 16 || db == 32))
-      return std::make_pair(db, rm ? 1 : 0);
-    return std::make_pair(0, 1);
+    int db = 0, rm = 1, l8 = 0;
+    if (e && std::sscanf(e, "%d,%d,%d", &db, &rm, &l8) >= 1 && (db == 8 || db == 16 || db == 32))
+      return std::make_tuple(db, rm ? 1 : 0, l8 ? 1 : 0);
+    return std::make_tuple(0, 1, 0);
   }();
+  const bool sub_byte = widen8_for(nbits) > 0;
+  int db, rm, l8;
   if (pl->fixed_db) {
-    dg.db = pl->fixed_db;
-    dg.remap = pl->fixed_remap;
-  } else if (forced.first) {
-    dg.db = forced.first;
-    dg.remap = forced.second;
+    db = pl->fixed_db;
+    rm = pl->fixed_remap;
+    l8 = pl->fixed_lanes8 && sub_byte;
+  } else if (std::get<0>(forced)) {
+    db = std::get<0>(forced);
+    rm = std::get<1>(forced);
+    l8 = std::get<2>(forced) && sub_byte;
   } else {
-    const auto key = std::make_tuple(dg.n_dm, gr.G, dg.n_live, dg.ppitch);
+    const auto key = std::make_tuple(dg.n_dm, gr.G, dg.n_live, dg.ppitch, (int)nbits);
     auto it = pl->tune.find(key);
     if (it == pl->tune.end()) {
       if (!pl->tev[0]) {
@@ -712,45 +735,49 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
         HD_CHECK(hipEventCreate(&pl->tev[1]));
       }
       float best = 1e30f;
-      std::pair<int, int> choice{16, 1};
-      for (int k = 0; k < 3; ++k)
-        for (int rm = 0; rm < 2; ++rm) {
-          // timed on the first (at most) 48 time tiles of every group: the
-          // tiles are alike, and large shapes would otherwise spend seconds here
-          hd::DedispGeom t = dg;
-          t.db = 8 << k;
-          t.remap = rm;
-          t.wlen = pl->wlen_db[k];
-          t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
-          if ((size_t)t.wlen * 2 * hd::dedisp_chunk_channels() * 4 > 160 * 1024) continue;
-          auto launch = [&]() {
-            return hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
-                                         pl->pairs_db[k].as<int>(), t, pl->plane.as<float>(), s,
-                                         gr);
-          };
-          if (launch() != hipSuccess) continue;
-          HD_CHECK(hipEventRecord(pl->tev[0], s));
-          HD_CHECK(launch());
-          HD_CHECK(hipEventRecord(pl->tev[1], s));
-          HD_CHECK(hipEventSynchronize(pl->tev[1]));
-          float ms = 0.f;
-          HD_CHECK(hipEventElapsedTime(&ms, pl->tev[0], pl->tev[1]));
-          if (ms < best) {
-            best = ms;
-            choice = {t.db, rm};
+      std::tuple<int, int, int> choice{16, 1, 0};
+      for (int lanes = 0; lanes < (sub_byte ? 2 : 1); ++lanes)
+        for (int k = 0; k < 3; ++k)
+          for (int r = 0; r < 2; ++r) {
+            // timed on the first (at most) 48 time tiles of every group: the
+            // tiles are alike, and large shapes would otherwise spend seconds here
+            hd::DedispGeom t = dg;
+            apply_tile(pl, t, 8 << k, r, lanes, nbits);
+            t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
+            if ((size_t)t.wlen * 2 * hd::dedisp_chunk_channels() * 4 > 160 * 1024) continue;
+            auto launch = [&]() {
+              return hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
+                                           pl->pairs_db[k].as<int>(), t, pl->plane.as<float>(),
+                                           s, gr);
+            };
+            if (launch() != hipSuccess) continue;
+            HD_CHECK(hipEventRecord(pl->tev[0], s));
+            HD_CHECK(launch());
+            HD_CHECK(hipEventRecord(pl->tev[1], s));
+            HD_CHECK(hipEventSynchronize(pl->tev[1]));
+            float ms = 0.f;
+            HD_CHECK(hipEventElapsedTime(&ms, pl->tev[0], pl->tev[1]));
+            if (ms < best) {
+              best = ms;
+              choice = std::make_tuple(8 << k, r, lanes);
+            }
           }
-        }
       it = pl->tune.emplace(key, choice).first;
       if (pl->params.verbosity >= 2)
-        std::cout << "dedispersion tile: " << choice.first << " trials per block, "
-                  << (choice.second ? "XCD-aware" : "linear") << " order (" << best << " ms)"
+        std::cout << "dedispersion tile: " << std::get<0>(choice) << " trials per block, "
+                  << (std::get<1>(choice) ? "XCD-aware" : "linear") << " order, "
+                  << (std::get<2>(choice) ? "8" : "16") << "-bit lanes (" << best << " ms)"
                   << std::endl;
     }
-    dg.db = it->second.first;
-    dg.remap = it->second.second;
+    db = std::get<0>(it->second);
+    rm = std::get<1>(it->second);
+    l8 = std::get<2>(it->second);
   }
-  pl->last_db = dg.db;
-  pl->last_remap = dg.remap;
+  apply_tile(pl, dg, db, rm, l8, nbits);
+  *pairs = pl->pairs_db[db == 8 ? 0 : (db == 16 ? 1 : 2)].as<int>();
+  pl->last_db = db;
+  pl->last_remap = rm;
+  pl->last_lanes8 = l8;
   return HD_NO_ERROR;
 }
 
@@ -1077,12 +1104,11 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   HD_CHECK(pl->plane.ensure(rows * ppitch * sizeof(float)));
   hd::DedispGeom dg{R, n_live, sh0.nout, xpitch, ppitch, (int)pl->wlen};
   {
-    hd_error te = choose_dedisp_tile(pl, dg, gr, s);
+    const int* pairs = nullptr;
+    hd_error te = choose_dedisp_tile(pl, dg, gr, nbits, s, &pairs);
     if (te != HD_NO_ERROR) return te;
-    const int k = dg.db == 8 ? 0 : (dg.db == 16 ? 1 : 2);
-    dg.wlen = pl->wlen_db[k];
-    HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
-                                   pl->pairs_db[k].as<int>(), dg, pl->plane.as<float>(), s, gr));
+    HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pairs, dg,
+                                   pl->plane.as<float>(), s, gr));
   }
   record(pl, 2);
   // baseline points and per-beam RMS (normalisation is fused into the scan)
@@ -1763,18 +1789,22 @@ extern "C" hd_size hd_get_last_proposal(hd_pipeline pl, unsigned char* flags, hd
   return n;
 }
 
-extern "C" void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap) {
+extern "C" void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap,
+                                   int* byte_lanes) {
   if (!pl) return;
   if (trials_per_block) *trials_per_block = pl->last_db;
   if (xcd_remap) *xcd_remap = pl->last_remap;
+  if (byte_lanes) *byte_lanes = pl->last_lanes8;
 }
 
-extern "C" hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap) {
+extern "C" hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap,
+                                      int byte_lanes) {
   if (!pl) return HD_INVALID_PIPELINE;
   if (trials_per_block != 0 && trials_per_block != 8 && trials_per_block != 16 &&
       trials_per_block != 32)
     return HD_INVALID_STRIDE;
   pl->fixed_db = trials_per_block;
   pl->fixed_remap = xcd_remap ? 1 : 0;
+  pl->fixed_lanes8 = byte_lanes ? 1 : 0;
   return HD_NO_ERROR;
 }

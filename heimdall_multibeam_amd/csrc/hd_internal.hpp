@@ -142,6 +142,8 @@ struct DedispGeom {
   int wlen;            // LDS dwords per staged copy (the largest block's need)
   int db = 16;         // trials per block (8, 16 or 32; the staging pairs are planned for it)
   int remap = 1;       // XCD-aware block order
+  int widen8 = 0;      // > 0: 8-bit-lane kernel for samples <= 255/widen8 (sub-byte input)
+  int wlen8 = 0;       // its LDS dwords per staged copy
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,
@@ -149,6 +151,7 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
 int dedisp_tile_time();     // outputs per block in time
 int dedisp_tile_dms();      // default DMs per block
 int dedisp_window_dwords(int spread);  // staged dwords per copy for a delay span
+int dedisp8_window_dwords(int spread); // the same for the 8-bit-lane kernel
 int dedisp_chunk_channels();           // channels staged per step
 
 hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,

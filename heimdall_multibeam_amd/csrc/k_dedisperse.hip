@@ -312,12 +312,177 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dedispersion of sub-byte samples (values <= 15), 8-bit staging.  Same block
+// shape as k_dedisperse16, but each staged dword holds 4 samples: the
+// channel's byte window is staged as 4 copies shifted by 0..3 bytes,
+// C_k[j] = (x[4j+k], .., x[4j+k+3]), so a trial's offset o reads copy o & 3
+// at dword (o>>2) + lane + 64q, and one 32-bit add accumulates 4 samples in
+// 8-bit lanes.  The lanes never carry within `widen` channels (widen * max
+// sample <= 255); they are then added into 32-bit sums.  Half the LDS reads
+// and adds of the 16-bit kernel for the same sums.
+constexpr int D8_Q = 4;                   // dwords (4 samples each) per lane per trial
+constexpr int D8_PF = 4;                  // staging dwords prefetched per thread
+
+template <int DB>
+__global__ __launch_bounds__(D2_BS, 4) void k_dedisperse8(const uint8_t* __restrict__ x,
+                                                      const int* __restrict__ delays,
+                                                      DedispGeom g, int hw_max, int widen,
+                                                      float* __restrict__ out,
+                                                      const GroupDev* __restrict__ gd) {
+  extern __shared__ uint32_t lds8[];  // [D2_CC][4][hw_max]
+  constexpr int NA = DB / D2_WAVES;
+  __shared__ __align__(16) int sdly[DB][D2_CC];
+  const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
+  const int trial0 = gd ? (int)gd[blockIdx.z].trial0 : 0;
+  x += (size_t)(gd ? gd[blockIdx.z].xslot : blockIdx.z) * g.n_live * g.xpitch;
+  out += (size_t)blockIdx.z * g.n_dm * g.ppitch;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const unsigned nbx = gridDim.x;
+  unsigned lin = blockIdx.y * nbx + blockIdx.x;
+  if (g.remap) {
+    const unsigned nblk = nbx * gridDim.y;
+    const unsigned q = nblk / 8, r = nblk % 8, xx = lin % 8;
+    lin = xx * q + min(xx, r) + lin / 8;
+  }
+  const int mb = (int)(lin % nbx);
+  const int m0 = mb * DB;
+  const int ml = min(m0 + DB, g.n_dm) - 1;
+  const size_t t0 = (size_t)(lin / nbx) * D2_TB;
+  if (t0 > nout) return;
+  const int* drow0 = delays + (size_t)(trial0 + m0) * g.n_live;
+  const int* drowl = delays + (size_t)(trial0 + ml) * g.n_live;
+
+  uint32_t pa[NA][D8_Q];
+  uint32_t ua[NA][D8_Q][4];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int q = 0; q < D8_Q; ++q) {
+      pa[a][q] = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ua[a][q][e] = 0;
+    }
+  int pending = 0;
+  const int nchunk = (g.n_live + D2_CC - 1) / D2_CC;
+  constexpr int TPC = D2_BS / D2_CC;
+  const int scc = tid / TPC;
+  const int sjj = tid % TPC;
+  // dwords of each staged copy that channel c needs: its window spans the
+  // block's delays (first trial's rounded down to a dword) plus 1024 samples
+  auto ndw = [&](int c) {
+    const int span = drowl[c] - (drow0[c] & ~3);
+    return (D2_TB + span + 3) / 4 + 1;
+  };
+  uint32_t pf_lo[D8_PF], pf_hi[D8_PF];
+  auto prefetch = [&](int k) {
+    const int c = k * D2_CC + scc;
+    if (k >= nchunk || c >= g.n_live) return;
+    const int nd = ndw(c);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(
+        x + (size_t)c * g.xpitch + t0 + (drow0[c] & ~3));
+#pragma unroll
+    for (int r = 0; r < D8_PF; ++r) {
+      const int j = sjj + r * TPC;
+      if (j < nd) {
+        pf_lo[r] = src[j];
+        pf_hi[r] = src[j + 1];
+      }
+    }
+  };
+  // copy k, dword j: bytes 4j+k .. 4j+k+3 of the window
+  auto put = [&](int j, uint32_t lo, uint32_t hi) {
+    uint32_t* C = lds8 + (size_t)(scc * 4) * hw_max;
+    C[j] = lo;
+    C[hw_max + j] = __builtin_amdgcn_alignbyte(hi, lo, 1);
+    C[2 * hw_max + j] = __builtin_amdgcn_alignbyte(hi, lo, 2);
+    C[3 * hw_max + j] = __builtin_amdgcn_alignbyte(hi, lo, 3);
+  };
+  prefetch(0);
+  int pf_delay = 0;
+  auto prefetch_delay = [&](int k) {
+    if (k < nchunk && tid < DB * D2_CC) {
+      const int c = min(k * D2_CC + tid % D2_CC, g.n_live - 1);
+      const int mm = min(m0 + tid / D2_CC, g.n_dm - 1);
+      const int o = delays[(size_t)(trial0 + mm) * g.n_live + c] - (drow0[c] & ~3);
+      pf_delay = ((tid % D2_CC) * 4 + (o & 3)) * hw_max + (o >> 2);
+    }
+  };
+  prefetch_delay(0);
+  for (int k = 0; k < nchunk; ++k) {
+    const int c0 = k * D2_CC;
+    const int ncc = min(D2_CC, g.n_live - c0);
+    __syncthreads();
+    if (tid < DB * D2_CC) sdly[tid / D2_CC][tid % D2_CC] = pf_delay;
+    if (scc < ncc) {
+      const int nd = ndw(c0 + scc);
+#pragma unroll
+      for (int r = 0; r < D8_PF; ++r) {
+        const int j = sjj + r * TPC;
+        if (j < nd) put(j, pf_lo[r], pf_hi[r]);
+      }
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(
+          x + (size_t)(c0 + scc) * g.xpitch + t0 + (drow0[c0 + scc] & ~3));
+      for (int j = sjj + D8_PF * TPC; j < nd; j += TPC) put(j, src[j], src[j + 1]);
+    }
+    __syncthreads();
+    prefetch(k + 1);
+    prefetch_delay(k + 1);
+    for (int cc = 0; cc < ncc; ++cc) {
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const uint32_t* src = lds8 + sdly[wave + D2_WAVES * a][cc] + lane;
+#pragma unroll
+        for (int q = 0; q < D8_Q; ++q) pa[a][q] += src[64 * q];
+      }
+    }
+    pending += ncc;
+    if (pending + D2_CC > widen || c0 + D2_CC >= g.n_live) {
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int q = 0; q < D8_Q; ++q) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) ua[a][q][e] += (pa[a][q] >> (8 * e)) & 0xFFu;
+          pa[a][q] = 0;
+        }
+      pending = 0;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    const int m = m0 + wave + D2_WAVES * a;
+    if (m >= g.n_dm) continue;
+    float* orow = out + (size_t)m * g.ppitch;
+#pragma unroll
+    for (int q = 0; q < D8_Q; ++q) {
+      const size_t t = t0 + 4 * (lane + 64 * q);
+      if (t + 3 < g.ppitch) {
+        float4 v;
+        v.x = (t + 0 < nout) ? (float)ua[a][q][0] : 0.f;
+        v.y = (t + 1 < nout) ? (float)ua[a][q][1] : 0.f;
+        v.z = (t + 2 < nout) ? (float)ua[a][q][2] : 0.f;
+        v.w = (t + 3 < nout) ? (float)ua[a][q][3] : 0.f;
+        *reinterpret_cast<float4*>(orow + t) = v;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 int dedisp_tile_time() { return D2_TB; }
 // Default trials per block (the pipeline's autotuner may choose 8 or 32).
 int dedisp_tile_dms() { return D2_DB; }
 int dedisp_chunk_channels() { return D2_CC; }
+int dedisp8_window_dwords(int spread) {
+  // copies of 4 samples per dword: the 1024-sample tile plus the span, one
+  // extra dword for the shifted copies' last bytes, rounded to 64
+  const int need = (D2_TB + spread + 3) / 4 + 1;
+  return (need + 63) / 64 * 64;
+}
 int dedisp_window_dwords(int spread) {
   // E/O copies: dword (o>>1) + 511 with o <= spread + 3, rounded to 64
   const int need = (spread + 3) / 2 + 512 + 1;
@@ -348,6 +513,29 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
   if (db != 8 && db != 16 && db != 32) return hipErrorInvalidValue;
   dim3 grid((unsigned)((g.n_dm + db - 1) / db), (unsigned)(g.ppitch / D2_TB),
             (unsigned)ga.G);
+  if (g.widen8 > 0) {
+    if (g.widen8 < D2_CC || g.wlen8 <= 0) return hipErrorInvalidValue;
+    const size_t lds8 = (size_t)D2_CC * 4 * (size_t)g.wlen8 * sizeof(uint32_t);
+    if (lds8 > 160 * 1024) return hipErrorInvalidValue;
+    const void* fn8 = db == 32 ? reinterpret_cast<const void*>(&k_dedisperse8<32>)
+                    : db == 8  ? reinterpret_cast<const void*>(&k_dedisperse8<8>)
+                               : reinterpret_cast<const void*>(&k_dedisperse8<16>);
+    if (lds8 > 64 * 1024) {
+      hipError_t e =
+          hipFuncSetAttribute(fn8, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds8);
+      if (e != hipSuccess) return e;
+    }
+    if (db == 32)
+      hipLaunchKernelGGL(k_dedisperse8<32>, grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8,
+                         g.widen8, plane, ga.gd);
+    else if (db == 8)
+      hipLaunchKernelGGL(k_dedisperse8<8>, grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8,
+                         g.widen8, plane, ga.gd);
+    else
+      hipLaunchKernelGGL(k_dedisperse8<16>, grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8,
+                         g.widen8, plane, ga.gd);
+    return hipGetLastError();
+  }
   const size_t lds = (size_t)D2_CC * 2 * (size_t)g.wlen * sizeof(uint32_t);
   if (lds > 160 * 1024) return hipErrorInvalidValue;  // delay span of a DM block too wide
   const void* fn = db == 32 ? reinterpret_cast<const void*>(&k_dedisperse16<32>)

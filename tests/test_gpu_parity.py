@@ -138,8 +138,34 @@ def test_every_dedisp_tile_gives_the_same_search(oracle, db):
         pl.set_output(False)
         pl.set_dedisp_tile(db, remap)
         err, processed = pl.execute(fb, w.nsamps, w.nbits)
-        assert pl.dedisp_tile() == (db, remap)
+        assert pl.dedisp_tile() == (db, remap, 0)
         gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
                    text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
         pl.close()
         _compare(gpu, r, oracle, w)
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4])
+def test_byte_lane_dedispersion_matches_oracle(oracle, nbits):
+    """The 8-bit-lane kernel for sub-byte input, fixed for every tile height,
+    matches the oracle bit for bit (and so the 16-bit kernel)."""
+    import dataclasses
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import synth
+    vmax = (1 << nbits) - 1
+    w = dataclasses.replace(SMALL_128, nbits=nbits, mean=vmax / 2.0, sigma=max(vmax / 3.0, 0.5))
+    fb = synth.make_gulp(w, 40 + nbits)
+    p = oracle_params_for(oracle, w)
+    r = oracle.execute(p, fb, w.nsamps, nbits, 0, 1)
+    for db in (8, 16, 32):
+        print("case byte lanes", nbits, "bits, tile", db, flush=True)
+        pl = H.Pipeline(params_for(w))
+        pl.set_output(False)
+        pl.set_dedisp_tile(db, 1, 1)
+        err, processed = pl.execute(fb, w.nsamps, nbits)
+        assert pl.dedisp_tile() == (db, 1, 1)
+        gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
+                   text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
+        pl.close()
+        _compare(gpu, r, oracle, w)
+
