@@ -117,14 +117,18 @@ def batch_main(a):
         pl.execute_batch(gulps, w.nbits, device=True)
         print("warmup %d done" % i, file=sys.stderr, flush=True)
     hip.synchronize()
-    stage = {k: [] for k in H.Pipeline.STAGES}
     t0 = time.perf_counter()
+    for i in range(a.steps):
+        pl.execute_batch(gulps, w.nbits, device=True)
+    hip.synchronize()
+    elapsed = time.perf_counter() - t0
+    graphs = pl.graph_stats()
+    pl.set_graph(False)
+    stage = {k: [] for k in H.Pipeline.STAGES}
     for i in range(a.steps):
         pl.execute_batch(gulps, w.nbits, device=True)
         for k, v in pl.timings().items():
             stage[k].append(v)
-    hip.synchronize()
-    elapsed = time.perf_counter() - t0
     n_cand = len(pl.candidates()["peak"])
     n_giants = len(pl.giants()["peak"])
     value = beam_s * a.steps / elapsed
@@ -151,6 +155,7 @@ def batch_main(a):
                    "giants": n_giants,
                    "parallelism": "1 GPU"},
         "stage_ms": {k: round(float(np.mean(v)), 4) for k, v in stage.items() if v},
+        "hip_graph": {"captures": graphs[0], "replays": graphs[1]},
         "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
                         "byte_lanes": pl.dedisp_tile()[2],
                         "chosen_by": "autotuner (first call of the shape)"},
@@ -374,17 +379,24 @@ def main():
         pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
         print("warmup %d done" % i, file=sys.stderr, flush=True)
     hip.synchronize()
-    stage = {k: [] for k in H.Pipeline.STAGES}
     n_cand = 0
     t0 = time.perf_counter()
     for i in range(a.steps):
         pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
-        for k, v in pl.timings().items():
-            stage[k].append(v)
         n_cand = len(pl.candidates()["peak"])
     hip.synchronize()
-    n_giants = len(pl.giants()["peak"])
     elapsed = time.perf_counter() - t0
+    n_giants = len(pl.giants()["peak"])
+    graphs = pl.graph_stats()
+    # per-stage times (and the dedispersion kernel's, for the roofline) from
+    # the same number of calls with the kernels launched one by one: in the
+    # graph the chain is one span
+    pl.set_graph(False)
+    stage = {k: [] for k in H.Pipeline.STAGES}
+    for i in range(a.steps):
+        pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
+        for k, v in pl.timings().items():
+            stage[k].append(v)
     ms_per_step = elapsed / a.steps * 1e3
     value = beam_s * a.steps / elapsed
     print("steps done: %.2f ms/step, %d candidates" % (ms_per_step, n_cand), file=sys.stderr,
@@ -424,6 +436,9 @@ def main():
                    "giants": n_giants,
                    "parallelism": "1 GPU"},
         "stage_ms": stages_ms,
+        "hip_graph": {"captures": graphs[0], "replays": graphs[1],
+                      "note": "timed steps replay the captured stage chain; stage_ms and "
+                              "the roofline come from as many calls launched kernel by kernel"},
         "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
                         "byte_lanes": pl.dedisp_tile()[2],
                         "chosen_by": "autotuner (first call of the shape)"},

@@ -123,6 +123,10 @@ def lib():
     L.hd_get_dedisp_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int)]
     L.hd_get_dedisp_tile.restype = None
+    L.hd_set_graph.argtypes = [vp, C.c_int]
+    L.hd_set_graph.restype = None
+    L.hd_get_graph_stats.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.hd_get_graph_stats.restype = sz
     L.hd_set_dedisp_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.hd_set_dedisp_tile.restype = C.c_int
     L.hd_clean_filterbank_rfi.argtypes = [sz, C.c_float, C.c_float, vp, sz, sz, vp,
@@ -201,6 +205,16 @@ class Pipeline:
 
     def set_output(self, enabled):
         lib().hd_set_output_enabled(self.handle, 1 if enabled else 0)
+
+    def set_graph(self, enabled):
+        """Stage chain as a HIP graph (default on) or launched kernel by kernel."""
+        lib().hd_set_graph(self.handle, 1 if enabled else 0)
+
+    def graph_stats(self):
+        """(graphs captured, graph replays) of this pipeline."""
+        n = C.c_size_t(0)
+        c = lib().hd_get_graph_stats(self.handle, C.byref(n))
+        return c, n.value
 
     def dedisp_tile(self):
         """(trials per block, XCD-aware order, 8-bit lanes) of the last call's dedispersion."""

@@ -145,6 +145,11 @@ struct hd_pipeline_t {
   std::map<std::tuple<int, int, int, size_t, int>, std::tuple<int, int, int>> tune;
   hipEvent_t tev[2] = {};
   int last_db = 16, last_remap = 1, last_lanes8 = 0;
+  // the stage chain as a HIP graph (hd_set_graph)
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::string gkey;
+  size_t graph_captures = 0, graph_launches = 0;
   int fixed_db = 0, fixed_remap = 1, fixed_lanes8 = 0;  // hd_set_dedisp_tile (0: autotune)
   // results of the last call
   size_t n_giants = 0;
@@ -989,6 +994,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   }
   const uint64_t span = (uint64_t)p.cand_sep_time << wt.wi[wt.nw - 1].filter_idx;
   pl->hgd.assign(G, hd::GroupDev{});
+  for (hd::GroupDev& d : pl->hgd) std::memset(&d, 0, sizeof(d));  // bytes key the graph
   uint64_t t_off = 0;
   for (int g = 0; g < G; ++g) {
     const CallShape& cs = shapes[active[g]];
@@ -1100,57 +1106,23 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   gr.host = hrg.data();
   gr.G = Gr;
   const size_t rows = (size_t)Gr * R;
-  // dedisperse
+  // Every buffer of the stage chain (dedispersion .. thresholding) is sized
+  // first: the chain may be captured as a graph, which must not allocate.
   HD_CHECK(pl->plane.ensure(rows * ppitch * sizeof(float)));
-  hd::DedispGeom dg{R, n_live, sh0.nout, xpitch, ppitch, (int)pl->wlen};
-  {
-    const int* pairs = nullptr;
-    hd_error te = choose_dedisp_tile(pl, dg, gr, nbits, s, &pairs);
-    if (te != HD_NO_ERROR) return te;
-    HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pairs, dg,
-                                   pl->plane.as<float>(), s, gr));
-  }
-  record(pl, 2);
-  // baseline points and per-beam RMS (normalisation is fused into the scan)
   HD_CHECK(pl->pts.ensure((rows * gr.prow + 64) * sizeof(float)));
   HD_CHECK(pl->rms.ensure(rows * nbmax * sizeof(float)));
-  HD_CHECK(hd::launch_baseline_points(pl->plane.as<float>(), sh0, R, pl->pts.as<float>(), s,
-                                      gr));
-  record(pl, 3);
-  {
-    const size_t words = hd::beam_rms_scratch_words(sh0, R, gr);
-    HD_CHECK(pl->brms.ensure(std::max<size_t>(words, 1) * sizeof(float)));
-    HD_CHECK(hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh0, R,
-                                 pl->rms.as<float>(), s, pl->brms.as<float>(), words, gr));
-  }
-  record(pl, 4);
+  const size_t rms_words = hd::beam_rms_scratch_words(sh0, R, gr);
+  HD_CHECK(pl->brms.ensure(std::max<size_t>(rms_words, 1) * sizeof(float)));
   // normalised prefix sums; rows start 3 floats into the allocation so that
   // S[1..] is 16-byte aligned for the scan's vector stores
   HD_CHECK(pl->S.ensure((rows * spitch + 8) * sizeof(float)));
   float* const Sbase = pl->S.as<float>() + 3;
-  HD_CHECK(hd::launch_scan(pl->plane.as<float>(), ppitch, 0, R, Sbase, spitch, s,
-                           pl->pts.as<float>(), pl->rms.as<float>(), nullptr, gr));
-  record(pl, 5);
-  // block RMS: the ts == 1 widths together, then each decimated width in order
   const size_t np_max = hd::block_rms_np_max(gr);
   HD_CHECK(pl->partial.ensure(rows * wt.nw * std::max<size_t>(np_max, 1) * sizeof(float)));
   HD_CHECK(pl->wrms.ensure(rows * wt.nw * sizeof(float)));
   HD_CHECK(pl->winv.ensure(rows * wt.nw * sizeof(float)));
   int n_fresh = 0;
   while (n_fresh < wt.nw && wt.wi[n_fresh].ts == 1) ++n_fresh;
-  HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, 0, n_fresh, R,
-                                         pl->partial.as<float>(), pl->winv.as<float>(), s, gr));
-  HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, 0, n_fresh, R,
-                                       pl->wrms.as<float>(), pl->winv.as<float>(), s, gr));
-  for (int j = n_fresh; j < wt.nw; ++j) {
-    HD_CHECK(hd::launch_block_rms_partials(Sbase, spitch, wt, j, 1, R,
-                                           pl->partial.as<float>(), pl->winv.as<float>(), s,
-                                           gr));
-    HD_CHECK(hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, j, 1, R,
-                                         pl->wrms.as<float>(), pl->winv.as<float>(), s, gr));
-  }
-  record(pl, 6);
-  // threshold + ordered compaction
   hd::GiantBuffers gb{};
   gb.chunk = 4096;
   size_t n_w0 = 0;
@@ -1163,10 +1135,9 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   HD_CHECK(pl->counter.ensure(sizeof(unsigned) * 4));
   HD_CHECK(pl->scan_tmp.ensure(hd::exclusive_scan_tmp_words(std::max<size_t>(nkeys, 1 << 20)) *
                                sizeof(unsigned)));
-  unsigned nsamp = 0;
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    HD_CHECK(pl->skey.ensure(pl->sample_cap * sizeof(uint64_t)));
-    HD_CHECK(pl->sval.ensure(pl->sample_cap * sizeof(float)));
+  HD_CHECK(pl->skey.ensure(pl->sample_cap * sizeof(uint64_t)));
+  HD_CHECK(pl->sval.ensure(pl->sample_cap * sizeof(float)));
+  auto bind_gb = [&]() {
     gb.skey = pl->skey.as<uint64_t>();
     gb.sval = pl->sval.as<float>();
     gb.scap = pl->sample_cap;
@@ -1175,9 +1146,126 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     gb.coff = pl->coff.as<unsigned>();
     gb.cpos = pl->cpos.as<unsigned>();
     gb.scan_tmp = pl->scan_tmp.as<unsigned>();
-    HD_CHECK(hipMemsetAsync(gb.counter, 0, sizeof(unsigned), s));
-    HD_CHECK(hd::launch_threshold(Sbase, spitch, wt, pl->winv.as<float>(), R,
-                                  p.detect_thresh, gb, s, gr));
+  };
+  bind_gb();
+  hd::DedispGeom dg;
+  std::memset(&dg, 0, sizeof(dg));  // padding too: the struct's bytes key the graph
+  dg.n_dm = R;
+  dg.n_live = n_live;
+  dg.nout = sh0.nout;
+  dg.xpitch = xpitch;
+  dg.ppitch = ppitch;
+  dg.wlen = (int)pl->wlen;
+  const int* pairs = nullptr;
+  {
+    hd_error te = choose_dedisp_tile(pl, dg, gr, nbits, s, &pairs);  // may time candidates
+    if (te != HD_NO_ERROR) return te;
+  }
+  auto threshold = [&]() -> hipError_t {
+    hipError_t e = hd::launch_fill_u32(gb.counter, 1, 0u, s);
+    if (e == hipSuccess)
+      e = hd::launch_threshold(Sbase, spitch, wt, pl->winv.as<float>(), R, p.detect_thresh, gb,
+                               s, gr);
+    return e;
+  };
+  // The chain: kernel launches only.  With `timed`, stage events in between.
+  auto chain = [&](bool timed) -> hipError_t {
+    hipError_t e = hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pairs, dg,
+                                         pl->plane.as<float>(), s, gr);
+    if (e != hipSuccess) return e;
+    if (timed) record(pl, 2);
+    e = hd::launch_baseline_points(pl->plane.as<float>(), sh0, R, pl->pts.as<float>(), s, gr);
+    if (e != hipSuccess) return e;
+    if (timed) record(pl, 3);
+    e = hd::launch_beam_rms(pl->plane.as<float>(), pl->pts.as<float>(), sh0, R,
+                            pl->rms.as<float>(), s, pl->brms.as<float>(), rms_words, gr);
+    if (e != hipSuccess) return e;
+    if (timed) record(pl, 4);
+    e = hd::launch_scan(pl->plane.as<float>(), ppitch, 0, R, Sbase, spitch, s,
+                        pl->pts.as<float>(), pl->rms.as<float>(), nullptr, gr);
+    if (e != hipSuccess) return e;
+    if (timed) record(pl, 5);
+    // block RMS: the ts == 1 widths together, then each decimated width in order
+    e = hd::launch_block_rms_partials(Sbase, spitch, wt, 0, n_fresh, R, pl->partial.as<float>(),
+                                      pl->winv.as<float>(), s, gr);
+    if (e == hipSuccess)
+      e = hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, 0, n_fresh, R,
+                                      pl->wrms.as<float>(), pl->winv.as<float>(), s, gr);
+    for (int j = n_fresh; j < wt.nw && e == hipSuccess; ++j) {
+      e = hd::launch_block_rms_partials(Sbase, spitch, wt, j, 1, R, pl->partial.as<float>(),
+                                        pl->winv.as<float>(), s, gr);
+      if (e == hipSuccess)
+        e = hd::launch_block_rms_finish(pl->partial.as<float>(), wt.nw, j, 1, R,
+                                        pl->wrms.as<float>(), pl->winv.as<float>(), s, gr);
+    }
+    if (e != hipSuccess) return e;
+    if (timed) record(pl, 6);
+    return threshold();
+  };
+  if (pl->use_graph && !(pl->gating || pl->coarse_only)) {
+    // Graph of the chain, captured once per set of launch arguments (shapes,
+    // tile, buffers, the row-group table without its input pointers) and
+    // replayed while they repeat.  The row-group table itself is uploaded
+    // above on every call.  Its span is timed as the dedispersion stage.
+    // (the structs below are zero-filled when made, so their padding is too)
+    std::string key;
+    auto put = [&](const void* v, size_t n) { key.append(static_cast<const char*>(v), n); };
+    for (const hd::GroupDev& d : hrg) {
+      hd::GroupDev k;
+      std::memcpy(&k, &d, sizeof(k));
+      k.fb = nullptr;
+      put(&k, sizeof(k));
+    }
+    put(&sh0, sizeof(sh0));
+    put(&wt, sizeof(wt));
+    put(&dg, sizeof(dg));
+    put(&gb, sizeof(gb));
+    const void* ptrs[] = {pl->x.p, pl->delays.p, pairs, pl->plane.p, pl->pts.p, pl->rms.p,
+                          pl->brms.p, pl->S.p, pl->partial.p, pl->wrms.p, pl->winv.p,
+                          pl->rdev.p};
+    put(ptrs, sizeof(ptrs));
+    const size_t sc[] = {rows, (size_t)R, (size_t)Gr, gr.prow, gr.nbmax, rms_words, np_max,
+                         nkeys, spitch, (size_t)n_fresh};
+    put(sc, sizeof(sc));
+    put(&p.detect_thresh, sizeof(p.detect_thresh));
+    if (!pl->gexec || key != pl->gkey) {
+      if (pl->gexec) {
+        (void)hipGraphExecDestroy(pl->gexec);
+        pl->gexec = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      HD_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const hipError_t ce = chain(false);
+      const hipError_t ee = hipStreamEndCapture(s, &graph);
+      if (ce != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return gpu_err(ce != hipSuccess ? ce : ee);
+      }
+      const hipError_t ie = hipGraphInstantiate(&pl->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        pl->gexec = nullptr;
+        return gpu_err(ie);
+      }
+      pl->gkey = key;
+      ++pl->graph_captures;
+    }
+    HD_CHECK(hipGraphLaunch(pl->gexec, s));
+    ++pl->graph_launches;
+    for (int ev = 2; ev <= 6; ++ev) record(pl, ev);
+  } else {
+    HD_CHECK(chain(true));
+  }
+  // samples above threshold; with too many for the buffer, grow it and redo
+  // the threshold (eagerly: the graph is recaptured for the new buffer)
+  unsigned nsamp = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (attempt > 0) {
+      HD_CHECK(pl->skey.ensure(pl->sample_cap * sizeof(uint64_t)));
+      HD_CHECK(pl->sval.ensure(pl->sample_cap * sizeof(float)));
+      bind_gb();
+      HD_CHECK(threshold());
+    }
     HD_CHECK(hipMemcpyAsync(pl->h_pinned, gb.counter, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     HD_CHECK(hipStreamSynchronize(s));
     nsamp = pl->h_pinned[0];
@@ -1441,6 +1529,10 @@ extern "C" hd_error hd_create_pipeline(hd_pipeline* pipeline_, hd_params params)
   pl->params.channel_zaps = nullptr;
   pl->params.sigproc_file = nullptr;
   pl->device = params.gpu_id;
+  {
+    const char* g = getenv("HD_GRAPH");
+    pl->use_graph = g ? atoi(g) != 0 : true;
+  }
   DeviceGuard dg(pl->device);
   if (params.verbosity >= 1) {
     std::cout << "Process " << params.beam << " using GPU " << params.gpu_id << std::endl;
@@ -1592,6 +1684,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->co_ncols, &pl->co_flags, &pl->co_pos};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : pl->pairs_db) b.release();
+    if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
     for (hipEvent_t& e : pl->tev)
       if (e) (void)hipEventDestroy(e);
     if (pl->net) hd_cnn_destroy(pl->net);
@@ -1807,4 +1900,22 @@ extern "C" hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int
   pl->fixed_remap = xcd_remap ? 1 : 0;
   pl->fixed_lanes8 = byte_lanes ? 1 : 0;
   return HD_NO_ERROR;
+}
+
+extern "C" void hd_set_graph(hd_pipeline pl, int enabled) {
+  if (!pl) return;
+  pl->use_graph = enabled != 0;
+  if (!pl->use_graph && pl->gexec) {
+    DeviceGuard dg(pl->device);
+    (void)hipStreamSynchronize(pl->stream);
+    (void)hipGraphExecDestroy(pl->gexec);
+    pl->gexec = nullptr;
+    pl->gkey.clear();
+  }
+}
+
+extern "C" hd_size hd_get_graph_stats(hd_pipeline pl, hd_size* launches) {
+  if (!pl) return 0;
+  if (launches) *launches = pl->graph_launches;
+  return pl->graph_captures;
 }

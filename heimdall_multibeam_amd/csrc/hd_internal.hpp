@@ -204,6 +204,8 @@ struct GiantBuffers {
 hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
                             const float* winv, int n_dm, float thresh, GiantBuffers& gb,
                             hipStream_t s, const GroupArgs& ga);
+// p[0..n) = v with a kernel (no memset node when a stream is captured).
+hipError_t launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s);
 hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n,
                                      unsigned* tmp, hipStream_t s);
 size_t exclusive_scan_tmp_words(size_t n);

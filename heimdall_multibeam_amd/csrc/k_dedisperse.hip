@@ -505,6 +505,16 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
   return hipGetLastError();
 }
 
+// Raises a kernel's dynamic-LDS limit once (never during a later capture:
+// every shape is launched eagerly before its chain is captured).
+hipError_t lds_limit(const void* fn, int slot, size_t lds) {
+  static size_t set[6] = {0, 0, 0, 0, 0, 0};
+  if (lds <= 64 * 1024 || lds <= set[slot]) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) set[slot] = lds;
+  return e;
+}
+
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,
                              const GroupArgs& ga) {
@@ -520,9 +530,8 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
     const void* fn8 = db == 32 ? reinterpret_cast<const void*>(&k_dedisperse8<32>)
                     : db == 8  ? reinterpret_cast<const void*>(&k_dedisperse8<8>)
                                : reinterpret_cast<const void*>(&k_dedisperse8<16>);
-    if (lds8 > 64 * 1024) {
-      hipError_t e =
-          hipFuncSetAttribute(fn8, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds8);
+    {
+      const hipError_t e = lds_limit(fn8, 3 + (db == 8 ? 0 : db == 16 ? 1 : 2), lds8);
       if (e != hipSuccess) return e;
     }
     if (db == 32)
@@ -541,8 +550,8 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
   const void* fn = db == 32 ? reinterpret_cast<const void*>(&k_dedisperse16<32>)
                  : db == 8  ? reinterpret_cast<const void*>(&k_dedisperse16<8>)
                             : reinterpret_cast<const void*>(&k_dedisperse16<16>);
-  if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  {
+    const hipError_t e = lds_limit(fn, db == 8 ? 0 : db == 16 ? 1 : 2, lds);
     if (e != hipSuccess) return e;
   }
   if (db == 32)

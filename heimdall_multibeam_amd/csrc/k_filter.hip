@@ -625,6 +625,17 @@ hipError_t launch_segment_heads(const GiantBuffers& gb, const WidthTable& wt, un
   return hipGetLastError();
 }
 
+__global__ void k_fill_u32(unsigned* __restrict__ p, size_t n, unsigned v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(unsigned* p, size_t n, unsigned v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_segment_reduce(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
                                  unsigned ngiants, int n_dm, GiantOut go, uint64_t* best,
                                  hipStream_t s, const GroupDev* gd) {

@@ -86,6 +86,16 @@ void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap, i
  * applies to 1-, 2- and 4-bit input only), or returns to the autotuner (0). */
 hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap, int byte_lanes);
 
+/* Runs the per-gulp stage chain from dedispersion to thresholding as a HIP
+ * graph: captured on the first call of a set of launch arguments (shapes,
+ * dedispersion tile, buffers) and replayed while they repeat (on by default;
+ * HD_GRAPH=0 in the environment or this call with 0 launches the kernels one
+ * by one).  Gated calls always launch one by one.  In graph mode the stage
+ * timings report the whole chain as the dedispersion stage. */
+void hd_set_graph(hd_pipeline pl, int enabled);
+/* Graphs captured so far; launches (may be NULL) receives the replays. */
+hd_size hd_get_graph_stats(hd_pipeline pl, hd_size* launches);
+
 /* When enabled, hd_execute* skips the .cand file / socket output (used by
  * benchmarks and tests that read candidates through the accessors). */
 void hd_set_output_enabled(hd_pipeline pl, int enabled);
