@@ -25,6 +25,12 @@ HBM_PEAK_GBS = 8000.0
 VALU_INT_PEAK = 256 * 64 * 2.4e9  # int32 lane adds/s (256 CUs x 64 lanes x 2.4 GHz)
 
 
+def tile_json(t):
+    """The dedispersion tile the autotuner chose (Pipeline.dedisp_tile())."""
+    return {"trials_per_block": t[0], "xcd_remap": t[1], "byte_lanes": t[2],
+            "channels_per_step": t[3], "chosen_by": "autotuner (first call of the shape)"}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,9 +162,7 @@ def batch_main(a):
                    "parallelism": "1 GPU"},
         "stage_ms": {k: round(float(np.mean(v)), 4) for k, v in stage.items() if v},
         "hip_graph": {"captures": graphs[0], "replays": graphs[1]},
-        "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
-                        "byte_lanes": pl.dedisp_tile()[2],
-                        "chosen_by": "autotuner (first call of the shape)"},
+        "dedisp_tile": tile_json(pl.dedisp_tile()),
         "roofline": {"bound": "hbm", "achieved": round(alg / t_dd / 1e9, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(alg / t_dd / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
@@ -287,8 +291,7 @@ def gated_eval(a, nbeams=64, per_call=16, steps=1, warmup=1, max_ops=None):
                        % (nbeams, w.nchans, w.nbits, w.dt * 1e6, w.nsamps, w.dm_min, w.dm_max,
                           len(dms), w.boxcar_max, w.min_tscrunch_width),
            "full_beam_s_per_s": round(beam_s / t_full, 3), "candidates_snr8": len(strong),
-           "dedisp_tile": {"trials_per_block": full_tile[0], "xcd_remap": full_tile[1],
-                           "byte_lanes": full_tile[2]},
+           "dedisp_tile": tile_json(full_tile),
            "full_stage_ms": {k: round(sum(r[3].get(k, 0.0) for r in full[1]), 3)
                              for k in H.Pipeline.STAGES},
            "operating_points": []}
@@ -439,9 +442,7 @@ def main():
         "hip_graph": {"captures": graphs[0], "replays": graphs[1],
                       "note": "timed steps replay the captured stage chain; stage_ms and "
                               "the roofline come from as many calls launched kernel by kernel"},
-        "dedisp_tile": {"trials_per_block": pl.dedisp_tile()[0], "xcd_remap": pl.dedisp_tile()[1],
-                        "byte_lanes": pl.dedisp_tile()[2],
-                        "chosen_by": "autotuner (first call of the shape)"},
+        "dedisp_tile": tile_json(pl.dedisp_tile()),
         "roofline": roof,
         "cpu_baseline": None,
     }

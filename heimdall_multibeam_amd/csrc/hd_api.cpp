@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <arpa/inet.h>
 #include <cmath>
 #include <cstdio>
@@ -139,18 +140,20 @@ struct hd_pipeline_t {
   size_t wlen = 0;  // staged dwords per copy of the widest DM block (any tile height)
   // dedispersion tile heights the autotuner chooses from, with their staging
   // windows; the choice per launch shape (rows, groups, live channels, pitch)
-  DevBuf pairs_db[3];
-  int wlen_db[3] = {0, 0, 0};
-  int wlen8_db[3] = {0, 0, 0};          // 8-bit-lane kernel (sub-byte samples)
-  std::map<std::tuple<int, int, int, size_t, int>, std::tuple<int, int, int>> tune;
+  // staging plans per tile shape, index log2(db / 8) + 3 * log2(cc / 4)
+  DevBuf pairs_db[9];
+  int wlen_db[9] = {};
+  int wlen8_db[9] = {};  // 8-bit-lane kernel (sub-byte samples)
+  std::map<std::tuple<int, int, int, int>, std::array<int, 4>> tune;
   hipEvent_t tev[2] = {};
-  int last_db = 16, last_remap = 1, last_lanes8 = 0;
+  int last_db = 16, last_remap = 1, last_lanes8 = 0, last_cc = 4;
   // the stage chain as a HIP graph (hd_set_graph)
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
   std::string gkey;
   size_t graph_captures = 0, graph_launches = 0;
-  int fixed_db = 0, fixed_remap = 1, fixed_lanes8 = 0;  // hd_set_dedisp_tile (0: autotune)
+  // hd_set_dedisp_tile (fixed_db 0: autotune)
+  int fixed_db = 0, fixed_remap = 1, fixed_lanes8 = 0, fixed_cc = 4;
   // results of the last call
   size_t n_giants = 0;
   std::vector<float> r_peak, r_dms;
@@ -419,8 +422,7 @@ struct DeviceGuard {  // binds the pipeline's device, restores the caller's
 // staged together).  d: n_rows x nl delays; returns the pairs and the widest
 // window.
 std::vector<int> staging_pairs(const std::vector<int>& d, size_t n_rows, size_t nl, int DB,
-                               int* hw_out, int* spread_out = nullptr) {
-  const int CC = hd::dedisp_chunk_channels();
+                               int CC, int* hw_out, int* spread_out = nullptr) {
   const size_t nblk = (n_rows + DB - 1) / DB;
   const size_t nchunk = (nl + CC - 1) / CC;
   std::vector<int> pairs(std::max<size_t>(nblk * nchunk, 1));
@@ -467,12 +469,13 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   HD_CHECK(hipMemcpy(pl->live.p, pl->live_list.data(), nl * sizeof(int), hipMemcpyHostToDevice));
   HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
   pl->wlen = 0;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 9; ++k) {
     int hw = 2, spread = 0;
-    const std::vector<int> pairs = staging_pairs(d, n_rows, nl, 8 << k, &hw, &spread);
+    const std::vector<int> pairs = staging_pairs(d, n_rows, nl, 8 << (k % 3), 4 << (k / 3), &hw,
+                                                 &spread);
     pl->wlen_db[k] = hw;
     pl->wlen8_db[k] = hd::dedisp8_window_dwords(spread);
-    pl->wlen = std::max(pl->wlen, (size_t)hw);
+    pl->wlen = std::max(pl->wlen, (size_t)hw);  // sizes the input rows' padding
     HD_CHECK(pl->pairs_db[k].ensure(pairs.size() * sizeof(int)));
     HD_CHECK(hipMemcpy(pl->pairs_db[k].p, pairs.data(), pairs.size() * sizeof(int),
                        hipMemcpyHostToDevice));
@@ -504,7 +507,8 @@ hd_error build_coarse(hd_pipeline pl) {
     for (size_t l = 0; l < nl; ++l)
       d[m * nl + l] = hdplan::delay(pl->co_dms[m], pl->co_kc[pl->live_list[l]]);
   int hw_max = 2;
-  const std::vector<int> pairs = staging_pairs(d, nc, nl, hd::dedisp_tile_dms(), &hw_max);
+  const std::vector<int> pairs =
+      staging_pairs(d, nc, nl, hd::dedisp_tile_dms(), hd::dedisp_chunk_channels(), &hw_max);
   pl->co_wlen = (size_t)hw_max;
   HD_CHECK(pl->co_delays.ensure(d.size() * sizeof(int)));
   HD_CHECK(pl->co_pairs.ensure(pairs.size() * sizeof(int)));
@@ -690,22 +694,28 @@ struct GulpIn {
 };
 
 // Dedispersion autotuner.  Every tile height (8, 16 or 32 trials per block),
-// block order (XCD-aware or linear) and, for sub-byte samples, lane width
-// (16-bit pairs or 8-bit quads) computes the same exact sums, so the choice
-// only moves time.  The first launch of a shape (rows per group, groups, live
-// channels, row pitch, sample bits) times each candidate on the real input
-// with HIP events (one warm-up launch and one timed launch each, over a
-// slice of the time tiles, into the plane buffer that the real launch then
-// overwrites) and keeps the fastest; HD_DEDISP_TILE="db,remap[,lanes8]" fixes
-// the choice instead.  Sets dg's tile fields and returns the staging pairs.
+// channels staged per step (4, 8 or 16), block order (XCD-aware or linear) and,
+// for sub-byte samples, lane width (16-bit pairs or 8-bit quads) computes the
+// same exact sums, so the choice only moves time.  The first launch of a
+// shape (rows per group, grouped or not, live channels, sample bits) times
+// each candidate on the real input with HIP events (one warm-up launch and
+// one timed launch each, over a slice of the time tiles and groups, into the
+// plane buffer that the real launch then overwrites) and keeps the fastest;
+// HD_DEDISP_TILE="db,remap[,lanes8[,cc]]" fixes the choice instead.  Sets
+// dg's tile fields and returns the staging pairs.
 int widen8_for(size_t nbits) {
   return nbits <= 4 ? 255 / ((1 << nbits) - 1) : 0;  // channels per 8-bit lane
 }
 
-void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes8,
+int tile_index(int db, int cc) {
+  return (db == 8 ? 0 : (db == 16 ? 1 : 2)) + (cc == 4 ? 0 : (cc == 8 ? 3 : 6));
+}
+
+void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes8, int cc,
                 size_t nbits) {
-  const int k = db == 8 ? 0 : (db == 16 ? 1 : 2);
+  const int k = tile_index(db, cc);
   dg.db = db;
+  dg.cc = cc;
   dg.remap = remap;
   dg.wlen = pl->wlen_db[k];
   dg.widen8 = lanes8 ? widen8_for(nbits) : 0;
@@ -714,25 +724,25 @@ void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes
 
 hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
                             size_t nbits, hipStream_t s, const int** pairs) {
-  static const std::tuple<int, int, int> forced = [] {
+  using Tile = std::array<int, 4>;  // trials per block, remap, 8-bit lanes, channels per step
+  static const Tile forced = [] {
     const char* e = getenv("HD_DEDISP_TILE");
-    int db = 0, rm = 1, l8 = 0;
-    if (e && std::sscanf(e, "%d,%d,%d", &db, &rm, &l8) >= 1 && (db == 8 || db == 16 || db == 32))
-      return std::make_tuple(db, rm ? 1 : 0, l8 ? 1 : 0);
-    return std::make_tuple(0, 1, 0);
+    int db = 0, rm = 1, l8 = 0, cc = 4;
+    if (e && std::sscanf(e, "%d,%d,%d,%d", &db, &rm, &l8, &cc) >= 1 &&
+        (db == 8 || db == 16 || db == 32))
+      return Tile{db, rm ? 1 : 0, l8 ? 1 : 0, (cc == 8 || cc == 16) ? cc : 4};
+    return Tile{0, 1, 0, 4};
   }();
   const bool sub_byte = widen8_for(nbits) > 0;
-  int db, rm, l8;
+  Tile c;
   if (pl->fixed_db) {
-    db = pl->fixed_db;
-    rm = pl->fixed_remap;
-    l8 = pl->fixed_lanes8 && sub_byte;
-  } else if (std::get<0>(forced)) {
-    db = std::get<0>(forced);
-    rm = std::get<1>(forced);
-    l8 = std::get<2>(forced) && sub_byte;
+    c = Tile{pl->fixed_db, pl->fixed_remap, pl->fixed_lanes8 && sub_byte, pl->fixed_cc};
+  } else if (forced[0]) {
+    c = Tile{forced[0], forced[1], forced[2] && sub_byte, forced[3]};
   } else {
-    const auto key = std::make_tuple(dg.n_dm, gr.G, dg.n_live, dg.ppitch, (int)nbits);
+    // rows per group, live channels, sample bits, grouped or not: the number of
+    // groups and the row pitch only scale the grid
+    const auto key = std::make_tuple(dg.n_dm, gr.gd ? 1 : 0, dg.n_live, (int)nbits);
     auto it = pl->tune.find(key);
     if (it == pl->tune.end()) {
       if (!pl->tev[0]) {
@@ -740,49 +750,58 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
         HD_CHECK(hipEventCreate(&pl->tev[1]));
       }
       float best = 1e30f;
-      std::tuple<int, int, int> choice{16, 1, 0};
+      Tile choice{16, 1, 0, 4};
       for (int lanes = 0; lanes < (sub_byte ? 2 : 1); ++lanes)
-        for (int k = 0; k < 3; ++k)
-          for (int r = 0; r < 2; ++r) {
-            // timed on the first (at most) 48 time tiles of every group: the
-            // tiles are alike, and large shapes would otherwise spend seconds here
-            hd::DedispGeom t = dg;
-            apply_tile(pl, t, 8 << k, r, lanes, nbits);
-            t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
-            if ((size_t)t.wlen * 2 * hd::dedisp_chunk_channels() * 4 > 160 * 1024) continue;
-            auto launch = [&]() {
-              return hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(),
-                                           pl->pairs_db[k].as<int>(), t, pl->plane.as<float>(),
-                                           s, gr);
-            };
-            if (launch() != hipSuccess) continue;
-            HD_CHECK(hipEventRecord(pl->tev[0], s));
-            HD_CHECK(launch());
-            HD_CHECK(hipEventRecord(pl->tev[1], s));
-            HD_CHECK(hipEventSynchronize(pl->tev[1]));
-            float ms = 0.f;
-            HD_CHECK(hipEventElapsedTime(&ms, pl->tev[0], pl->tev[1]));
-            if (ms < best) {
-              best = ms;
-              choice = std::make_tuple(8 << k, r, lanes);
+        for (int cc = 4; cc <= 16; cc *= 2)
+          for (int k = 0; k < 3; ++k)
+            for (int r = 0; r < 2; ++r) {
+              // timed on the first (at most) 48 time tiles of the first (at most)
+              // 8 groups: the tiles are alike, and large shapes would otherwise
+              // spend seconds here
+              hd::DedispGeom t = dg;
+              apply_tile(pl, t, 8 << k, r, lanes, cc, nbits);
+              t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
+              hd::GroupArgs tg = gr;
+              tg.G = std::min(gr.G, 8);
+              if (hd::dedisp_lds_bytes(t) > 160 * 1024) continue;
+              const int* tp = pl->pairs_db[tile_index(8 << k, cc)].as<int>();
+              auto launch = [&]() {
+                return hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), tp, t,
+                                             pl->plane.as<float>(), s, tg);
+              };
+              if (launch() != hipSuccess) continue;
+              HD_CHECK(hipEventRecord(pl->tev[0], s));
+              HD_CHECK(launch());
+              HD_CHECK(hipEventRecord(pl->tev[1], s));
+              HD_CHECK(hipEventSynchronize(pl->tev[1]));
+              float ms = 0.f;
+              HD_CHECK(hipEventElapsedTime(&ms, pl->tev[0], pl->tev[1]));
+              if (ms < best) {
+                best = ms;
+                choice = Tile{8 << k, r, lanes, cc};
+              }
             }
-          }
       it = pl->tune.emplace(key, choice).first;
       if (pl->params.verbosity >= 2)
-        std::cout << "dedispersion tile: " << std::get<0>(choice) << " trials per block, "
-                  << (std::get<1>(choice) ? "XCD-aware" : "linear") << " order, "
-                  << (std::get<2>(choice) ? "8" : "16") << "-bit lanes (" << best << " ms)"
+        std::cout << "dedispersion tile: " << choice[0] << " trials per block, " << choice[3]
+                  << " channels per step, " << (choice[1] ? "XCD-aware" : "linear")
+                  << " order, " << (choice[2] ? "8" : "16") << "-bit lanes (" << best << " ms)"
                   << std::endl;
     }
-    db = std::get<0>(it->second);
-    rm = std::get<1>(it->second);
-    l8 = std::get<2>(it->second);
+    c = it->second;
   }
-  apply_tile(pl, dg, db, rm, l8, nbits);
-  *pairs = pl->pairs_db[db == 8 ? 0 : (db == 16 ? 1 : 2)].as<int>();
-  pl->last_db = db;
-  pl->last_remap = rm;
-  pl->last_lanes8 = l8;
+  if (c[2] && widen8_for(nbits) < c[3]) c[2] = 0;  // 8-bit lanes would carry within a step
+  apply_tile(pl, dg, c[0], c[1], c[2], c[3], nbits);
+  // a fixed tile whose staged copies exceed the LDS stages fewer channels per step
+  while (c[3] > 4 && hd::dedisp_lds_bytes(dg) > 160 * 1024) {
+    c[3] /= 2;
+    apply_tile(pl, dg, c[0], c[1], c[2], c[3], nbits);
+  }
+  *pairs = pl->pairs_db[tile_index(c[0], c[3])].as<int>();
+  pl->last_db = c[0];
+  pl->last_remap = c[1];
+  pl->last_lanes8 = c[2];
+  pl->last_cc = c[3];
   return HD_NO_ERROR;
 }
 
@@ -946,6 +965,9 @@ hd_error propose(hd_pipeline pl, hipStream_t s) {
 hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) {
   const hd_params& p = pl->params;
   hipStream_t s = pl->stream;
+  // the launchers report hipGetLastError(): start from a clean state, so that
+  // an error an earlier call already returned is not reported again here
+  (void)hipGetLastError();
   pl->n_giants = 0;
   pl->r_peak.clear(); pl->r_ind.clear(); pl->r_begin.clear(); pl->r_end.clear();
   pl->r_filter.clear(); pl->r_dm.clear(); pl->r_members.clear(); pl->r_dms.clear();
@@ -1883,22 +1905,26 @@ extern "C" hd_size hd_get_last_proposal(hd_pipeline pl, unsigned char* flags, hd
 }
 
 extern "C" void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap,
-                                   int* byte_lanes) {
+                                   int* byte_lanes, int* channels_per_step) {
   if (!pl) return;
   if (trials_per_block) *trials_per_block = pl->last_db;
   if (xcd_remap) *xcd_remap = pl->last_remap;
   if (byte_lanes) *byte_lanes = pl->last_lanes8;
+  if (channels_per_step) *channels_per_step = pl->last_cc;
 }
 
 extern "C" hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap,
-                                      int byte_lanes) {
+                                      int byte_lanes, int channels_per_step) {
   if (!pl) return HD_INVALID_PIPELINE;
   if (trials_per_block != 0 && trials_per_block != 8 && trials_per_block != 16 &&
       trials_per_block != 32)
     return HD_INVALID_STRIDE;
+  if (channels_per_step != 4 && channels_per_step != 8 && channels_per_step != 16)
+    return HD_INVALID_STRIDE;
   pl->fixed_db = trials_per_block;
   pl->fixed_remap = xcd_remap ? 1 : 0;
   pl->fixed_lanes8 = byte_lanes ? 1 : 0;
+  pl->fixed_cc = channels_per_step;
   return HD_NO_ERROR;
 }
 

@@ -144,15 +144,17 @@ struct DedispGeom {
   int remap = 1;       // XCD-aware block order
   int widen8 = 0;      // > 0: 8-bit-lane kernel for samples <= 255/widen8 (sub-byte input)
   int wlen8 = 0;       // its LDS dwords per staged copy
+  int cc = 4;          // channels staged per step (4, 8 or 16; the staging pairs are planned for it)
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,
                              const GroupArgs& ga = GroupArgs{});
+size_t dedisp_lds_bytes(const DedispGeom& g);  // LDS per block of g's tile (limit 160 KiB)
 int dedisp_tile_time();     // outputs per block in time
 int dedisp_tile_dms();      // default DMs per block
 int dedisp_window_dwords(int spread);  // staged dwords per copy for a delay span
 int dedisp8_window_dwords(int spread); // the same for the 8-bit-lane kernel
-int dedisp_chunk_channels();           // channels staged per step
+int dedisp_chunk_channels();           // default channels staged per step
 
 hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
                                   float* pts, hipStream_t s, const GroupArgs& ga = GroupArgs{});

@@ -132,19 +132,25 @@ constexpr int D2_BS = 512;
 constexpr int D2_WAVES = D2_BS / 64;     // 8: wave w sums trials w, w+8
 constexpr int D2_PF = 6;                  // staging items prefetched per thread
 
+// Waves per SIMD each tile shape is compiled for (its register budget is
+// 512 / waves): the largest at which its accumulators and the compiler's
+// load hoisting fit without spilling; register spills cost more than the
+// occupancy given up.
+constexpr int d16_waves(int db, int cc) { return db == 32 ? 2 : (db == 16 && cc >= 8 ? 3 : 4); }
+
 // chunk_pairs[block][chunk]: staged E/O dword pairs each channel of a
 // 4-channel chunk needs for this block of trials (host-computed from the
 // delay spans, so top-of-band chunks stage short windows).
-template <int DB>
-__global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __restrict__ x,
+template <int DB, int CC>
+__global__ __launch_bounds__(D2_BS, d16_waves(DB, CC)) void k_dedisperse16(const uint8_t* __restrict__ x,
                                                        const int* __restrict__ delays,
                                                        const int* __restrict__ chunk_pairs,
                                                        DedispGeom g, int hw_max, int xcd_remap,
                                                        float* __restrict__ out,
                                                        const GroupDev* __restrict__ gd) {
-  extern __shared__ uint32_t lds16[];  // [D2_CC][2][hw_max]
+  extern __shared__ uint32_t lds16[];  // [CC][2][hw_max]
   constexpr int NA = DB / D2_WAVES;  // trials per wave
-  __shared__ __align__(16) int sdly[DB][D2_CC];
+  __shared__ __align__(16) int sdly[DB][CC];
   // blockIdx.z: the group, with its own unpacked rows, output rows and length
   // its trials start at trial0 of the plan's delay table (a whole number of
   // blocks in) and its unpacked input is slot xslot of x
@@ -188,16 +194,16 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
   // channel c0+cc into E/O dwords 2j, 2j+1.  The global loads of chunk k+1
   // are issued into registers before chunk k is summed, so their latency
   // hides behind the LDS work.
-  const int nchunk = (g.n_live + D2_CC - 1) / D2_CC;
+  const int nchunk = (g.n_live + CC - 1) / CC;
   const int* cpairs = chunk_pairs + (size_t)(trial0 / DB + mb) * nchunk;
-  // D2_BS/D2_CC threads per staged channel; thread jj handles pairs jj + TPC*r
-  constexpr int TPC = D2_BS / D2_CC;
+  // D2_BS/CC threads per staged channel; thread jj handles pairs jj + TPC*r
+  constexpr int TPC = D2_BS / CC;
   const int scc = tid / TPC;
   const int sjj = tid % TPC;
   uint32_t pf_lo[D2_PF], pf_hi[D2_PF];
   auto prefetch = [&](int k) {
     if (k >= nchunk) return;
-    const int c = k * D2_CC + scc;
+    const int c = k * CC + scc;
     if (c < g.n_live) {
       const int pairs = cpairs[k];
       const uint32_t* src = reinterpret_cast<const uint32_t*>(
@@ -227,22 +233,22 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
   prefetch(0);
   int pf_delay = 0;
   auto prefetch_delay = [&](int k) {
-    if (k < nchunk && tid < DB * D2_CC) {
-      const int c = min(k * D2_CC + tid % D2_CC, g.n_live - 1);
-      const int mm = min(m0 + tid / D2_CC, g.n_dm - 1);
+    if (k < nchunk && tid < DB * CC) {
+      const int c = min(k * CC + tid % CC, g.n_live - 1);
+      const int mm = min(m0 + tid / CC, g.n_dm - 1);
       // the dword of the trial's staged copy (E or O of channel slot cc)
       // where lane 0 reads: cc*2 + (o&1) selects the copy, o>>1 the pair
       const int o = delays[(size_t)(trial0 + mm) * g.n_live + c] - (drow0[c] & ~3);
-      pf_delay = ((tid % D2_CC) * 2 + (o & 1)) * hw_max + (o >> 1);
+      pf_delay = ((tid % CC) * 2 + (o & 1)) * hw_max + (o >> 1);
     }
   };
   prefetch_delay(0);
   for (int k = 0; k < nchunk; ++k) {
-    const int c0 = k * D2_CC;
-    const int ncc = min(D2_CC, g.n_live - c0);
+    const int c0 = k * CC;
+    const int ncc = min(CC, g.n_live - c0);
     const int pairs = cpairs[k];
     __syncthreads();
-    if (tid < DB * D2_CC) sdly[tid / D2_CC][tid % D2_CC] = pf_delay;
+    if (tid < DB * CC) sdly[tid / CC][tid % CC] = pf_delay;
     if (scc < ncc) {
 #pragma unroll
       for (int r = 0; r < D2_PF; ++r) {
@@ -256,19 +262,28 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
     __syncthreads();
     prefetch(k + 1);
     prefetch_delay(k + 1);
-    if (ncc == D2_CC) {
-      // all four channels: every offset of the wave's trials read up front
-      int4 off[NA];
+    if (ncc == CC) {
+      // a full chunk, four channels at a time: their offsets for the wave's
+      // trials in one 16-byte read per trial
 #pragma unroll
-      for (int a = 0; a < NA; ++a) off[a] = *reinterpret_cast<const int4*>(&sdly[wave + D2_WAVES * a][0]);
-#pragma unroll
-      for (int cc = 0; cc < D2_CC; ++cc) {
+      for (int v = 0; v < CC / 4; ++v) {
+        int off[NA][4];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
-          const int o = cc == 0 ? off[a].x : cc == 1 ? off[a].y : cc == 2 ? off[a].z : off[a].w;
-          const uint32_t* src = lds16 + o + lane;
+          const int4 o4 = *reinterpret_cast<const int4*>(&sdly[wave + D2_WAVES * a][4 * v]);
+          off[a][0] = o4.x;
+          off[a][1] = o4.y;
+          off[a][2] = o4.z;
+          off[a][3] = o4.w;
+        }
 #pragma unroll
-          for (int q = 0; q < D2_Q; ++q) pa[a][q] += src[64 * q];
+        for (int cc = 0; cc < 4; ++cc) {
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+            const uint32_t* src = lds16 + off[a][cc] + lane;
+#pragma unroll
+            for (int q = 0; q < D2_Q; ++q) pa[a][q] += src[64 * q];
+          }
         }
       }
     } else {
@@ -282,7 +297,7 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
       }
     }
     pending += ncc;
-    if (pending > 256 - D2_CC || c0 + D2_CC >= g.n_live) {
+    if (pending > 256 - CC || c0 + CC >= g.n_live) {
 #pragma unroll
       for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -322,17 +337,20 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse16(const uint8_t* __rest
 // sample <= 255); they are then added into 32-bit sums.  Half the LDS reads
 // and adds of the 16-bit kernel for the same sums.
 constexpr int D8_Q = 4;                   // dwords (4 samples each) per lane per trial
+// (the 32-trial, 4-channel tile spills a few registers at 4 waves but runs
+// faster there than at 3)
+constexpr int d8_waves(int db, int cc) { return db == 32 && cc >= 8 ? 3 : 4; }
 constexpr int D8_PF = 4;                  // staging dwords prefetched per thread
 
-template <int DB>
-__global__ __launch_bounds__(D2_BS, 4) void k_dedisperse8(const uint8_t* __restrict__ x,
+template <int DB, int CC>
+__global__ __launch_bounds__(D2_BS, d8_waves(DB, CC)) void k_dedisperse8(const uint8_t* __restrict__ x,
                                                       const int* __restrict__ delays,
                                                       DedispGeom g, int hw_max, int widen,
                                                       float* __restrict__ out,
                                                       const GroupDev* __restrict__ gd) {
-  extern __shared__ uint32_t lds8[];  // [D2_CC][4][hw_max]
+  extern __shared__ uint32_t lds8[];  // [CC][4][hw_max]
   constexpr int NA = DB / D2_WAVES;
-  __shared__ __align__(16) int sdly[DB][D2_CC];
+  __shared__ __align__(16) int sdly[DB][CC];
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
   const int trial0 = gd ? (int)gd[blockIdx.z].trial0 : 0;
   x += (size_t)(gd ? gd[blockIdx.z].xslot : blockIdx.z) * g.n_live * g.xpitch;
@@ -366,8 +384,8 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse8(const uint8_t* __restr
       for (int e = 0; e < 4; ++e) ua[a][q][e] = 0;
     }
   int pending = 0;
-  const int nchunk = (g.n_live + D2_CC - 1) / D2_CC;
-  constexpr int TPC = D2_BS / D2_CC;
+  const int nchunk = (g.n_live + CC - 1) / CC;
+  constexpr int TPC = D2_BS / CC;
   const int scc = tid / TPC;
   const int sjj = tid % TPC;
   // dwords of each staged copy that channel c needs: its window spans the
@@ -378,7 +396,7 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse8(const uint8_t* __restr
   };
   uint32_t pf_lo[D8_PF], pf_hi[D8_PF];
   auto prefetch = [&](int k) {
-    const int c = k * D2_CC + scc;
+    const int c = k * CC + scc;
     if (k >= nchunk || c >= g.n_live) return;
     const int nd = ndw(c);
     const uint32_t* src = reinterpret_cast<const uint32_t*>(
@@ -403,19 +421,19 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse8(const uint8_t* __restr
   prefetch(0);
   int pf_delay = 0;
   auto prefetch_delay = [&](int k) {
-    if (k < nchunk && tid < DB * D2_CC) {
-      const int c = min(k * D2_CC + tid % D2_CC, g.n_live - 1);
-      const int mm = min(m0 + tid / D2_CC, g.n_dm - 1);
+    if (k < nchunk && tid < DB * CC) {
+      const int c = min(k * CC + tid % CC, g.n_live - 1);
+      const int mm = min(m0 + tid / CC, g.n_dm - 1);
       const int o = delays[(size_t)(trial0 + mm) * g.n_live + c] - (drow0[c] & ~3);
-      pf_delay = ((tid % D2_CC) * 4 + (o & 3)) * hw_max + (o >> 2);
+      pf_delay = ((tid % CC) * 4 + (o & 3)) * hw_max + (o >> 2);
     }
   };
   prefetch_delay(0);
   for (int k = 0; k < nchunk; ++k) {
-    const int c0 = k * D2_CC;
-    const int ncc = min(D2_CC, g.n_live - c0);
+    const int c0 = k * CC;
+    const int ncc = min(CC, g.n_live - c0);
     __syncthreads();
-    if (tid < DB * D2_CC) sdly[tid / D2_CC][tid % D2_CC] = pf_delay;
+    if (tid < DB * CC) sdly[tid / CC][tid % CC] = pf_delay;
     if (scc < ncc) {
       const int nd = ndw(c0 + scc);
 #pragma unroll
@@ -439,7 +457,7 @@ __global__ __launch_bounds__(D2_BS, 4) void k_dedisperse8(const uint8_t* __restr
       }
     }
     pending += ncc;
-    if (pending + D2_CC > widen || c0 + D2_CC >= g.n_live) {
+    if (pending + CC > widen || c0 + CC >= g.n_live) {
 #pragma unroll
       for (int a = 0; a < NA; ++a)
 #pragma unroll
@@ -508,61 +526,79 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
 // Raises a kernel's dynamic-LDS limit once (never during a later capture:
 // every shape is launched eagerly before its chain is captured).
 hipError_t lds_limit(const void* fn, int slot, size_t lds) {
-  static size_t set[6] = {0, 0, 0, 0, 0, 0};
+  static size_t set[18] = {0};
   if (lds <= 64 * 1024 || lds <= set[slot]) return hipSuccess;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess) set[slot] = lds;
   return e;
 }
 
+// Kernel instances by tile index: (log2(db / 8)) + 3 * log2(cc / 4).
+int tile_slot(int db, int cc) {
+  return (db == 8 ? 0 : db == 16 ? 1 : 2) + (cc == 4 ? 0 : cc == 8 ? 3 : 6);
+}
+template <int DB, int CC>
+const void* fn16() {
+  return reinterpret_cast<const void*>(&k_dedisperse16<DB, CC>);
+}
+template <int DB, int CC>
+const void* fn8() {
+  return reinterpret_cast<const void*>(&k_dedisperse8<DB, CC>);
+}
+const void* kernel16(int slot) {
+  static const void* const t[9] = {fn16<8, 4>(),  fn16<16, 4>(),  fn16<32, 4>(),
+                                   fn16<8, 8>(),  fn16<16, 8>(),  fn16<32, 8>(),
+                                   fn16<8, 16>(), fn16<16, 16>(), fn16<32, 16>()};
+  return t[slot];
+}
+const void* kernel8(int slot) {
+  static const void* const t[9] = {fn8<8, 4>(),  fn8<16, 4>(),  fn8<32, 4>(),
+                                   fn8<8, 8>(),  fn8<16, 8>(),  fn8<32, 8>(),
+                                   fn8<8, 16>(), fn8<16, 16>(), fn8<32, 16>()};
+  return t[slot];
+}
+
+// LDS of a tile: the staged copies (dynamic) plus the trial offsets (static).
+size_t dedisp_lds_bytes(const DedispGeom& g) {
+  const size_t offs = (size_t)g.db * g.cc * sizeof(int);
+  if (g.widen8 > 0) return (size_t)g.cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t) + offs;
+  return (size_t)g.cc * 2 * (size_t)g.wlen * sizeof(uint32_t) + offs;
+}
+
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,
                              const GroupArgs& ga) {
   // DM blocks fastest: blocks resident together share one time tile's input
-  const int db = g.db;
+  const int db = g.db, cc = g.cc;
   if (db != 8 && db != 16 && db != 32) return hipErrorInvalidValue;
+  if (cc != 4 && cc != 8 && cc != 16) return hipErrorInvalidValue;
   dim3 grid((unsigned)((g.n_dm + db - 1) / db), (unsigned)(g.ppitch / D2_TB),
             (unsigned)ga.G);
+  const int slot = tile_slot(db, cc);
+  // the delay span of a DM block too wide for this tile's staged copies
+  if (dedisp_lds_bytes(g) > 160 * 1024) return hipErrorInvalidValue;
   if (g.widen8 > 0) {
-    if (g.widen8 < D2_CC || g.wlen8 <= 0) return hipErrorInvalidValue;
-    const size_t lds8 = (size_t)D2_CC * 4 * (size_t)g.wlen8 * sizeof(uint32_t);
-    if (lds8 > 160 * 1024) return hipErrorInvalidValue;
-    const void* fn8 = db == 32 ? reinterpret_cast<const void*>(&k_dedisperse8<32>)
-                    : db == 8  ? reinterpret_cast<const void*>(&k_dedisperse8<8>)
-                               : reinterpret_cast<const void*>(&k_dedisperse8<16>);
-    {
-      const hipError_t e = lds_limit(fn8, 3 + (db == 8 ? 0 : db == 16 ? 1 : 2), lds8);
-      if (e != hipSuccess) return e;
-    }
-    if (db == 32)
-      hipLaunchKernelGGL(k_dedisperse8<32>, grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8,
-                         g.widen8, plane, ga.gd);
-    else if (db == 8)
-      hipLaunchKernelGGL(k_dedisperse8<8>, grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8,
-                         g.widen8, plane, ga.gd);
-    else
-      hipLaunchKernelGGL(k_dedisperse8<16>, grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8,
-                         g.widen8, plane, ga.gd);
+    if (g.widen8 < cc || g.wlen8 <= 0) return hipErrorInvalidValue;
+    const size_t lds8 = (size_t)cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t);
+    const void* f8 = kernel8(slot);
+    const hipError_t e = lds_limit(f8, 9 + slot, lds8);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(reinterpret_cast<void (*)(const uint8_t*, const int*, DedispGeom, int, int,
+                                                 float*, const GroupDev*)>(
+                           const_cast<void*>(f8)),
+                       grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8, g.widen8, plane,
+                       ga.gd);
     return hipGetLastError();
   }
-  const size_t lds = (size_t)D2_CC * 2 * (size_t)g.wlen * sizeof(uint32_t);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;  // delay span of a DM block too wide
-  const void* fn = db == 32 ? reinterpret_cast<const void*>(&k_dedisperse16<32>)
-                 : db == 8  ? reinterpret_cast<const void*>(&k_dedisperse16<8>)
-                            : reinterpret_cast<const void*>(&k_dedisperse16<16>);
-  {
-    const hipError_t e = lds_limit(fn, db == 8 ? 0 : db == 16 ? 1 : 2, lds);
-    if (e != hipSuccess) return e;
-  }
-  if (db == 32)
-    hipLaunchKernelGGL(k_dedisperse16<32>, grid, dim3(D2_BS), lds, s, x, d_delays,
-                       d_chunk_pairs, g, g.wlen, g.remap, plane, ga.gd);
-  else if (db == 8)
-    hipLaunchKernelGGL(k_dedisperse16<8>, grid, dim3(D2_BS), lds, s, x, d_delays,
-                       d_chunk_pairs, g, g.wlen, g.remap, plane, ga.gd);
-  else
-    hipLaunchKernelGGL(k_dedisperse16<16>, grid, dim3(D2_BS), lds, s, x, d_delays,
-                       d_chunk_pairs, g, g.wlen, g.remap, plane, ga.gd);
+  const size_t lds = (size_t)cc * 2 * (size_t)g.wlen * sizeof(uint32_t);
+  const void* fn = kernel16(slot);
+  const hipError_t e = lds_limit(fn, slot, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(reinterpret_cast<void (*)(const uint8_t*, const int*, const int*, DedispGeom,
+                                               int, int, float*, const GroupDev*)>(
+                         const_cast<void*>(fn)),
+                     grid, dim3(D2_BS), lds, s, x, d_delays, d_chunk_pairs, g, g.wlen, g.remap,
+                     plane, ga.gd);
   return hipGetLastError();
 }
 

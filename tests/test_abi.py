@@ -136,12 +136,9 @@ def test_manual_killmask_end_exclusive():
 def test_create_pipeline_without_gpu_fails_loudly():
     """No silent CPU path: without a device the pipeline refuses to exist."""
     import heimdall_multibeam_amd as H
-    try:
-        import torch
-        if torch.cuda.is_available():
-            pytest.skip("a GPU is present")
-    except ImportError:
-        pass
+    from tests.conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
     with pytest.raises(H.HdError) as e:
         H.Pipeline(H.default_params())
     assert e.value.code == H.HD_INVALID_DEVICE_INDEX

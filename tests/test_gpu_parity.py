@@ -124,21 +124,22 @@ def test_first_idx_and_beam_offset(oracle):
 
 @pytest.mark.parametrize("db", [8, 16, 32])
 def test_every_dedisp_tile_gives_the_same_search(oracle, db):
-    """The autotuner's candidates (tile height, block order) all compute the
-    same exact sums: fixed to each, the search matches the oracle bit for bit."""
+    """The autotuner's candidates (tile height, block order, channels staged
+    per step) all compute the same exact sums: fixed to each, the search
+    matches the oracle bit for bit."""
     import heimdall_multibeam_amd as H
     from heimdall_multibeam_amd import synth
     w = SMALL_128
     fb = synth.make_gulp(w, 31)
     p = oracle_params_for(oracle, w)
     r = oracle.execute(p, fb, w.nsamps, w.nbits, 0, 1)
-    for remap in (0, 1):
-        print("case tile", db, "remap", remap, flush=True)
+    for remap, cc in ((0, 4), (1, 4), (1, 8), (1, 16)):
+        print("case tile", db, "remap", remap, "channels per step", cc, flush=True)
         pl = H.Pipeline(params_for(w))
         pl.set_output(False)
-        pl.set_dedisp_tile(db, remap)
+        pl.set_dedisp_tile(db, remap, 0, cc)
         err, processed = pl.execute(fb, w.nsamps, w.nbits)
-        assert pl.dedisp_tile() == (db, remap, 0)
+        assert pl.dedisp_tile() == (db, remap, 0, cc)
         gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
                    text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
         pl.close()
@@ -157,13 +158,13 @@ def test_byte_lane_dedispersion_matches_oracle(oracle, nbits):
     fb = synth.make_gulp(w, 40 + nbits)
     p = oracle_params_for(oracle, w)
     r = oracle.execute(p, fb, w.nsamps, nbits, 0, 1)
-    for db in (8, 16, 32):
-        print("case byte lanes", nbits, "bits, tile", db, flush=True)
+    for db, cc in ((8, 4), (16, 4), (32, 4), (16, 8), (32, 8), (32, 16)):
+        print("case byte lanes", nbits, "bits, tile", db, "channels per step", cc, flush=True)
         pl = H.Pipeline(params_for(w))
         pl.set_output(False)
-        pl.set_dedisp_tile(db, 1, 1)
+        pl.set_dedisp_tile(db, 1, 1, cc)
         err, processed = pl.execute(fb, w.nsamps, nbits)
-        assert pl.dedisp_tile() == (db, 1, 1)
+        assert pl.dedisp_tile() == (db, 1, 1, cc)
         gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
                    text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
         pl.close()
