@@ -756,13 +756,14 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
           for (int k = 0; k < 3; ++k)
             for (int r = 0; r < 2; ++r) {
               // timed on the first (at most) 48 time tiles of the first (at most)
-              // 8 groups: the tiles are alike, and large shapes would otherwise
-              // spend seconds here
+              // 64 groups: the tiles are alike, and large shapes would otherwise
+              // spend seconds here; fewer groups would leave a launch too small
+              // to rank the tiles as the full launch does
               hd::DedispGeom t = dg;
               apply_tile(pl, t, 8 << k, r, lanes, cc, nbits);
               t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
               hd::GroupArgs tg = gr;
-              tg.G = std::min(gr.G, 8);
+              tg.G = std::min(gr.G, 64);
               if (hd::dedisp_lds_bytes(t) > 160 * 1024) continue;
               const int* tp = pl->pairs_db[tile_index(8 << k, cc)].as<int>();
               auto launch = [&]() {

@@ -75,6 +75,27 @@ __device__ float lds_finish_f(float* A, float* B, size_t n) {
   return result;
 }
 
+// dst[t] = src[t * stride] for t < n (LDS from global), with SU loads in
+// flight per thread before their LDS writes: a plain loop waits out the
+// memory latency once per element.
+__device__ inline void stage_lds(float* dst, const float* __restrict__ src, size_t n,
+                                 size_t stride) {
+  constexpr int SU = 8;
+  for (size_t t0 = threadIdx.x; t0 < n; t0 += (size_t)SU * FB_BS) {
+    float r[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const size_t t = t0 + (size_t)u * FB_BS;
+      r[u] = t < n ? src[t * stride] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const size_t t = t0 + (size_t)u * FB_BS;
+      if (t < n) dst[t] = r[u];
+    }
+  }
+}
+
 // One block per (chunk, DM): level-L0 medians of |F_w| for widths
 // [first_w, first_w + n_group).  `tile_ok` selects the LDS-staged path for a
 // group of ts == 1 widths.  A single decimated width (ts > 1) stages the
@@ -128,8 +149,7 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
   const size_t n1 = bs.chunk / 5;
   float* B = A + (n1 ? n1 : 1);
   if (use_tile) {
-    const float* src = S + pos0 + wt.off - bmax;
-    for (size_t t = threadIdx.x; t < tlen; t += FB_BS) T[t] = src[t];
+    stage_lds(T, S + pos0 + wt.off - bmax, tlen, 1);
     __syncthreads();
   }
   unsigned wts = 0;
@@ -139,8 +159,7 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     sscale = winv[src_j];
     wts = (ws.ahead + ws.behind) / ws.ts;
     const size_t span = bs.chunk + wts;
-    const float* src = S + wt.off - ws.behind + (size_t)ws.ts * pos0;
-    for (size_t t = threadIdx.x; t < span; t += FB_BS) T[t] = src[t * ws.ts];
+    stage_lds(T, S + wt.off - ws.behind + (size_t)ws.ts * pos0, span, ws.ts);
     __syncthreads();
   }
   // level-L0 median of the chunk's leaves for one width (each mode below
@@ -255,7 +274,7 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     const float* __restrict__ Sall, size_t spitch, const GroupDev* __restrict__ gd,
     WidthTable wt, unsigned amax, unsigned bmax, const float* __restrict__ winv_all,
     float thresh, GiantBuffers gb) {
-  extern __shared__ float T[];  // TH_CHUNK + amax + bmax prefix sums
+  extern __shared__ float4 T4[];  // TH_CHUNK + amax + bmax prefix sums (+ alignment)
   __shared__ unsigned wcnt[TH_PER][FB_BS / 64];
   __shared__ unsigned sbase;
   const size_t c = blockIdx.x;
@@ -279,12 +298,38 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       return;
     }
   }
-  const size_t tbase = p0 + wt.off - bmax;        // S index held in T[0]
+  // T[t] holds S[tbase + t] (0 outside [0, s_len)).  Staged in 16-byte
+  // pieces from the 16-byte boundary at or below S + tbase, several loads in
+  // flight per thread before their LDS writes.
+  const long long tbase = (long long)p0 + (long long)wt.off - (long long)bmax;
   const unsigned tlen = (unsigned)(TH_CHUNK + amax + bmax);
-  for (unsigned t = threadIdx.x; t < tlen; t += FB_BS) {
-    const size_t si = tbase + t;
-    T[t] = si < s_len ? S[si] : 0.f;
+  const unsigned sh = (unsigned)((((uintptr_t)S >> 2) + (unsigned long long)tbase) & 3u);
+  const long long tal = tbase - sh;               // S + tal is 16-byte aligned
+  const unsigned n4 = (tlen + sh + 3) / 4;
+  const long long slen = (long long)s_len;
+  constexpr int SU = 4;
+  for (unsigned v0 = threadIdx.x; v0 < n4; v0 += SU * FB_BS) {
+    float4 r[SU];
+#pragma unroll
+    for (int u = 0; u < SU; ++u) {
+      const unsigned v = v0 + u * FB_BS;
+      const long long si = tal + 4ll * v;
+      if (v < n4) {
+        if (si >= 0 && si + 3 < slen) {
+          r[u] = *reinterpret_cast<const float4*>(S + si);
+        } else {
+          r[u].x = (si >= 0 && si < slen) ? S[si] : 0.f;
+          r[u].y = (si + 1 >= 0 && si + 1 < slen) ? S[si + 1] : 0.f;
+          r[u].z = (si + 2 >= 0 && si + 2 < slen) ? S[si + 2] : 0.f;
+          r[u].w = (si + 3 >= 0 && si + 3 < slen) ? S[si + 3] : 0.f;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SU; ++u)
+      if (v0 + u * FB_BS < n4) T4[v0 + u * FB_BS] = r[u];
   }
+  const float* T = reinterpret_cast<const float*>(T4) + sh;
   __syncthreads();
   const unsigned tid = threadIdx.x;
   const int lane = (int)(tid & 63), wave = (int)(tid >> 6);
@@ -579,8 +624,8 @@ hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
     bmax = wt.wi[j].behind > bmax ? wt.wi[j].behind : bmax;
   }
   const size_t tlen = (size_t)TH_CHUNK + amax + bmax;
-  const size_t lds = tlen * sizeof(float);
-  if (lds > 65536) return hipErrorInvalidValue;  // boxcar_max beyond ~12000
+  const size_t lds = (tlen + 8) * sizeof(float);  // + the 16-byte alignment of the staging
+  if (lds > 65536) return hipErrorInvalidValue;   // boxcar_max beyond ~12000
   dim3 grid((unsigned)gb.nchunks_per_w, n_dm, ga.G);
   hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), lds, s, S, spitch, ga.gd, wt, amax, bmax,
                      winv, thresh, gb);

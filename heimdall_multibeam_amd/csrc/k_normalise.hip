@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
 constexpr int SC_ROWS = 8;
 constexpr int SC_T = 1024;
 constexpr int SC_PAD = 4;
-constexpr int SC_MOVERS = 256;                  // waves 1..4
+constexpr int SC_MOVERS = 512;                  // waves 1..8
 constexpr int SC_THREADS = 64 + SC_MOVERS;
 constexpr int SC_VEC = SC_ROWS * SC_T / 4;       // float4 per tile
 constexpr int SC_PER = SC_VEC / SC_MOVERS;       // float4 per mover thread
@@ -267,6 +267,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
   __shared__ float sP[STAGED ? SC_PMAX + 1 : 1];
   __shared__ float sR[STAGED ? SC_RMAX : 1];
   const int tid = threadIdx.x;
+  // the add chain is the critical path: its wave goes first when it shares
+  // a SIMD with a mover wave
+  if (tid < 64) __builtin_amdgcn_s_setprio(3);
   // blocks: (group, SC_ROWS-row slice of its n_series rows); rows are
   // g*n_series + r in z, S, pts and rms
   const int bpg = (n_series + SC_ROWS - 1) / SC_ROWS;
@@ -299,12 +302,15 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
     const float* R = STAGED ? sR + (size_t)row * nbm : rms + (size_t)(r0 + row) * nbm;
     return normalised_at(x, P, R, sh, t);
   };
-  auto load_tile = [&](size_t tile, int b) {
+  // Tile loads go into registers (nxt) one iteration before they are
+  // normalised and written to LDS, so the barrier between iterations (which
+  // waits for LDS traffic only) does not wait for HBM: the loads of tile p+2
+  // are in flight while the scan lane works through tile p.
+  float4 nxt[SC_PER];
+  auto load_raw = [&](size_t tile) {
     const int mt = tid - 64;
     const bool full = z_vec && (tile + 1) * SC_T <= count;
-    float4 v[SC_PER];
-    // all global loads first, so that they are in flight together (rows past
-    // the last series load the last series and are zeroed afterwards)
+    // rows past the last series load the last series (zeroed in commit)
     if (full) {
 #pragma unroll
       for (int q = 0; q < SC_PER; ++q) {
@@ -312,12 +318,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
         const int row = idx / (SC_T / 4);
         const int c4 = (idx % (SC_T / 4)) * 4;
         const int rr = min(r0 + row, n_series - 1);
-        v[q] = *reinterpret_cast<const float4*>(z + (size_t)rr * zpitch + tile * SC_T + c4);
-      }
-#pragma unroll
-      for (int q = 0; q < SC_PER; ++q) {
-        const int row = (mt + SC_MOVERS * q) / (SC_T / 4);
-        if (r0 + row >= n_series) v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        nxt[q] = *reinterpret_cast<const float4*>(z + (size_t)rr * zpitch + tile * SC_T + c4);
       }
     } else {
 #pragma unroll
@@ -326,37 +327,36 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
         const int row = idx / (SC_T / 4);
         const int c4 = (idx % (SC_T / 4)) * 4;
         const size_t t = tile * SC_T + c4;
-        v[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+        nxt[q] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (r0 + row < n_series) {
           const float* src = z + (size_t)(r0 + row) * zpitch + t;
-          if (t + 0 < count) v[q].x = src[0];
-          if (t + 1 < count) v[q].y = src[1];
-          if (t + 2 < count) v[q].z = src[2];
-          if (t + 3 < count) v[q].w = src[3];
+          if (t + 0 < count) nxt[q].x = src[0];
+          if (t + 1 < count) nxt[q].y = src[1];
+          if (t + 2 < count) nxt[q].z = src[2];
+          if (t + 3 < count) nxt[q].w = src[3];
         }
       }
     }
-    if (rms) {
-#pragma unroll
-      for (int q = 0; q < SC_PER; ++q) {
-        const int idx = mt + SC_MOVERS * q;
-        const int row = idx / (SC_T / 4);
-        const int c4 = (idx % (SC_T / 4)) * 4;
-        const size_t t = tile * SC_T + c4;
-        if (r0 + row < n_series) {
-          if (t + 0 < count) v[q].x = norm(row, t + 0, v[q].x);
-          if (t + 1 < count) v[q].y = norm(row, t + 1, v[q].y);
-          if (t + 2 < count) v[q].z = norm(row, t + 2, v[q].z);
-          if (t + 3 < count) v[q].w = norm(row, t + 3, v[q].w);
-        }
-      }
-    }
+  };
+  // nxt (tile `tile`) normalised into buf[b]
+  auto commit = [&](size_t tile, int b) {
+    const int mt = tid - 64;
 #pragma unroll
     for (int q = 0; q < SC_PER; ++q) {
       const int idx = mt + SC_MOVERS * q;
       const int row = idx / (SC_T / 4);
       const int c4 = (idx % (SC_T / 4)) * 4;
-      *reinterpret_cast<float4*>(&buf[b][row][c4]) = v[q];
+      const size_t t = tile * SC_T + c4;
+      float4 v = nxt[q];
+      if (r0 + row >= n_series) {
+        v = make_float4(0.f, 0.f, 0.f, 0.f);
+      } else if (rms) {
+        if (t + 0 < count) v.x = norm(row, t + 0, v.x);
+        if (t + 1 < count) v.y = norm(row, t + 1, v.y);
+        if (t + 2 < count) v.z = norm(row, t + 2, v.z);
+        if (t + 3 < count) v.w = norm(row, t + 3, v.w);
+      }
+      *reinterpret_cast<float4*>(&buf[b][row][c4]) = v;
     }
   };
   auto store_tile = [&](size_t tile, int b) {
@@ -388,7 +388,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
 
   if (tid < SC_ROWS && r0 + tid < n_series) S[(size_t)(r0 + tid) * spitch] = 0.f;
   if (STAGED) __syncthreads();
-  if (tid >= 64 && ntiles > 0) load_tile(0, 0);
+  if (tid >= 64 && ntiles > 0) {
+    load_raw(0);
+    commit(0, 0);
+    if (ntiles > 1) load_raw(1);
+  }
   __syncthreads();
   float run = 0.f;
   for (size_t p = 0; p <= ntiles; ++p) {
@@ -435,7 +439,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
       }
     } else {
       if (p >= 1) store_tile(p - 1, b ^ 1);
-      if (p + 1 < ntiles) load_tile(p + 1, b ^ 1);
+      if (p + 1 < ntiles) commit(p + 1, b ^ 1);
+      if (p + 2 < ntiles) load_raw(p + 2);
     }
     __syncthreads();
   }
