@@ -224,8 +224,8 @@ class Pipeline:
         return db.value, rm.value, l8.value, cc.value
 
     def set_dedisp_tile(self, trials_per_block, xcd_remap=1, byte_lanes=0, channels_per_step=4):
-        """Fix the dedispersion tile (8, 16 or 32 trials per block; 8-bit lanes for
-        sub-byte input; 4, 8 or 16 channels staged per step), or 0 for the autotuner."""
+        """Fix the dedispersion tile (8, 16 or 32 trials per block; byte staging;
+        4, 8 or 16 channels staged per step), or 0 for the autotuner."""
         err = lib().hd_set_dedisp_tile(self.handle, trials_per_block, xcd_remap, byte_lanes,
                                        channels_per_step)
         if err != HD_NO_ERROR:

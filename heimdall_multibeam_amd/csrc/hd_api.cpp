@@ -703,8 +703,11 @@ struct GulpIn {
 // plane buffer that the real launch then overwrites) and keeps the fastest;
 // HD_DEDISP_TILE="db,remap[,lanes8[,cc]]" fixes the choice instead.  Sets
 // dg's tile fields and returns the staging pairs.
+// Channels the byte-staged kernel sums between widenings: 8-bit lanes for
+// sub-byte samples (lanes * max sample <= 255), 16-bit lanes of the even/odd
+// split for 8-bit samples.
 int widen8_for(size_t nbits) {
-  return nbits <= 4 ? 255 / ((1 << nbits) - 1) : 0;  // channels per 8-bit lane
+  return nbits <= 4 ? 255 / ((1 << nbits) - 1) : 256;
 }
 
 int tile_index(int db, int cc) {
@@ -720,11 +723,12 @@ void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes
   dg.wlen = pl->wlen_db[k];
   dg.widen8 = lanes8 ? widen8_for(nbits) : 0;
   dg.wlen8 = lanes8 ? pl->wlen8_db[k] : 0;
+  dg.split8 = lanes8 && nbits == 8 ? 1 : 0;
 }
 
 hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
                             size_t nbits, hipStream_t s, const int** pairs) {
-  using Tile = std::array<int, 4>;  // trials per block, remap, 8-bit lanes, channels per step
+  using Tile = std::array<int, 4>;  // trials per block, remap, byte staging, channels per step
   static const Tile forced = [] {
     const char* e = getenv("HD_DEDISP_TILE");
     int db = 0, rm = 1, l8 = 0, cc = 4;
@@ -733,12 +737,11 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
       return Tile{db, rm ? 1 : 0, l8 ? 1 : 0, (cc == 8 || cc == 16) ? cc : 4};
     return Tile{0, 1, 0, 4};
   }();
-  const bool sub_byte = widen8_for(nbits) > 0;
   Tile c;
   if (pl->fixed_db) {
-    c = Tile{pl->fixed_db, pl->fixed_remap, pl->fixed_lanes8 && sub_byte, pl->fixed_cc};
+    c = Tile{pl->fixed_db, pl->fixed_remap, pl->fixed_lanes8, pl->fixed_cc};
   } else if (forced[0]) {
-    c = Tile{forced[0], forced[1], forced[2] && sub_byte, forced[3]};
+    c = forced;
   } else {
     // rows per group, live channels, sample bits, grouped or not: the number of
     // groups and the row pitch only scale the grid
@@ -751,7 +754,7 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
       }
       float best = 1e30f;
       Tile choice{16, 1, 0, 4};
-      for (int lanes = 0; lanes < (sub_byte ? 2 : 1); ++lanes)
+      for (int lanes = 0; lanes < 2; ++lanes)
         for (int cc = 4; cc <= 16; cc *= 2)
           for (int k = 0; k < 3; ++k)
             for (int r = 0; r < 2; ++r) {
@@ -791,7 +794,7 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
     }
     c = it->second;
   }
-  if (c[2] && widen8_for(nbits) < c[3]) c[2] = 0;  // 8-bit lanes would carry within a step
+  if (c[2] && widen8_for(nbits) < c[3]) c[2] = 0;  // byte lanes would carry within a step
   apply_tile(pl, dg, c[0], c[1], c[2], c[3], nbits);
   // a fixed tile whose staged copies exceed the LDS stages fewer channels per step
   while (c[3] > 4 && hd::dedisp_lds_bytes(dg) > 160 * 1024) {

@@ -142,8 +142,9 @@ struct DedispGeom {
   int wlen;            // LDS dwords per staged copy (the largest block's need)
   int db = 16;         // trials per block (8, 16 or 32; the staging pairs are planned for it)
   int remap = 1;       // XCD-aware block order
-  int widen8 = 0;      // > 0: 8-bit-lane kernel for samples <= 255/widen8 (sub-byte input)
+  int widen8 = 0;      // > 0: byte-staged kernel, widening its lanes every widen8 channels
   int wlen8 = 0;       // its LDS dwords per staged copy
+  int split8 = 0;      // byte-staged kernel for 8-bit samples (even/odd byte sums)
   int cc = 4;          // channels staged per step (4, 8 or 16; the staging pairs are planned for it)
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,

@@ -328,22 +328,33 @@ __global__ __launch_bounds__(D2_BS, d16_waves(DB, CC)) void k_dedisperse16(const
 }
 
 // ---------------------------------------------------------------------------
-// Dedispersion of sub-byte samples (values <= 15), 8-bit staging.  Same block
-// shape as k_dedisperse16, but each staged dword holds 4 samples: the
-// channel's byte window is staged as 4 copies shifted by 0..3 bytes,
-// C_k[j] = (x[4j+k], .., x[4j+k+3]), so a trial's offset o reads copy o & 3
-// at dword (o>>2) + lane + 64q, and one 32-bit add accumulates 4 samples in
-// 8-bit lanes.  The lanes never carry within `widen` channels (widen * max
-// sample <= 255); they are then added into 32-bit sums.  Half the LDS reads
-// and adds of the 16-bit kernel for the same sums.
+// Dedispersion from byte-staged samples.  Same block shape as k_dedisperse16,
+// but each staged dword holds 4 samples: the channel's byte window is staged
+// as 4 copies shifted by 0..3 bytes, C_k[j] = (x[4j+k], .., x[4j+k+3]), so a
+// trial's offset o reads copy o & 3 at dword (o>>2) + lane + 64q: half the
+// LDS reads of the 16-bit kernel for the same sums.
+//
+// SPLIT = false (sub-byte samples, values <= 15): one 32-bit add accumulates
+// 4 samples in 8-bit lanes, which never carry within `widen` channels
+// (widen * max sample <= 255); they are then added into 32-bit sums.
+//
+// SPLIT = true (8-bit samples): two sums per dword.  A = sum of the raw dwords
+// (mod 2^32) and O = sum of bytes 1 and 3 moved to 16-bit lanes, (v >> 8) &
+// 0x00ff00ff (one byte permute).  O's lanes hold at most 257 * 255, and the
+// even bytes' 16-bit-lane sum is E = A - (O << 8) mod 2^32, exact because E
+// itself fits 32 bits; so after up to 256 channels E and O give all 4
+// samples' sums.  One LDS read, one permute and one add per channel per
+// 4 samples (the adds pair up as 3-operand adds).
 constexpr int D8_Q = 4;                   // dwords (4 samples each) per lane per trial
 // (the 32-trial, 4-channel tile spills a few registers at 4 waves but runs
-// faster there than at 3)
-constexpr int d8_waves(int db, int cc) { return db == 32 && cc >= 8 ? 3 : 4; }
+// faster there than at 3; the split sums need the 16-bit kernel's budgets)
+constexpr int d8_waves(int db, int cc, bool split) {
+  return split ? d16_waves(db, cc) : (db == 32 && cc >= 8 ? 3 : 4);
+}
 constexpr int D8_PF = 4;                  // staging dwords prefetched per thread
 
-template <int DB, int CC>
-__global__ __launch_bounds__(D2_BS, d8_waves(DB, CC)) void k_dedisperse8(const uint8_t* __restrict__ x,
+template <int DB, int CC, bool SPLIT>
+__global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(const uint8_t* __restrict__ x,
                                                       const int* __restrict__ delays,
                                                       DedispGeom g, int hw_max, int widen,
                                                       float* __restrict__ out,
@@ -374,12 +385,14 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC)) void k_dedisperse8(const u
   const int* drowl = delays + (size_t)(trial0 + ml) * g.n_live;
 
   uint32_t pa[NA][D8_Q];
+  uint32_t po[NA][SPLIT ? D8_Q : 1];  // odd bytes in 16-bit lanes (SPLIT)
   uint32_t ua[NA][D8_Q][4];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
     for (int q = 0; q < D8_Q; ++q) {
       pa[a][q] = 0;
+      if (SPLIT) po[a][SPLIT ? q : 0] = 0;
 #pragma unroll
       for (int e = 0; e < 4; ++e) ua[a][q][e] = 0;
     }
@@ -453,7 +466,11 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC)) void k_dedisperse8(const u
       for (int a = 0; a < NA; ++a) {
         const uint32_t* src = lds8 + sdly[wave + D2_WAVES * a][cc] + lane;
 #pragma unroll
-        for (int q = 0; q < D8_Q; ++q) pa[a][q] += src[64 * q];
+        for (int q = 0; q < D8_Q; ++q) {
+          const uint32_t v = src[64 * q];
+          pa[a][q] += v;
+          if (SPLIT) po[a][SPLIT ? q : 0] += __builtin_amdgcn_perm(0u, v, 0x0C030C01u);
+        }
       }
     }
     pending += ncc;
@@ -462,8 +479,18 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC)) void k_dedisperse8(const u
       for (int a = 0; a < NA; ++a)
 #pragma unroll
         for (int q = 0; q < D8_Q; ++q) {
+          if (SPLIT) {
+            const uint32_t od = po[a][SPLIT ? q : 0];
+            const uint32_t ev = pa[a][q] - (od << 8);
+            ua[a][q][0] += ev & 0xFFFFu;
+            ua[a][q][1] += od & 0xFFFFu;
+            ua[a][q][2] += ev >> 16;
+            ua[a][q][3] += od >> 16;
+            po[a][SPLIT ? q : 0] = 0;
+          } else {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) ua[a][q][e] += (pa[a][q] >> (8 * e)) & 0xFFu;
+            for (int e = 0; e < 4; ++e) ua[a][q][e] += (pa[a][q] >> (8 * e)) & 0xFFu;
+          }
           pa[a][q] = 0;
         }
       pending = 0;
@@ -526,7 +553,7 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
 // Raises a kernel's dynamic-LDS limit once (never during a later capture:
 // every shape is launched eagerly before its chain is captured).
 hipError_t lds_limit(const void* fn, int slot, size_t lds) {
-  static size_t set[18] = {0};
+  static size_t set[27] = {0};
   if (lds <= 64 * 1024 || lds <= set[slot]) return hipSuccess;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess) set[slot] = lds;
@@ -541,9 +568,9 @@ template <int DB, int CC>
 const void* fn16() {
   return reinterpret_cast<const void*>(&k_dedisperse16<DB, CC>);
 }
-template <int DB, int CC>
+template <int DB, int CC, bool SPLIT>
 const void* fn8() {
-  return reinterpret_cast<const void*>(&k_dedisperse8<DB, CC>);
+  return reinterpret_cast<const void*>(&k_dedisperse8<DB, CC, SPLIT>);
 }
 const void* kernel16(int slot) {
   static const void* const t[9] = {fn16<8, 4>(),  fn16<16, 4>(),  fn16<32, 4>(),
@@ -551,11 +578,16 @@ const void* kernel16(int slot) {
                                    fn16<8, 16>(), fn16<16, 16>(), fn16<32, 16>()};
   return t[slot];
 }
-const void* kernel8(int slot) {
-  static const void* const t[9] = {fn8<8, 4>(),  fn8<16, 4>(),  fn8<32, 4>(),
-                                   fn8<8, 8>(),  fn8<16, 8>(),  fn8<32, 8>(),
-                                   fn8<8, 16>(), fn8<16, 16>(), fn8<32, 16>()};
-  return t[slot];
+const void* kernel8(int slot, bool split) {
+  static const void* const t[9] = {
+      fn8<8, 4, false>(),  fn8<16, 4, false>(),  fn8<32, 4, false>(),
+      fn8<8, 8, false>(),  fn8<16, 8, false>(),  fn8<32, 8, false>(),
+      fn8<8, 16, false>(), fn8<16, 16, false>(), fn8<32, 16, false>()};
+  static const void* const ts[9] = {
+      fn8<8, 4, true>(),  fn8<16, 4, true>(),  fn8<32, 4, true>(),
+      fn8<8, 8, true>(),  fn8<16, 8, true>(),  fn8<32, 8, true>(),
+      fn8<8, 16, true>(), fn8<16, 16, true>(), fn8<32, 16, true>()};
+  return split ? ts[slot] : t[slot];
 }
 
 // LDS of a tile: the staged copies (dynamic) plus the trial offsets (static).
@@ -579,9 +611,10 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
   if (dedisp_lds_bytes(g) > 160 * 1024) return hipErrorInvalidValue;
   if (g.widen8 > 0) {
     if (g.widen8 < cc || g.wlen8 <= 0) return hipErrorInvalidValue;
+    if (g.split8 && g.widen8 > 256) return hipErrorInvalidValue;   // 16-bit lanes would carry
     const size_t lds8 = (size_t)cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t);
-    const void* f8 = kernel8(slot);
-    const hipError_t e = lds_limit(f8, 9 + slot, lds8);
+    const void* f8 = kernel8(slot, g.split8 != 0);
+    const hipError_t e = lds_limit(f8, (g.split8 ? 18 : 9) + slot, lds8);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(reinterpret_cast<void (*)(const uint8_t*, const int*, DedispGeom, int, int,
                                                  float*, const GroupDev*)>(

@@ -76,17 +76,17 @@ hd_size hd_get_dm_list(hd_pipeline pl, hd_float* out, hd_size cap);
 hd_size hd_get_last_timings(hd_pipeline pl, double* ms, hd_size cap);
 
 /* The dedispersion tile of the last call: trials per block (8, 16 or 32),
- * whether blocks were ordered for the 8 XCDs' L2 caches, whether samples
- * were summed in 8-bit lanes (sub-byte input) rather than 16-bit pairs, and
- * the channels staged in LDS per step (4, 8 or 16).  The first call of a shape
- * times every candidate on the real input and keeps the fastest (all give
- * the same sums); HD_DEDISP_TILE="db,remap,lanes8,cc" in the environment
- * fixes it for a process.  Any output pointer may be NULL. */
+ * whether blocks were ordered for the 8 XCDs' L2 caches, whether samples were
+ * staged as bytes (4 per LDS word: summed in 8-bit lanes for 1-, 2- and 4-bit
+ * input, as even/odd 16-bit-lane sums for 8-bit input) rather than as 16-bit
+ * pairs, and the channels staged in LDS per step (4, 8 or 16).  The first
+ * call of a shape times every candidate on the real input and keeps the
+ * fastest (all give the same sums); HD_DEDISP_TILE="db,remap,lanes8,cc" in
+ * the environment fixes it for a process.  Any output pointer may be NULL. */
 void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap, int* byte_lanes,
                         int* channels_per_step);
-/* Fixes the dedispersion tile (trials_per_block 8, 16 or 32; byte_lanes
- * applies to 1-, 2- and 4-bit input only; channels_per_step 4, 8 or 16), or
- * returns to the autotuner (trials_per_block 0). */
+/* Fixes the dedispersion tile (trials_per_block 8, 16 or 32; channels_per_step
+ * 4, 8 or 16), or returns to the autotuner (trials_per_block 0). */
 hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap, int byte_lanes,
                             int channels_per_step);
 

@@ -146,15 +146,21 @@ def test_every_dedisp_tile_gives_the_same_search(oracle, db):
         _compare(gpu, r, oracle, w)
 
 
-@pytest.mark.parametrize("nbits", [1, 2, 4])
-def test_byte_lane_dedispersion_matches_oracle(oracle, nbits):
-    """The 8-bit-lane kernel for sub-byte input, fixed for every tile height,
-    matches the oracle bit for bit (and so the 16-bit kernel)."""
+@pytest.mark.parametrize("nbits,level", [(1, "mid"), (2, "mid"), (4, "mid"), (8, "mid"),
+                                         (8, "saturated")])
+def test_byte_lane_dedispersion_matches_oracle(oracle, nbits, level):
+    """The byte-staged kernel (8-bit lanes for sub-byte input, even/odd
+    16-bit-lane sums for 8-bit input), fixed for every tile height, matches
+    the oracle bit for bit (and so the 16-bit kernel).  "saturated" puts most
+    8-bit samples at 255, the largest sums the lanes must hold."""
     import dataclasses
     import heimdall_multibeam_amd as H
     from heimdall_multibeam_amd import synth
     vmax = (1 << nbits) - 1
-    w = dataclasses.replace(SMALL_128, nbits=nbits, mean=vmax / 2.0, sigma=max(vmax / 3.0, 0.5))
+    mean, sigma = vmax / 2.0, max(vmax / 3.0, 0.5)
+    if level == "saturated":
+        mean, sigma = 250.0, 12.0
+    w = dataclasses.replace(SMALL_128, nbits=nbits, mean=mean, sigma=sigma)
     fb = synth.make_gulp(w, 40 + nbits)
     p = oracle_params_for(oracle, w)
     r = oracle.execute(p, fb, w.nsamps, nbits, 0, 1)
