@@ -335,19 +335,19 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const int lane = (int)(tid & 63), wave = (int)(tid >> 6);
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;  // lanes under this one
   const unsigned tmax = tlen - 1;
-  for (int j = 0; j < wt.nw; ++j) {
+  // Outputs p = q*FB_BS + tid (q < TH_PER) of width j in this chunk: their
+  // values, and the bits of those kept and above the threshold.  Returns
+  // false when the chunk holds no output of the width (outputs i0 .. i1-1).
+  auto eval_width = [&](int j, float (&vals)[TH_PER], unsigned& mask, size_t& i0) -> bool {
     const WidthInfo wi = wt.wi[j];
     const float inv = winv_all[row * wt.nw + j];
     // outputs i with i*ts in [p0, p0 + TH_CHUNK)
-    const size_t i0 = (p0 + wi.ts - 1) / wi.ts;
+    i0 = (p0 + wi.ts - 1) / wi.ts;
     size_t i1 = (p0 + TH_CHUNK + wi.ts - 1) / wi.ts;
     const size_t n_w = nwv[j];
     if (i1 > n_w) i1 = n_w;
-    const size_t key = (row * wt.nw + j) * gb.nchunks_per_w + c;
-    if (i0 >= i1) {
-      if (tid == 0) gb.ccount[key] = 0;
-      continue;
-    }
+    mask = 0;
+    if (i0 >= i1) return false;
     const unsigned nout = (unsigned)(i1 - i0);
     // find_giants.cu:55-58 keeps i with (i mod stride) < bc692.  When the
     // chunk's outputs all lie in one beam period that is a single bound.
@@ -356,8 +356,6 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
                             i1 - 1 <= 0xffffffffull;
     const size_t lim = beam0 * stride + bc692;
     const unsigned rbase = (unsigned)(i0 * wi.ts - p0 + bmax);   // T index of output i0
-    float vals[TH_PER];
-    unsigned mask = 0;
     if (one_period) {
       // kept: outputs p < pk (i < min(i1, lim)); the others are evaluated
       // from in-tile reads and dropped
@@ -398,6 +396,34 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
           if (in_beam && v > thresh) mask |= 1u << q;
         }
       }
+    }
+    return true;
+  };
+  // Screening pass without barriers: most chunks hold no output above the
+  // threshold at any width, and then every key of the chunk is empty.
+  {
+    bool hit = false;
+    for (int j = 0; j < wt.nw; ++j) {
+      float vals[TH_PER];
+      unsigned mask;
+      size_t i0;
+      eval_width(j, vals, mask, i0);
+      hit |= mask != 0;
+    }
+    if (!__syncthreads_or(hit)) {
+      for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
+        gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
+      return;
+    }
+  }
+  for (int j = 0; j < wt.nw; ++j) {
+    const size_t key = (row * wt.nw + j) * gb.nchunks_per_w + c;
+    float vals[TH_PER];
+    unsigned mask;
+    size_t i0;
+    if (!eval_width(j, vals, mask, i0)) {
+      if (tid == 0) gb.ccount[key] = 0;
+      continue;
     }
     const int any = __syncthreads_or(mask != 0);
     if (!any) {
