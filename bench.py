@@ -23,6 +23,18 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 VALU_INT_PEAK = 256 * 64 * 2.4e9  # int32 lane adds/s (256 CUs x 64 lanes x 2.4 GHz)
+# LDS read rate of the dedispersion inner loop's instruction (ds_read2st64_b32),
+# measured by heimdall_multibeam_amd/bin/probe_lds: 98.9 B/clk/CU at 2.4 GHz
+LDS_READ2_PEAK = 98.9 * 256 * 2.4e9  # bytes/s
+
+
+def lds_roof(adds, t, tile):
+    """LDS-read view of the dedispersion kernel: every sample-add reads one
+    16-bit lane (2 bytes) of staged pairs, or one byte with byte staging."""
+    per = 1 if tile[2] else 2
+    b = adds * per
+    return {"lds_read_bytes": b, "lds_read_peak_tbs": round(LDS_READ2_PEAK / 1e12, 2),
+            "lds_read_frac": round(b / t / LDS_READ2_PEAK, 4)}
 
 
 def tile_json(t):
@@ -168,7 +180,8 @@ def batch_main(a):
                      "frac": round(alg / t_dd / 1e9 / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": "k_dedisperse", "kernel_ms": round(t_dd * 1e3, 4),
                      "algorithmic_bytes": alg, "int_adds_per_s": adds / t_dd,
-                     "valu_int32_frac": round(adds / t_dd / VALU_INT_PEAK, 4)},
+                     "valu_int32_frac": round(adds / t_dd / VALU_INT_PEAK, 4),
+                     **lds_roof(adds, t_dd, pl.dedisp_tile())},
         "cpu_baseline": None,
     }
     if not a.no_cpu_baseline:
@@ -417,7 +430,8 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "kernel": "k_dedisperse", "kernel_ms": round(t_dd * 1e3, 4),
             "algorithmic_bytes": alg,
-            "int_adds_per_s": adds / t_dd, "valu_int32_frac": round(adds / t_dd / VALU_INT_PEAK, 4)}
+            "int_adds_per_s": adds / t_dd, "valu_int32_frac": round(adds / t_dd / VALU_INT_PEAK, 4),
+            **lds_roof(adds, t_dd, pl.dedisp_tile())}
     stages_ms = {k: round(float(np.mean(v)), 4) for k, v in stage.items() if v}
     out = {
         "metric": "beam-seconds searched/s/GPU at DM 0-2000",
