@@ -108,6 +108,7 @@ struct hd_pipeline_t {
   DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm, g_best;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
   DevBuf c_sind, c_sfd;
+  DevBuf c_ecount, c_eoff, c_adj;  // coincidence lists
   DevBuf o_peak, o_ind, o_begin, o_end, o_filter, o_dm, o_members;
   DevBuf clean_out, clean_work;
   DevBuf gdev, g_group, o_group;        // group table, group of each giant / candidate
@@ -1412,15 +1413,24 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   HD_CHECK(pl->c_members.ensure(ng * sizeof(unsigned)));
   HD_CHECK(pl->c_rflag.ensure(ng * sizeof(unsigned)));
   HD_CHECK(pl->c_rpos.ensure(ng * sizeof(unsigned)));
-  HD_CHECK(pl->c_tmp.ensure(hd::exclusive_scan_tmp_words(ng) * sizeof(unsigned)));
+  HD_CHECK(pl->c_tmp.ensure(hd::exclusive_scan_tmp_words(ng + 1) * sizeof(unsigned)));
   HD_CHECK(pl->c_sind.ensure(ng * sizeof(uint64_t)));
   HD_CHECK(pl->c_sfd.ensure(ng * sizeof(uint32_t)));
+  HD_CHECK(pl->c_ecount.ensure((ng + 1) * sizeof(uint32_t)));
+  HD_CHECK(pl->c_eoff.ensure((ng + 1) * sizeof(uint32_t)));
   hd::ClusterBuffers cb{pl->c_keys.as<uint64_t>(), pl->c_lo.as<uint32_t>(), pl->c_hi.as<uint32_t>(),
                         pl->c_labels.as<int32_t>(), pl->c_flag.as<unsigned>(),
                         pl->c_best.as<uint64_t>(), pl->c_members.as<unsigned>(),
                         pl->c_rflag.as<unsigned>(), pl->c_rpos.as<unsigned>(),
                         pl->c_tmp.as<unsigned>(), npad, pl->c_sind.as<uint64_t>(),
                         pl->c_sfd.as<uint32_t>(), nbands};
+  cb.ecount = pl->c_ecount.as<uint32_t>();
+  cb.eoff = pl->c_eoff.as<uint32_t>();
+  cb.adj_alloc = [pl](size_t entries) -> uint32_t* {
+    if (pl->c_adj.ensure(std::max<size_t>(entries, 1) * sizeof(uint32_t)) != hipSuccess)
+      return nullptr;
+    return pl->c_adj.as<uint32_t>();
+  };
   const unsigned max_filter = wt.wi[wt.nw - 1].filter_idx;
   int iters = 0;
   HD_CHECK(hd::launch_cluster(pl->g_ind.as<uint64_t>(), pl->g_filter.as<uint32_t>(),
@@ -1704,7 +1714,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->c_members, &pl->c_rflag, &pl->c_rpos, &pl->c_tmp, &pl->o_peak,
                       &pl->o_ind, &pl->o_begin, &pl->o_end, &pl->o_filter, &pl->o_dm,
                       &pl->o_members, &pl->gdev, &pl->g_group, &pl->o_group, &pl->c_sind,
-                      &pl->c_sfd, &pl->rdev, &pl->co_delays, &pl->co_pairs, &pl->co_xc,
+                      &pl->c_sfd, &pl->c_ecount, &pl->c_eoff, &pl->c_adj, &pl->rdev, &pl->co_delays, &pl->co_pairs, &pl->co_xc,
                       &pl->co_plane, &pl->co_pts, &pl->co_rms, &pl->co_brms, &pl->co_bf, &pl->co_bw,
                       &pl->co_gdev, &pl->co_tiles, &pl->co_scores, &pl->co_toff, &pl->co_ntpos,
                       &pl->co_ncols, &pl->co_flags, &pl->co_pos};

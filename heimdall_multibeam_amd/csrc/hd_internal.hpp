@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
 namespace hd {
@@ -249,6 +250,13 @@ struct ClusterBuffers {
   uint64_t* s_ind = nullptr;
   uint32_t* s_fd = nullptr;    // dm << 5 | filter
   unsigned band_cap = 0;       // lo/hi entries per giant (filter-banded windows)
+  // coincidence lists (optional, banded windows only): for each giant the
+  // earlier giants it coincides with, built once so that the relaxation
+  // passes read labels only.  ecount and eoff hold n + 1 entries; adj_alloc
+  // returns room for the given number of entries (null: windows instead).
+  uint32_t* ecount = nullptr;
+  uint32_t* eoff = nullptr;
+  std::function<uint32_t*(size_t)> adj_alloc;
 };
 // With group (per giant) and gd, giants of different groups never coincide
 // and each group uses its own beam length; otherwise nsamps_beam applies.

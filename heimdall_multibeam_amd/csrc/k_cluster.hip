@@ -317,6 +317,79 @@ __global__ __launch_bounds__(256) void k_relax_banded_wave(
   }
 }
 
+// Coincidence lists over the banded windows, one wave per giant i: MODE 0
+// counts the earlier giants j < i coincident with i into ecount[i]; MODE 1
+// writes them to adj[eoff[i] ..] in window order.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_edges_banded_wave(
+    const uint64_t* __restrict__ ind, const uint32_t* __restrict__ filter,
+    const uint32_t* __restrict__ dm, unsigned n, unsigned nb, const uint64_t* __restrict__ keys,
+    const uint64_t* __restrict__ s_ind, const uint32_t* __restrict__ s_fd,
+    const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, uint64_t time_tol,
+    uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam_one,
+    const uint32_t* __restrict__ group, const GroupDev* __restrict__ gd,
+    uint32_t* __restrict__ ecount, const uint32_t* __restrict__ eoff, uint32_t* __restrict__ adj) {
+  const unsigned i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const unsigned lane = threadIdx.x & 63;
+  if (i >= n) return;  // whole wave
+  const uint64_t nsamps_beam = group ? gd[group[i]].nsamps_beam : nsamps_beam_one;
+  const uint64_t ii = ind[i];
+  const uint32_t fi = filter[i], di = dm[i];
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const unsigned base = MODE == 1 ? eoff[i] : 0u;
+  unsigned count = 0;
+  for (unsigned b = 0; b < nb; ++b) {
+    const unsigned p0 = lo[(size_t)i * nb + b], p1 = hi[(size_t)i * nb + b];
+    for (unsigned pb = p0; pb < p1; pb += 64) {  // the same bounds in every lane
+      const unsigned p = pb + lane;
+      bool hit = false;
+      unsigned j = 0;
+      if (p < p1) {
+        j = (unsigned)(keys[p] & 0xffffffffull);
+        if (j < i) {
+          const uint32_t fd = s_fd[p];
+          hit = coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol,
+                           dm_tol, nsamps_beam);
+        }
+      }
+      const uint64_t m = __ballot(hit);
+      if (MODE == 1 && hit) adj[base + count + (unsigned)__popcll(m & below)] = j;
+      count += (unsigned)__popcll(m);
+    }
+  }
+  if (MODE == 0 && lane == 0) ecount[i] = count;
+}
+
+// The relaxation over the coincidence lists, one wave per giant.
+__global__ __launch_bounds__(256) void k_relax_lists_wave(unsigned n,
+                                                          const uint32_t* __restrict__ eoff,
+                                                          const uint32_t* __restrict__ adj,
+                                                          int32_t* labels, unsigned* flag) {
+  const unsigned i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const unsigned lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int32_t cur = __atomic_load_n(&labels[i], __ATOMIC_RELAXED);
+  int32_t best = cur;
+  if (lane == 0) {
+    const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
+    if (jump < best) best = jump;
+  }
+  const unsigned p1 = eoff[i + 1];
+  for (unsigned p = eoff[i] + lane; p < p1; p += 64) {
+    const int32_t lj = __atomic_load_n(&labels[adj[p]], __ATOMIC_RELAXED);
+    if (lj < best) best = lj;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t other = __shfl_xor(best, o);
+    best = other < best ? other : best;
+  }
+  if (lane == 0 && best < cur) {
+    __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
+    *flag = 1u;
+  }
+}
+
 __global__ void k_clear_flag(unsigned* flag) { *flag = 0u; }
 
 __global__ void k_merge_init(unsigned n, const int32_t* __restrict__ labels, uint64_t* best,
@@ -410,12 +483,40 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
   if (sorted)
     hipLaunchKernelGGL(k_gather_sorted, dim3((n + T - 1) / T), dim3(T), 0, s, cb.sort_keys, n,
                        ind, filter, dm, cb.s_ind, cb.s_fd);
+  // coincidence lists: counted, offsets scanned, sized on the host, written
+  // (at most kMaxEdges entries; beyond that the passes scan the windows)
+  constexpr size_t kMaxEdges = size_t(1) << 26;
+  uint32_t* adj = nullptr;
+  if (banded && cb.ecount && cb.eoff && cb.adj_alloc) {
+    hipLaunchKernelGGL(k_edges_banded_wave<0>, dim3((n + 3) / 4), dim3(256), 0, s, ind, filter,
+                       dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi, time_tol,
+                       filter_tol, dm_tol, nsamps_beam, group, gd, cb.ecount, cb.eoff,
+                       (uint32_t*)nullptr);
+    e = launch_fill_u32(cb.ecount + n, 1, 0u, s);
+    if (e == hipSuccess) e = launch_exclusive_scan_u32(cb.ecount, cb.eoff, (size_t)n + 1,
+                                                       cb.scan_tmp, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_flag_pinned, cb.eoff + n, sizeof(unsigned), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const size_t edges = *h_flag_pinned;
+    if (edges <= kMaxEdges) adj = cb.adj_alloc(edges);
+    if (!adj) (void)hipGetLastError();  // a failed allocation leaves the windows path
+    if (adj)
+      hipLaunchKernelGGL(k_edges_banded_wave<1>, dim3((n + 3) / 4), dim3(256), 0, s, ind,
+                         filter, dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi,
+                         time_tol, filter_tol, dm_tol, nsamps_beam, group, gd, cb.ecount,
+                         cb.eoff, adj);
+  }
   // relaxation passes run in batches between host checks of the change flag
   constexpr int kBatch = 4;
   for (int it = 0; it < 100000; it += kBatch) {
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, cb.flag);
     for (int b = 0; b < kBatch; ++b) {
-      if (banded)
+      if (adj)
+        hipLaunchKernelGGL(k_relax_lists_wave, dim3((n + 3) / 4), dim3(256), 0, s, n, cb.eoff,
+                           adj, cb.labels, cb.flag);
+      else if (banded)
         hipLaunchKernelGGL(k_relax_banded_wave, dim3((n + 3) / 4), dim3(256), 0, s, ind, filter,
                            dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi, time_tol,
                            filter_tol, dm_tol, nsamps_beam, group, gd, cb.labels, cb.flag);
