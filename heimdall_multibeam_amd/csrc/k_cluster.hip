@@ -360,31 +360,33 @@ __global__ __launch_bounds__(256) void k_edges_banded_wave(
   if (MODE == 0 && lane == 0) ecount[i] = count;
 }
 
-// The relaxation over the coincidence lists, one wave per giant.
-__global__ __launch_bounds__(256) void k_relax_lists_wave(unsigned n,
-                                                          const uint32_t* __restrict__ eoff,
-                                                          const uint32_t* __restrict__ adj,
-                                                          int32_t* labels, unsigned* flag) {
-  const unsigned i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const unsigned lane = threadIdx.x & 63;
-  if (i >= n) return;
-  const int32_t cur = __atomic_load_n(&labels[i], __ATOMIC_RELAXED);
+// The relaxation over the coincidence lists, 8 lanes per giant.
+constexpr int kListLanes = 8;
+__global__ __launch_bounds__(256) void k_relax_lists(unsigned n,
+                                                     const uint32_t* __restrict__ eoff,
+                                                     const uint32_t* __restrict__ adj,
+                                                     int32_t* labels, unsigned* flag) {
+  const unsigned i = (blockIdx.x * 256 + threadIdx.x) / kListLanes;
+  const unsigned sub = threadIdx.x % kListLanes;
+  // giants past n still take part in the shuffles (with nothing to read)
+  const bool live = i < n;
+  const int32_t cur = live ? __atomic_load_n(&labels[i], __ATOMIC_RELAXED) : 0x7fffffff;
   int32_t best = cur;
-  if (lane == 0) {
+  if (live && sub == 0) {
     const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
     if (jump < best) best = jump;
   }
-  const unsigned p1 = eoff[i + 1];
-  for (unsigned p = eoff[i] + lane; p < p1; p += 64) {
+  const unsigned p1 = live ? eoff[i + 1] : 0u;
+  for (unsigned p = (live ? eoff[i] : 0u) + sub; p < p1; p += kListLanes) {
     const int32_t lj = __atomic_load_n(&labels[adj[p]], __ATOMIC_RELAXED);
     if (lj < best) best = lj;
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
+  for (int o = kListLanes / 2; o > 0; o >>= 1) {
     const int32_t other = __shfl_xor(best, o);
     best = other < best ? other : best;
   }
-  if (lane == 0 && best < cur) {
+  if (live && sub == 0 && best < cur) {
     __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
     *flag = 1u;
   }
@@ -509,13 +511,14 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
                          cb.eoff, adj);
   }
   // relaxation passes run in batches between host checks of the change flag
-  constexpr int kBatch = 4;
+  // (list passes are cheap next to a check, window passes are not)
+  const int kBatch = adj ? 2 : 4;
   for (int it = 0; it < 100000; it += kBatch) {
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, cb.flag);
     for (int b = 0; b < kBatch; ++b) {
       if (adj)
-        hipLaunchKernelGGL(k_relax_lists_wave, dim3((n + 3) / 4), dim3(256), 0, s, n, cb.eoff,
-                           adj, cb.labels, cb.flag);
+        hipLaunchKernelGGL(k_relax_lists, dim3((n + 256 / kListLanes - 1) / (256 / kListLanes)),
+                           dim3(256), 0, s, n, cb.eoff, adj, cb.labels, cb.flag);
       else if (banded)
         hipLaunchKernelGGL(k_relax_banded_wave, dim3((n + 3) / 4), dim3(256), 0, s, ind, filter,
                            dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi, time_tol,
