@@ -399,17 +399,53 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     }
     return true;
   };
+  // Whether any kept output of width j in this thread's positions exceeds the
+  // threshold.  For undecimated widths in one beam period it tests only the
+  // largest difference: rounding is monotonic, so for inv > 0
+  // max_q RN(d_q * inv) = RN(max_q d_q * inv), and the test gives exactly
+  // the answer of testing every output (NaN differences never exceed it in
+  // either form); other widths evaluate every output.
+  auto screen_width = [&](int j) -> bool {
+    const WidthInfo wi = wt.wi[j];
+    const size_t i0 = (p0 + wi.ts - 1) / wi.ts;
+    size_t i1 = (p0 + TH_CHUNK + wi.ts - 1) / wi.ts;
+    if (i1 > nwv[j]) i1 = nwv[j];
+    if (i0 >= i1) return false;
+    const size_t beam0 = (size_t)(unsigned)i0 / stride;
+    const bool one_period = (size_t)(unsigned)(i1 - 1) / stride == beam0 &&
+                            i1 - 1 <= 0xffffffffull;
+    if (wi.ts != 1 || !one_period) {
+      float vals[TH_PER];
+      unsigned mask;
+      size_t i0x;
+      eval_width(j, vals, mask, i0x);
+      return mask != 0;
+    }
+    const float inv = winv_all[row * wt.nw + j];
+    const size_t lim = beam0 * stride + bc692;
+    const size_t kend = i1 < lim ? i1 : lim;
+    const unsigned pk = kend > i0 ? (unsigned)(kend - i0) : 0u;
+    const unsigned rbase = (unsigned)(i0 - p0 + bmax);
+    const float* Ta = T + rbase + wi.ahead + tid;
+    const float* Tb = T + rbase - wi.behind + tid;
+    float mx = -INFINITY;
+    if (pk >= (unsigned)TH_CHUNK) {
+#pragma unroll
+      for (int q = 0; q < TH_PER; ++q) mx = fmaxf(mx, Ta[q * FB_BS] - Tb[q * FB_BS]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < TH_PER; ++q) {
+        const float d = Ta[q * FB_BS] - Tb[q * FB_BS];
+        if ((unsigned)q * FB_BS + tid < pk) mx = fmaxf(mx, d);
+      }
+    }
+    return mx * inv > thresh;
+  };
   // Screening pass without barriers: most chunks hold no output above the
   // threshold at any width, and then every key of the chunk is empty.
   {
     bool hit = false;
-    for (int j = 0; j < wt.nw; ++j) {
-      float vals[TH_PER];
-      unsigned mask;
-      size_t i0;
-      eval_width(j, vals, mask, i0);
-      hit |= mask != 0;
-    }
+    for (int j = 0; j < wt.nw; ++j) hit |= screen_width(j);
     if (!__syncthreads_or(hit)) {
       for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
         gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
