@@ -172,17 +172,59 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     }
     return lds_pyramid<FB_BS>(A, B, (int)n1, bs.L0 - 1)[0];
   };
+  if (use_tile) {
+    // One wave per width (widths first_w + w, + 4, ...): the chunk's median
+    // pyramid runs inside the wave, in its own LDS buffers, with no
+    // block-wide barrier; the groups of 5 are those of the reference.
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* WA = A + (size_t)wave * (n1 + n1 / 5 + 2);
+    float* WB = WA + (n1 ? n1 : 1);
+    auto wave_sync = [] {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    for (int g = wave; g < n_group; g += FB_BS / 64) {
+      const int j = first_w + g;
+      const WidthInfo wi = wt.wi[j];
+      const unsigned oa = bmax + wi.ahead, ob = bmax - wi.behind;
+      auto leaf = [&](unsigned r) -> float { return fabsf(T[r + oa] - T[r + ob]); };
+      float value;
+      if (bs.L0 == 0) {
+        value = leaf(0);
+      } else {
+        for (size_t q = lane; q < n1; q += 64) {
+          const unsigned r = 5 * (unsigned)q;  // pos - pos0 of the group's first leaf
+          WA[q] = med5(leaf(r), leaf(r + 1), leaf(r + 2), leaf(r + 3), leaf(r + 4));
+        }
+        float* src = WA;
+        float* dst = WB;
+        size_t n = n1;
+        for (int l = 1; l < bs.L0; ++l) {
+          wave_sync();
+          const size_t m = n / 5;
+          for (size_t q = lane; q < m; q += 64) {
+            const float* p = src + 5 * q;
+            dst[q] = med5(p[0], p[1], p[2], p[3], p[4]);
+          }
+          float* t = src;
+          src = dst;
+          dst = t;
+          n = m;
+        }
+        wave_sync();
+        value = src[0];
+      }
+      if (lane == 0) partial[(row * wt.nw + j) * np_max + c] = value;
+      wave_sync();  // the buffers are rewritten for the wave's next width
+    }
+    return;
+  }
   for (int g = 0; g < n_group; ++g) {
     const int j = first_w + g;
     const WidthInfo wi = wt.wi[j];
     float value;
-    if (use_tile) {
-      const unsigned oa = bmax + wi.ahead, ob = bmax - wi.behind;
-      value = chunk_value([&](size_t k) -> float {
-        const unsigned r = (unsigned)(k - k0);  // pos - pos0
-        return fabsf(T[r + oa] - T[r + ob]);
-      });
-    } else if (src_j == j) {
+    if (src_j == j) {
       value = chunk_value([&](size_t k) -> float {
         const unsigned r = (unsigned)(k - k0);
         return fabsf(T[r + wts] - T[r]);
@@ -640,7 +682,8 @@ hipError_t launch_block_rms_partials(const float* S, size_t spitch, const WidthT
   }
   const size_t tlen = chunk + amax + bmax;
   const size_t n1 = chunk / 5;
-  const size_t rest = (n1 ? n1 : 1) + n1 / 5 + 8;
+  // level buffers: block-wide, or one pair per wave (the LDS-tile path)
+  const size_t rest = std::max((n1 ? n1 : 1) + n1 / 5 + 8, (size_t)(FB_BS / 64) * (n1 + n1 / 5 + 2));
   constexpr size_t kMaxLds = 65536 / sizeof(float);
   if (tlen + rest > kMaxLds) tile_ok = 0;  // keep LDS <= 64 KiB
   // single decimated width: room for the widest range it or an earlier width reads
