@@ -346,12 +346,11 @@ __global__ __launch_bounds__(D2_BS, d16_waves(DB, CC)) void k_dedisperse16(const
 // samples' sums.  One LDS read, one permute and one add per channel per
 // 4 samples (the adds pair up as 3-operand adds).
 constexpr int D8_Q = 4;                   // dwords (4 samples each) per lane per trial
-// (the 32-trial, 4-channel tile spills a few registers at 4 waves but runs
-// faster there than at 3; the split sums need the 16-bit kernel's budgets)
+// (32-trial tiles at 3 waves; the split sums at 32 trials need 2)
 constexpr int d8_waves(int db, int cc, bool split) {
-  return split ? d16_waves(db, cc) : (db == 32 && cc >= 8 ? 3 : 4);
+  return db == 32 ? (split ? 2 : 3) : (split && db == 16 && cc >= 8 ? 3 : 4);
 }
-constexpr int D8_PF = 4;                  // staging dwords prefetched per thread
+constexpr int D8_PF = 2;                  // staging items (4 dwords) prefetched per thread
 
 template <int DB, int CC, bool SPLIT>
 __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(const uint8_t* __restrict__ x,
@@ -407,29 +406,36 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(
     const int span = drowl[c] - (drow0[c] & ~3);
     return (D2_TB + span + 3) / 4 + 1;
   };
-  uint32_t pf_lo[D8_PF], pf_hi[D8_PF];
+  // Staging item u of a channel: source dwords 4u .. 4u+4 give dwords
+  // 4u .. 4u+3 of each of the 4 copies, one 16-byte LDS write per copy (the
+  // copies' rows are 16-byte aligned: hw_max is a multiple of 64).  The last
+  // item may read up to 7 dwords past the window, inside the row's padding.
+  auto ngrp = [&](int c) { return (ndw(c) + 3) / 4; };
+  uint32_t pf[D8_PF][5];
   auto prefetch = [&](int k) {
     const int c = k * CC + scc;
     if (k >= nchunk || c >= g.n_live) return;
-    const int nd = ndw(c);
+    const int ng = ngrp(c);
     const uint32_t* src = reinterpret_cast<const uint32_t*>(
         x + (size_t)c * g.xpitch + t0 + (drow0[c] & ~3));
 #pragma unroll
     for (int r = 0; r < D8_PF; ++r) {
-      const int j = sjj + r * TPC;
-      if (j < nd) {
-        pf_lo[r] = src[j];
-        pf_hi[r] = src[j + 1];
+      const int u = sjj + r * TPC;
+      if (u < ng) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) pf[r][i] = src[4 * u + i];
       }
     }
   };
   // copy k, dword j: bytes 4j+k .. 4j+k+3 of the window
-  auto put = [&](int j, uint32_t lo, uint32_t hi) {
-    uint32_t* C = lds8 + (size_t)(scc * 4) * hw_max;
-    C[j] = lo;
-    C[hw_max + j] = __builtin_amdgcn_alignbyte(hi, lo, 1);
-    C[2 * hw_max + j] = __builtin_amdgcn_alignbyte(hi, lo, 2);
-    C[3 * hw_max + j] = __builtin_amdgcn_alignbyte(hi, lo, 3);
+  auto put4 = [&](int u, const uint32_t (&d)[5]) {
+    uint32_t* C = lds8 + (size_t)(scc * 4) * hw_max + 4 * u;
+    *reinterpret_cast<uint4*>(C) = make_uint4(d[0], d[1], d[2], d[3]);
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+      *reinterpret_cast<uint4*>(C + k * hw_max) = make_uint4(
+          __builtin_amdgcn_alignbyte(d[1], d[0], k), __builtin_amdgcn_alignbyte(d[2], d[1], k),
+          __builtin_amdgcn_alignbyte(d[3], d[2], k), __builtin_amdgcn_alignbyte(d[4], d[3], k));
   };
   prefetch(0);
   int pf_delay = 0;
@@ -448,30 +454,55 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(
     __syncthreads();
     if (tid < DB * CC) sdly[tid / CC][tid % CC] = pf_delay;
     if (scc < ncc) {
-      const int nd = ndw(c0 + scc);
+      const int ng = ngrp(c0 + scc);
 #pragma unroll
       for (int r = 0; r < D8_PF; ++r) {
-        const int j = sjj + r * TPC;
-        if (j < nd) put(j, pf_lo[r], pf_hi[r]);
+        const int u = sjj + r * TPC;
+        if (u < ng) put4(u, pf[r]);
       }
       const uint32_t* src = reinterpret_cast<const uint32_t*>(
           x + (size_t)(c0 + scc) * g.xpitch + t0 + (drow0[c0 + scc] & ~3));
-      for (int j = sjj + D8_PF * TPC; j < nd; j += TPC) put(j, src[j], src[j + 1]);
+      for (int u = sjj + D8_PF * TPC; u < ng; u += TPC) {
+        uint32_t d[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) d[i] = src[4 * u + i];
+        put4(u, d);
+      }
     }
     __syncthreads();
     prefetch(k + 1);
     prefetch_delay(k + 1);
-    for (int cc = 0; cc < ncc; ++cc) {
+    auto add = [&](int a, const uint32_t* src) {
 #pragma unroll
-      for (int a = 0; a < NA; ++a) {
-        const uint32_t* src = lds8 + sdly[wave + D2_WAVES * a][cc] + lane;
-#pragma unroll
-        for (int q = 0; q < D8_Q; ++q) {
-          const uint32_t v = src[64 * q];
-          pa[a][q] += v;
-          if (SPLIT) po[a][SPLIT ? q : 0] += __builtin_amdgcn_perm(0u, v, 0x0C030C01u);
-        }
+      for (int q = 0; q < D8_Q; ++q) {
+        const uint32_t v = src[64 * q];
+        pa[a][q] += v;
+        if (SPLIT) po[a][SPLIT ? q : 0] += __builtin_amdgcn_perm(0u, v, 0x0C030C01u);
       }
+    };
+    if (ncc == CC) {
+      // a full chunk, four channels at a time: their offsets for the wave's
+      // trials in one 16-byte read per trial
+#pragma unroll
+      for (int v = 0; v < CC / 4; ++v) {
+        int off[NA][4];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+          const int4 o4 = *reinterpret_cast<const int4*>(&sdly[wave + D2_WAVES * a][4 * v]);
+          off[a][0] = o4.x;
+          off[a][1] = o4.y;
+          off[a][2] = o4.z;
+          off[a][3] = o4.w;
+        }
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+          for (int a = 0; a < NA; ++a) add(a, lds8 + off[a][cc] + lane);
+      }
+    } else {
+      for (int cc = 0; cc < ncc; ++cc)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) add(a, lds8 + sdly[wave + D2_WAVES * a][cc] + lane);
     }
     pending += ncc;
     if (pending + CC > widen || c0 + CC >= g.n_live) {
