@@ -75,6 +75,34 @@ __device__ float lds_finish_f(float* A, float* B, size_t n) {
   return result;
 }
 
+// Orders one wave's LDS writes before its later reads (the wave's LDS
+// operations execute in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// `levels` median-of-5 levels over src[0 .. n) inside one wave, ping-ponging
+// between src and dst; returns the buffer holding the last level.
+__device__ __forceinline__ float* wave_pyramid(float* src, float* dst, size_t n, int levels) {
+  const unsigned lane = threadIdx.x & 63;
+  for (int l = 0; l < levels; ++l) {
+    wave_sync();
+    const size_t m = n / 5;
+    for (size_t q = lane; q < m; q += 64) {
+      const float* p = src + 5 * q;
+      dst[q] = med5(p[0], p[1], p[2], p[3], p[4]);
+    }
+    float* t = src;
+    src = dst;
+    dst = t;
+    n = m;
+  }
+  wave_sync();
+  return src;
+}
+
 // dst[t] = src[t * stride] for t < n (LDS from global), with SU loads in
 // flight per thread before their LDS writes: a plain loop waits out the
 // memory latency once per element.
@@ -164,13 +192,16 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
   }
   // level-L0 median of the chunk's leaves for one width (each mode below
   // passes its own branch-free leaf function)
+  // (the value is meaningful in thread 0, which writes it: the first level
+  // runs block-wide, the others in wave 0 with no block-wide barrier)
   auto chunk_value = [&](auto leaf) -> float {
     if (bs.L0 == 0) return leaf(k0);
     for (size_t q = threadIdx.x; q < n1; q += FB_BS) {
       const size_t b = k0 + 5 * q;
       A[q] = med5(leaf(b), leaf(b + 1), leaf(b + 2), leaf(b + 3), leaf(b + 4));
     }
-    return lds_pyramid<FB_BS>(A, B, (int)n1, bs.L0 - 1)[0];
+    __syncthreads();
+    return threadIdx.x < 64 ? wave_pyramid(A, B, n1, bs.L0 - 1)[0] : 0.f;
   };
   if (use_tile) {
     // One wave per width (widths first_w + w, + 4, ...): the chunk's median
@@ -179,11 +210,6 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* WA = A + (size_t)wave * (n1 + n1 / 5 + 2);
     float* WB = WA + (n1 ? n1 : 1);
-    auto wave_sync = [] {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
     for (int g = wave; g < n_group; g += FB_BS / 64) {
       const int j = first_w + g;
       const WidthInfo wi = wt.wi[j];
@@ -197,23 +223,7 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
           const unsigned r = 5 * (unsigned)q;  // pos - pos0 of the group's first leaf
           WA[q] = med5(leaf(r), leaf(r + 1), leaf(r + 2), leaf(r + 3), leaf(r + 4));
         }
-        float* src = WA;
-        float* dst = WB;
-        size_t n = n1;
-        for (int l = 1; l < bs.L0; ++l) {
-          wave_sync();
-          const size_t m = n / 5;
-          for (size_t q = lane; q < m; q += 64) {
-            const float* p = src + 5 * q;
-            dst[q] = med5(p[0], p[1], p[2], p[3], p[4]);
-          }
-          float* t = src;
-          src = dst;
-          dst = t;
-          n = m;
-        }
-        wave_sync();
-        value = src[0];
+        value = wave_pyramid(WA, WB, n1, bs.L0 - 1)[0];
       }
       if (lane == 0) partial[(row * wt.nw + j) * np_max + c] = value;
       wave_sync();  // the buffers are rewritten for the wave's next width
