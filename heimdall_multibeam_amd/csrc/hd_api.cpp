@@ -1314,7 +1314,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     gb.oval = pl->oval.as<float>();
     gb.head = pl->head.as<unsigned>();
     gb.segid = pl->segid.as<unsigned>();
-    HD_CHECK(hd::launch_order_samples(gb, nkeys, s));
+    HD_CHECK(hd::launch_order_samples(gb, wt, (unsigned)nsamp, s));
     HD_CHECK(hd::launch_segment_heads(gb, wt, nsamp, s));
     HD_CHECK(hd::launch_exclusive_scan_u32(gb.head, gb.segid, nsamp, gb.scan_tmp, s));
     HD_CHECK(hipMemcpyAsync(pl->h_pinned, gb.segid + (nsamp - 1), sizeof(unsigned),

@@ -214,6 +214,10 @@ hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n
                                      unsigned* tmp, hipStream_t s);
 size_t exclusive_scan_tmp_words(size_t n);
 hipError_t launch_order_samples(GiantBuffers& gb, size_t nkeys, hipStream_t s);
+// The samples of launch_threshold (keys of gb.chunk positions) in key order,
+// one thread per sample.
+hipError_t launch_order_samples(GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
+                                hipStream_t s);
 hipError_t launch_segment_heads(const GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
                                 hipStream_t s);
 // giants SoA outputs

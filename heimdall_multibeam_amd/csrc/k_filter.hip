@@ -604,6 +604,22 @@ __global__ void k_order_samples(GiantBuffers gb, size_t nkeys) {
   }
 }
 
+// The same ordering with one thread per sample: a sample's key (trial, width,
+// chunk) follows from its record, output i of width j lying in the chunk of
+// position i*ts; its place is the key's ordered offset plus its rank in the
+// key's unordered run.
+__global__ void k_order_samples_by_sample(GiantBuffers gb, WidthTable wt, unsigned nsamp) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nsamp) return;
+  const uint64_t kv = gb.skey[p];
+  const uint64_t dmw = kv >> 32;
+  const unsigned j = (unsigned)(dmw % (uint64_t)wt.nw);
+  const uint64_t pos = (kv & 0xffffffffull) * wt.wi[j].ts;
+  const size_t key = (size_t)dmw * gb.nchunks_per_w + (size_t)(pos / gb.chunk);
+  const size_t dst = (size_t)gb.cpos[key] + (p - gb.coff[key]);
+  gb.okey[dst] = kv;
+  gb.oval[dst] = gb.sval[p];
+}
 __global__ void k_segment_heads(GiantBuffers gb, WidthTable wt, unsigned nsamp) {
   const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nsamp) return;
@@ -775,6 +791,14 @@ hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n
 hipError_t launch_order_samples(GiantBuffers& gb, size_t nkeys, hipStream_t s) {
   hipLaunchKernelGGL(k_order_samples, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s, gb,
                      nkeys);
+  return hipGetLastError();
+}
+
+hipError_t launch_order_samples(GiantBuffers& gb, const WidthTable& wt, unsigned nsamp,
+                                hipStream_t s) {
+  if (nsamp == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_order_samples_by_sample, dim3((nsamp + 255) / 256), dim3(256), 0, s, gb,
+                     wt, nsamp);
   return hipGetLastError();
 }
 
