@@ -644,17 +644,30 @@ __global__ void k_segment_heads(GiantBuffers gb, WidthTable wt, unsigned nsamp) 
 __global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp, unsigned n_dm,
                                   GiantOut go, uint64_t* best, const GroupDev* __restrict__ gd) {
   const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nsamp) return;
-  const unsigned h = gb.head[k];
-  const unsigned g = gb.segid[k] + h - 1u;
-  const uint64_t key = gb.okey[k];
+  const unsigned lane = threadIdx.x & 63;
+  const bool valid = k < nsamp;
+  const unsigned h = valid ? gb.head[k] : 1u;
+  const unsigned g = valid ? gb.segid[k] + h - 1u : 0xffffffffu;  // no segment past nsamp
+  const uint64_t key = valid ? gb.okey[k] : 0ull;
+  // the segment's best sample: (orderable S/N, ~index) maximised over each run
+  // of equal g inside the wave (segmented scan, runs are contiguous), then one
+  // atomic per run piece instead of one per sample
+  unsigned long long bk =
+      valid ? (((uint64_t)float_key(gb.oval[k]) << 32) | (uint64_t)(0xffffffffu - k)) : 0ull;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned long long ov = __shfl_up(bk, d);
+    const unsigned og = __shfl_up(g, d);
+    if (lane >= (unsigned)d && og == g && ov > bk) bk = ov;
+  }
+  const unsigned ng = __shfl_down(g, 1);
+  if (!valid) return;
+  if (lane == 63 || ng != g) atomicMax((unsigned long long*)&best[g], bk);
   const uint64_t dmw = key >> 32;
   const unsigned j = (unsigned)(dmw % (uint64_t)wt.nw);
   const WidthInfo wi = wt.wi[j];
   const uint64_t ro = wi.rel_off, sc = wi.ts;
   const uint64_t i = key & 0xffffffffull;
-  const uint64_t bk = ((uint64_t)float_key(gb.oval[k]) << 32) | (uint64_t)(0xffffffffu - k);
-  atomicMax((unsigned long long*)&best[g], (unsigned long long)bk);
   if (h) {
     const uint64_t row = dmw / (uint64_t)wt.nw;  // group * n_dm + trial
     go.begin[g] = (i + ro) * sc;
