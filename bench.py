@@ -40,7 +40,8 @@ def lds_roof(adds, t, tile):
 def tile_json(t):
     """The dedispersion tile the autotuner chose (Pipeline.dedisp_tile())."""
     return {"trials_per_block": t[0], "xcd_remap": t[1], "byte_lanes": t[2],
-            "channels_per_step": t[3], "chosen_by": "autotuner (first call of the shape)"}
+            "channels_per_step": t[3], "block_threads": t[4],
+            "chosen_by": "autotuner (first call of the shape)"}
 
 
 def parse():

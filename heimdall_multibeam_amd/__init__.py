@@ -121,13 +121,13 @@ def lib():
     L.hd_set_cleaning.argtypes = [vp, C.c_int]
     L.hd_set_cleaning.restype = None
     L.hd_get_dedisp_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.hd_get_dedisp_tile.restype = None
     L.hd_set_graph.argtypes = [vp, C.c_int]
     L.hd_set_graph.restype = None
     L.hd_get_graph_stats.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.hd_get_graph_stats.restype = sz
-    L.hd_set_dedisp_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.hd_set_dedisp_tile.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.hd_set_dedisp_tile.restype = C.c_int
     L.hd_clean_filterbank_rfi.argtypes = [sz, C.c_float, C.c_float, vp, sz, sz, vp,
                                           C.POINTER(C.c_int), C.c_float, C.c_float, C.c_float,
@@ -217,17 +217,19 @@ class Pipeline:
         return c, n.value
 
     def dedisp_tile(self):
-        """(trials per block, XCD-aware order, 8-bit lanes, channels per step) of the
-        last call's dedispersion."""
-        db, rm, l8, cc = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
-        lib().hd_get_dedisp_tile(self.handle, C.byref(db), C.byref(rm), C.byref(l8), C.byref(cc))
-        return db.value, rm.value, l8.value, cc.value
+        """(trials per block, XCD-aware order, byte staging, channels per step, threads
+        per block) of the last call's dedispersion."""
+        v = [C.c_int(0) for _ in range(5)]
+        lib().hd_get_dedisp_tile(self.handle, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
 
-    def set_dedisp_tile(self, trials_per_block, xcd_remap=1, byte_lanes=0, channels_per_step=4):
+    def set_dedisp_tile(self, trials_per_block, xcd_remap=1, byte_lanes=0, channels_per_step=4,
+                        block_threads=512):
         """Fix the dedispersion tile (8, 16 or 32 trials per block; byte staging;
-        4, 8 or 16 channels staged per step), or 0 for the autotuner."""
+        4, 8 or 16 channels staged per step; 512 or 1024 threads, the latter for
+        byte staging at 32 trials), or 0 for the autotuner."""
         err = lib().hd_set_dedisp_tile(self.handle, trials_per_block, xcd_remap, byte_lanes,
-                                       channels_per_step)
+                                       channels_per_step, block_threads)
         if err != HD_NO_ERROR:
             raise HdError(err, "hd_set_dedisp_tile")
 

@@ -145,16 +145,16 @@ struct hd_pipeline_t {
   DevBuf pairs_db[9];
   int wlen_db[9] = {};
   int wlen8_db[9] = {};  // 8-bit-lane kernel (sub-byte samples)
-  std::map<std::tuple<int, int, int, int>, std::array<int, 4>> tune;
+  std::map<std::tuple<int, int, int, int>, std::array<int, 5>> tune;
   hipEvent_t tev[2] = {};
-  int last_db = 16, last_remap = 1, last_lanes8 = 0, last_cc = 4;
+  int last_db = 16, last_remap = 1, last_lanes8 = 0, last_cc = 4, last_wide = 0;
   // the stage chain as a HIP graph (hd_set_graph)
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
   std::string gkey;
   size_t graph_captures = 0, graph_launches = 0;
   // hd_set_dedisp_tile (fixed_db 0: autotune)
-  int fixed_db = 0, fixed_remap = 1, fixed_lanes8 = 0, fixed_cc = 4;
+  int fixed_db = 0, fixed_remap = 1, fixed_lanes8 = 0, fixed_cc = 4, fixed_wide = 0;
   // results of the last call
   size_t n_giants = 0;
   std::vector<float> r_peak, r_dms;
@@ -716,7 +716,7 @@ int tile_index(int db, int cc) {
 }
 
 void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes8, int cc,
-                size_t nbits) {
+                int wide, size_t nbits) {
   const int k = tile_index(db, cc);
   dg.db = db;
   dg.cc = cc;
@@ -725,22 +725,24 @@ void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes
   dg.widen8 = lanes8 ? widen8_for(nbits) : 0;
   dg.wlen8 = lanes8 ? pl->wlen8_db[k] : 0;
   dg.split8 = lanes8 && nbits == 8 ? 1 : 0;
+  dg.wide = lanes8 && db == 32 ? wide : 0;
 }
 
 hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
                             size_t nbits, hipStream_t s, const int** pairs) {
-  using Tile = std::array<int, 4>;  // trials per block, remap, byte staging, channels per step
+  // trials per block, remap, byte staging, channels per step, 1024 threads
+  using Tile = std::array<int, 5>;
   static const Tile forced = [] {
     const char* e = getenv("HD_DEDISP_TILE");
-    int db = 0, rm = 1, l8 = 0, cc = 4;
-    if (e && std::sscanf(e, "%d,%d,%d,%d", &db, &rm, &l8, &cc) >= 1 &&
+    int db = 0, rm = 1, l8 = 0, cc = 4, wide = 0;
+    if (e && std::sscanf(e, "%d,%d,%d,%d,%d", &db, &rm, &l8, &cc, &wide) >= 1 &&
         (db == 8 || db == 16 || db == 32))
-      return Tile{db, rm ? 1 : 0, l8 ? 1 : 0, (cc == 8 || cc == 16) ? cc : 4};
-    return Tile{0, 1, 0, 4};
+      return Tile{db, rm ? 1 : 0, l8 ? 1 : 0, (cc == 8 || cc == 16) ? cc : 4, wide ? 1 : 0};
+    return Tile{0, 1, 0, 4, 0};
   }();
   Tile c;
   if (pl->fixed_db) {
-    c = Tile{pl->fixed_db, pl->fixed_remap, pl->fixed_lanes8, pl->fixed_cc};
+    c = Tile{pl->fixed_db, pl->fixed_remap, pl->fixed_lanes8, pl->fixed_cc, pl->fixed_wide};
   } else if (forced[0]) {
     c = forced;
   } else {
@@ -754,17 +756,18 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
         HD_CHECK(hipEventCreate(&pl->tev[1]));
       }
       float best = 1e30f;
-      Tile choice{16, 1, 0, 4};
+      Tile choice{16, 1, 0, 4, 0};
       for (int lanes = 0; lanes < 2; ++lanes)
         for (int cc = 4; cc <= 16; cc *= 2)
           for (int k = 0; k < 3; ++k)
+            for (int wide = 0; wide <= (lanes && k == 2 ? 1 : 0); ++wide)
             for (int r = 0; r < 2; ++r) {
               // timed on the first (at most) 48 time tiles of the first (at most)
               // 64 groups: the tiles are alike, and large shapes would otherwise
               // spend seconds here; fewer groups would leave a launch too small
               // to rank the tiles as the full launch does
               hd::DedispGeom t = dg;
-              apply_tile(pl, t, 8 << k, r, lanes, cc, nbits);
+              apply_tile(pl, t, 8 << k, r, lanes, cc, wide, nbits);
               t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
               hd::GroupArgs tg = gr;
               tg.G = std::min(gr.G, 64);
@@ -783,30 +786,31 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
               HD_CHECK(hipEventElapsedTime(&ms, pl->tev[0], pl->tev[1]));
               if (ms < best) {
                 best = ms;
-                choice = Tile{8 << k, r, lanes, cc};
+                choice = Tile{8 << k, r, lanes, cc, wide};
               }
             }
       it = pl->tune.emplace(key, choice).first;
       if (pl->params.verbosity >= 2)
         std::cout << "dedispersion tile: " << choice[0] << " trials per block, " << choice[3]
                   << " channels per step, " << (choice[1] ? "XCD-aware" : "linear")
-                  << " order, " << (choice[2] ? "8" : "16") << "-bit lanes (" << best << " ms)"
-                  << std::endl;
+                  << " order, " << (choice[2] ? "byte" : "16-bit pair") << " staging"
+                  << (choice[4] ? ", 1024 threads" : "") << " (" << best << " ms)" << std::endl;
     }
     c = it->second;
   }
   if (c[2] && widen8_for(nbits) < c[3]) c[2] = 0;  // byte lanes would carry within a step
-  apply_tile(pl, dg, c[0], c[1], c[2], c[3], nbits);
+  apply_tile(pl, dg, c[0], c[1], c[2], c[3], c[4], nbits);
   // a fixed tile whose staged copies exceed the LDS stages fewer channels per step
   while (c[3] > 4 && hd::dedisp_lds_bytes(dg) > 160 * 1024) {
     c[3] /= 2;
-    apply_tile(pl, dg, c[0], c[1], c[2], c[3], nbits);
+    apply_tile(pl, dg, c[0], c[1], c[2], c[3], c[4], nbits);
   }
   *pairs = pl->pairs_db[tile_index(c[0], c[3])].as<int>();
   pl->last_db = c[0];
   pl->last_remap = c[1];
   pl->last_lanes8 = c[2];
   pl->last_cc = c[3];
+  pl->last_wide = dg.wide;
   return HD_NO_ERROR;
 }
 
@@ -1919,26 +1923,29 @@ extern "C" hd_size hd_get_last_proposal(hd_pipeline pl, unsigned char* flags, hd
 }
 
 extern "C" void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap,
-                                   int* byte_lanes, int* channels_per_step) {
+                                   int* byte_lanes, int* channels_per_step, int* block_threads) {
   if (!pl) return;
   if (trials_per_block) *trials_per_block = pl->last_db;
   if (xcd_remap) *xcd_remap = pl->last_remap;
   if (byte_lanes) *byte_lanes = pl->last_lanes8;
   if (channels_per_step) *channels_per_step = pl->last_cc;
+  if (block_threads) *block_threads = pl->last_wide ? 1024 : 512;
 }
 
 extern "C" hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap,
-                                      int byte_lanes, int channels_per_step) {
+                                      int byte_lanes, int channels_per_step, int block_threads) {
   if (!pl) return HD_INVALID_PIPELINE;
   if (trials_per_block != 0 && trials_per_block != 8 && trials_per_block != 16 &&
       trials_per_block != 32)
     return HD_INVALID_STRIDE;
   if (channels_per_step != 4 && channels_per_step != 8 && channels_per_step != 16)
     return HD_INVALID_STRIDE;
+  if (block_threads != 0 && block_threads != 512 && block_threads != 1024) return HD_INVALID_STRIDE;
   pl->fixed_db = trials_per_block;
   pl->fixed_remap = xcd_remap ? 1 : 0;
   pl->fixed_lanes8 = byte_lanes ? 1 : 0;
   pl->fixed_cc = channels_per_step;
+  pl->fixed_wide = block_threads == 1024 ? 1 : 0;
   return HD_NO_ERROR;
 }
 

@@ -147,6 +147,7 @@ struct DedispGeom {
   int wlen8 = 0;       // its LDS dwords per staged copy
   int split8 = 0;      // byte-staged kernel for 8-bit samples (even/odd byte sums)
   int cc = 4;          // channels staged per step (4, 8 or 16; the staging pairs are planned for it)
+  int wide = 0;        // byte staging at 32 trials: 1024 threads (2 trials per wave) instead of 512
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,

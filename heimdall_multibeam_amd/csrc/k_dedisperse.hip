@@ -352,14 +352,15 @@ constexpr int d8_waves(int db, int cc, bool split) {
 }
 constexpr int D8_PF = 2;                  // staging items (4 dwords) prefetched per thread
 
-template <int DB, int CC, bool SPLIT>
-__global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(const uint8_t* __restrict__ x,
+template <int DB, int CC, bool SPLIT, int BS = D2_BS>
+__global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void k_dedisperse8(const uint8_t* __restrict__ x,
                                                       const int* __restrict__ delays,
                                                       DedispGeom g, int hw_max, int widen,
                                                       float* __restrict__ out,
                                                       const GroupDev* __restrict__ gd) {
   extern __shared__ uint32_t lds8[];  // [CC][4][hw_max]
-  constexpr int NA = DB / D2_WAVES;
+  constexpr int WAVES = BS / 64;
+  constexpr int NA = DB / WAVES;
   __shared__ __align__(16) int sdly[DB][CC];
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
   const int trial0 = gd ? (int)gd[blockIdx.z].trial0 : 0;
@@ -397,7 +398,7 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(
     }
   int pending = 0;
   const int nchunk = (g.n_live + CC - 1) / CC;
-  constexpr int TPC = D2_BS / CC;
+  constexpr int TPC = BS / CC;
   const int scc = tid / TPC;
   const int sjj = tid % TPC;
   // dwords of each staged copy that channel c needs: its window spans the
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(
         int off[NA][4];
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
-          const int4 o4 = *reinterpret_cast<const int4*>(&sdly[wave + D2_WAVES * a][4 * v]);
+          const int4 o4 = *reinterpret_cast<const int4*>(&sdly[wave + WAVES * a][4 * v]);
           off[a][0] = o4.x;
           off[a][1] = o4.y;
           off[a][2] = o4.z;
@@ -502,7 +503,7 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(
     } else {
       for (int cc = 0; cc < ncc; ++cc)
 #pragma unroll
-        for (int a = 0; a < NA; ++a) add(a, lds8 + sdly[wave + D2_WAVES * a][cc] + lane);
+        for (int a = 0; a < NA; ++a) add(a, lds8 + sdly[wave + WAVES * a][cc] + lane);
     }
     pending += ncc;
     if (pending + CC > widen || c0 + CC >= g.n_live) {
@@ -529,7 +530,7 @@ __global__ __launch_bounds__(D2_BS, d8_waves(DB, CC, SPLIT)) void k_dedisperse8(
   }
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
-    const int m = m0 + wave + D2_WAVES * a;
+    const int m = m0 + wave + WAVES * a;
     if (m >= g.n_dm) continue;
     float* orow = out + (size_t)m * g.ppitch;
 #pragma unroll
@@ -584,7 +585,7 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
 // Raises a kernel's dynamic-LDS limit once (never during a later capture:
 // every shape is launched eagerly before its chain is captured).
 hipError_t lds_limit(const void* fn, int slot, size_t lds) {
-  static size_t set[27] = {0};
+  static size_t set[33] = {0};
   if (lds <= 64 * 1024 || lds <= set[slot]) return hipSuccess;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess) set[slot] = lds;
@@ -599,9 +600,9 @@ template <int DB, int CC>
 const void* fn16() {
   return reinterpret_cast<const void*>(&k_dedisperse16<DB, CC>);
 }
-template <int DB, int CC, bool SPLIT>
+template <int DB, int CC, bool SPLIT, int BS = D2_BS>
 const void* fn8() {
-  return reinterpret_cast<const void*>(&k_dedisperse8<DB, CC, SPLIT>);
+  return reinterpret_cast<const void*>(&k_dedisperse8<DB, CC, SPLIT, BS>);
 }
 const void* kernel16(int slot) {
   static const void* const t[9] = {fn16<8, 4>(),  fn16<16, 4>(),  fn16<32, 4>(),
@@ -619,6 +620,15 @@ const void* kernel8(int slot, bool split) {
       fn8<8, 8, true>(),  fn8<16, 8, true>(),  fn8<32, 8, true>(),
       fn8<8, 16, true>(), fn8<16, 16, true>(), fn8<32, 16, true>()};
   return split ? ts[slot] : t[slot];
+}
+// 32 trials on 1024 threads (16 waves, 2 trials each), by channels per step
+const void* kernel8_wide(int cc, bool split) {
+  static const void* const t[3] = {fn8<32, 4, false, 1024>(), fn8<32, 8, false, 1024>(),
+                                   fn8<32, 16, false, 1024>()};
+  static const void* const ts[3] = {fn8<32, 4, true, 1024>(), fn8<32, 8, true, 1024>(),
+                                    fn8<32, 16, true, 1024>()};
+  const int k = cc == 4 ? 0 : (cc == 8 ? 1 : 2);
+  return split ? ts[k] : t[k];
 }
 
 // LDS of a tile: the staged copies (dynamic) plus the trial offsets (static).
@@ -644,13 +654,18 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
     if (g.widen8 < cc || g.wlen8 <= 0) return hipErrorInvalidValue;
     if (g.split8 && g.widen8 > 256) return hipErrorInvalidValue;   // 16-bit lanes would carry
     const size_t lds8 = (size_t)cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t);
-    const void* f8 = kernel8(slot, g.split8 != 0);
-    const hipError_t e = lds_limit(f8, (g.split8 ? 18 : 9) + slot, lds8);
+    const bool wide = g.wide && db == 32;
+    const void* f8 = wide ? kernel8_wide(cc, g.split8 != 0) : kernel8(slot, g.split8 != 0);
+    const hipError_t e =
+        lds_limit(f8, wide ? 27 + (cc == 4 ? 0 : cc == 8 ? 1 : 2) + (g.split8 ? 3 : 0)
+                           : (g.split8 ? 18 : 9) + slot,
+                  lds8);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(reinterpret_cast<void (*)(const uint8_t*, const int*, DedispGeom, int, int,
                                                  float*, const GroupDev*)>(
                            const_cast<void*>(f8)),
-                       grid, dim3(D2_BS), lds8, s, x, d_delays, g, g.wlen8, g.widen8, plane,
+                       grid, dim3(wide ? 1024 : D2_BS), lds8, s, x, d_delays, g, g.wlen8, g.widen8,
+                       plane,
                        ga.gd);
     return hipGetLastError();
   }

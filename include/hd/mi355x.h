@@ -79,16 +79,18 @@ hd_size hd_get_last_timings(hd_pipeline pl, double* ms, hd_size cap);
  * whether blocks were ordered for the 8 XCDs' L2 caches, whether samples were
  * staged as bytes (4 per LDS word: summed in 8-bit lanes for 1-, 2- and 4-bit
  * input, as even/odd 16-bit-lane sums for 8-bit input) rather than as 16-bit
- * pairs, and the channels staged in LDS per step (4, 8 or 16).  The first
- * call of a shape times every candidate on the real input and keeps the
- * fastest (all give the same sums); HD_DEDISP_TILE="db,remap,lanes8,cc" in
- * the environment fixes it for a process.  Any output pointer may be NULL. */
+ * pairs, the channels staged in LDS per step (4, 8 or 16), and the threads
+ * per block (512, or 1024 for byte staging at 32 trials).  The first call of
+ * a shape times every candidate on the real input and keeps the fastest (all
+ * give the same sums); HD_DEDISP_TILE="db,remap,bytes,cc,wide" in the
+ * environment fixes it for a process.  Any output pointer may be NULL. */
 void hd_get_dedisp_tile(hd_pipeline pl, int* trials_per_block, int* xcd_remap, int* byte_lanes,
-                        int* channels_per_step);
+                        int* channels_per_step, int* block_threads);
 /* Fixes the dedispersion tile (trials_per_block 8, 16 or 32; channels_per_step
- * 4, 8 or 16), or returns to the autotuner (trials_per_block 0). */
+ * 4, 8 or 16; block_threads 512 or 1024, the latter for byte staging at 32
+ * trials), or returns to the autotuner (trials_per_block 0). */
 hd_error hd_set_dedisp_tile(hd_pipeline pl, int trials_per_block, int xcd_remap, int byte_lanes,
-                            int channels_per_step);
+                            int channels_per_step, int block_threads);
 
 /* Runs the per-gulp stage chain from dedispersion to thresholding as a HIP
  * graph: captured on the first call of a set of launch arguments (shapes,

@@ -139,7 +139,7 @@ def test_every_dedisp_tile_gives_the_same_search(oracle, db):
         pl.set_output(False)
         pl.set_dedisp_tile(db, remap, 0, cc)
         err, processed = pl.execute(fb, w.nsamps, w.nbits)
-        assert pl.dedisp_tile() == (db, remap, 0, cc)
+        assert pl.dedisp_tile() == (db, remap, 0, cc, 512)
         gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
                    text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
         pl.close()
@@ -164,13 +164,15 @@ def test_byte_lane_dedispersion_matches_oracle(oracle, nbits, level):
     fb = synth.make_gulp(w, 40 + nbits)
     p = oracle_params_for(oracle, w)
     r = oracle.execute(p, fb, w.nsamps, nbits, 0, 1)
-    for db, cc in ((8, 4), (16, 4), (32, 4), (16, 8), (32, 8), (32, 16)):
-        print("case byte lanes", nbits, "bits, tile", db, "channels per step", cc, flush=True)
+    for db, cc, threads in ((8, 4, 512), (16, 4, 512), (32, 4, 512), (16, 8, 512), (32, 8, 512),
+                            (32, 16, 512), (32, 8, 1024), (32, 16, 1024)):
+        print("case byte lanes", nbits, "bits, tile", db, "channels per step", cc, "threads",
+              threads, flush=True)
         pl = H.Pipeline(params_for(w))
         pl.set_output(False)
-        pl.set_dedisp_tile(db, 1, 1, cc)
+        pl.set_dedisp_tile(db, 1, 1, cc, threads)
         err, processed = pl.execute(fb, w.nsamps, nbits)
-        assert pl.dedisp_tile() == (db, 1, 1, cc)
+        assert pl.dedisp_tile() == (db, 1, 1, cc, threads)
         gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
                    text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
         pl.close()
