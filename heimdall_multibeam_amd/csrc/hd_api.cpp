@@ -789,6 +789,30 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
                 choice = Tile{8 << k, r, lanes, cc, wide};
               }
             }
+      // The block order's benefit (L2 sharing within an XCD) grows with the
+      // time tiles a launch holds, so the winner's order is settled on all of
+      // them (at most 64 groups).
+      {
+        float bt[2] = {1e30f, 1e30f};
+        for (int r = 0; r < 2; ++r) {
+          hd::DedispGeom t = dg;
+          apply_tile(pl, t, choice[0], r, choice[2], choice[3], choice[4], nbits);
+          hd::GroupArgs tg = gr;
+          tg.G = std::min(gr.G, 64);
+          const int* tp = pl->pairs_db[tile_index(choice[0], choice[3])].as<int>();
+          auto launch = [&]() {
+            return hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), tp, t,
+                                         pl->plane.as<float>(), s, tg);
+          };
+          if (launch() != hipSuccess) continue;
+          HD_CHECK(hipEventRecord(pl->tev[0], s));
+          HD_CHECK(launch());
+          HD_CHECK(hipEventRecord(pl->tev[1], s));
+          HD_CHECK(hipEventSynchronize(pl->tev[1]));
+          HD_CHECK(hipEventElapsedTime(&bt[r], pl->tev[0], pl->tev[1]));
+        }
+        if (bt[0] < 1e30f || bt[1] < 1e30f) choice[1] = bt[1] <= bt[0] ? 1 : 0;
+      }
       it = pl->tune.emplace(key, choice).first;
       if (pl->params.verbosity >= 2)
         std::cout << "dedispersion tile: " << choice[0] << " trials per block, " << choice[3]
