@@ -1179,7 +1179,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   int n_fresh = 0;
   while (n_fresh < wt.nw && wt.wi[n_fresh].ts == 1) ++n_fresh;
   hd::GiantBuffers gb{};
-  gb.chunk = 4096;
+  gb.chunk = hd::threshold_chunk();
   size_t n_w0 = 0;
   for (int g = 0; g < G; ++g) n_w0 = std::max<size_t>(n_w0, pl->hgd[g].n_w[0]);
   gb.nchunks_per_w = (n_w0 + gb.chunk - 1) / gb.chunk;

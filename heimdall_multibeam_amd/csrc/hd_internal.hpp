@@ -206,6 +206,7 @@ struct GiantBuffers {
   size_t nchunks_per_w;
   size_t chunk;        // outputs per chunk
 };
+size_t threshold_chunk();  // prefix-sum positions per launch_threshold block (one key each)
 hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
                             const float* winv, int n_dm, float thresh, GiantBuffers& gb,
                             hipStream_t s, const GroupArgs& ga);

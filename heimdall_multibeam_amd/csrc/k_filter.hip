@@ -16,7 +16,7 @@
 //              the latest earlier width that reached that position
 //              (pipeline.cu:680-712 reuse d_filtered_series).
 //   threshold  GiantFinder's selection (find_giants.cu:47-60,137-183): per
-//              (DM, width, 4096-output chunk) the samples above threshold are
+//              (DM, width, 8192-position chunk) the samples above threshold are
 //              compacted in time order and appended to an unordered buffer;
 //              the per-chunk counts are scanned so that a second pass lays the
 //              samples out in the reference's order (DM, width, time).
@@ -306,8 +306,8 @@ __device__ unsigned block_exclusive_scan(unsigned v, unsigned* total) {
   return pre + x - v;
 }
 
-constexpr int TH_PER = 16;
-constexpr int TH_CHUNK = FB_BS * TH_PER;  // 4096 prefix-sum positions per block
+constexpr int TH_PER = 32;
+constexpr int TH_CHUNK = FB_BS * TH_PER;  // 8192 prefix-sum positions per block
 
 // One block per (chunk of TH_CHUNK positions i*ts, DM): the chunk's prefix
 // sums (plus the widest boxcar's halo) are staged in LDS once and every width
@@ -800,6 +800,8 @@ hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n
   hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(SC_BS), 0, s, out, n, offs);
   return hipGetLastError();
 }
+
+size_t threshold_chunk() { return TH_CHUNK; }
 
 hipError_t launch_order_samples(GiantBuffers& gb, size_t nkeys, hipStream_t s) {
   hipLaunchKernelGGL(k_order_samples, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s, gb,
