@@ -357,7 +357,7 @@ def gated_main(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the search has no multi-GPU path "
               "(beams or DM ranges would shard one process per GPU); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)

@@ -3,7 +3,7 @@
 // Clustering (Pipeline/label_candidate_clusters.cu:141-197, are_coincident.cuh:
 // 46-68).  The reference relabels every giant i to the minimum label over all
 // coincident giants in one parallel pass and then traces label chains; with a
-// single thread (the sequential semantics this build adopts, since the
+// single thread (the sequential semantics adopted here, since the
 // parallel pass is order dependent) that is
 //     L[i] = min(i, min{ L[j] : j < i, coincident(i, j) }),
 // i.e. L[i] is the smallest index reachable from i along coincident links to
