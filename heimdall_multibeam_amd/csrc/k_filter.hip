@@ -75,34 +75,6 @@ __device__ float lds_finish_f(float* A, float* B, size_t n) {
   return result;
 }
 
-// Orders one wave's LDS writes before its later reads (the wave's LDS
-// operations execute in order; this keeps the compiler from moving them).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// `levels` median-of-5 levels over src[0 .. n) inside one wave, ping-ponging
-// between src and dst; returns the buffer holding the last level.
-__device__ __forceinline__ float* wave_pyramid(float* src, float* dst, size_t n, int levels) {
-  const unsigned lane = threadIdx.x & 63;
-  for (int l = 0; l < levels; ++l) {
-    wave_sync();
-    const size_t m = n / 5;
-    for (size_t q = lane; q < m; q += 64) {
-      const float* p = src + 5 * q;
-      dst[q] = med5(p[0], p[1], p[2], p[3], p[4]);
-    }
-    float* t = src;
-    src = dst;
-    dst = t;
-    n = m;
-  }
-  wave_sync();
-  return src;
-}
-
 // dst[t] = src[t * stride] for t < n (LDS from global), with SU loads in
 // flight per thread before their LDS writes: a plain loop waits out the
 // memory latency once per element.
