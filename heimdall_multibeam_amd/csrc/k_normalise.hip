@@ -11,9 +11,9 @@
 //   normalise  normalise_multibeam (get_rms.cu:202-212): z = (x-baseline)/rms,
 //              in place (stage entry point), or fused into the scan's loads.
 //   scan       MatchedFilterPlan::prep (matched_filter.cu:21-32): the exclusive
-//              prefix sum in the reference's sequential order, one lane per
-//              series over row tiles staged through LDS so that HBM reads and
-//              writes stay coalesced.
+//              prefix sum in the reference's sequential order, 8 lanes per
+//              series taking turns over row tiles staged through LDS so that
+//              HBM reads and writes stay coalesced.
 #include <algorithm>
 
 #include "hd_device.hpp"
@@ -218,12 +218,16 @@ __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
 
 // Sequential exclusive scan per series: S[0] = 0, S[k] = S[k-1] + z[k-1].
 //
-// A block owns SC_ROWS series.  Wave 0 runs the dependent adds (one lane per
-// series) over a SC_T-sample tile held in LDS while waves 1..8 store the
-// previous tile's prefix sums and load the next tile (double buffer), so the
-// add chain never waits on HBM latency.  The scan lane works through its row
-// in batches of 32 values held in registers, loading the next batch before
-// it adds the current one, so LDS latency stays off the add chain too.
+// A block owns SC_ROWS series.  Wave 0 runs the dependent adds over a
+// SC_T-sample tile held in LDS while waves 1..8 store the previous tile's
+// prefix sums and load the next tile (double buffer), so the add chain never
+// waits on HBM latency.  Wave 0 gives each series 8 lanes: the tile is walked
+// in steps of 8 * SC_SEG samples, lane (series, seg) holds samples
+// seg*SC_SEG .. seg*SC_SEG + SC_SEG-1 of the step in registers, and the 8
+// segments take turns adding theirs onto the running sum, which a DPP move
+// hands from each lane to the next.  Every sample is still added in order,
+// one float add at a time, and the chain pays one LDS read and one write per
+// SC_SEG samples instead of per 4.
 //
 // When rms is given the loader normalises on the fly, z = (x - baseline)/rms
 // exactly as k_normalise does, so the pipeline never writes the normalised
@@ -235,7 +239,9 @@ constexpr int SC_MOVERS = 512;                  // waves 1..8
 constexpr int SC_THREADS = 64 + SC_MOVERS;
 constexpr int SC_VEC = SC_ROWS * SC_T / 4;       // float4 per tile
 constexpr int SC_PER = SC_VEC / SC_MOVERS;       // float4 per mover thread
-constexpr int SC_B4 = 8;                         // float4 per scan batch
+// samples per lane and step (32 measured slower)
+constexpr int SC_SEG = 16;
+static_assert(SC_T % (16 * SC_SEG) == 0, "two steps per loop trip");
 
 __device__ __forceinline__ void scan4(float4& v, float& run) {
   run = run + v.x;
@@ -246,6 +252,30 @@ __device__ __forceinline__ void scan4(float4& v, float& run) {
   v.z = run;
   run = run + v.w;
   v.w = run;
+}
+
+// DPP row moves (within each 16-lane row): lane i takes lane i-1's value
+// (row_shr:1), or lane (i-9) mod 16's (row_ror:9), which for the two 8-lane
+// groups of a row is each group's lane 7 into its lane 0.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  const int i = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false));
+}
+
+// One step of the segmented chain: segment k (lane seg == k of each 8-lane
+// group) adds its SEG samples onto run, then run moves one lane on; after
+// the last segment it wraps to the group's lane 0, ready for the next step.
+template <int SEG>
+__device__ __forceinline__ void scan_segments(float4 (&v)[SEG / 4], float& run, int seg) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (seg == k) {
+#pragma unroll
+      for (int i = 0; i < SEG / 4; ++i) scan4(v[i], run);
+    }
+    run = k < 7 ? dpp_move<0x111>(run) : dpp_move<0x129>(run);
+  }
 }
 
 constexpr int SC_PMAX = 1024;                    // staged baseline points per block
@@ -398,35 +428,38 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z
   for (size_t p = 0; p <= ntiles; ++p) {
     const int b = (int)(p & 1);
     if (tid < 64) {
-      if (p < ntiles && tid < SC_ROWS) {
-        const size_t rem = count - p * SC_T;
-        const int ncol = rem < (size_t)SC_T ? (int)rem : SC_T;
-        float* rowp = &buf[b][tid][0];
+      const int row = tid >> 3, seg = tid & 7;
+      if (p < ntiles && (size_t)SC_T * (p + 1) <= count) {
+        // full tile: lane (row, seg) holds samples 16*seg .. 16*seg + 15 of
+        // each 128-sample step of its row in registers; two steps in
+        // flight (load B, add A, store A, load A', add B ...)
+        constexpr int SEG = SC_SEG;
+        float4* s4 = reinterpret_cast<float4*>(&buf[b][row][SEG * seg]);
+        constexpr int STEP4 = SEG * 8 / 4;  // float4 per step and row
+        float4 va[SEG / 4], vb[SEG / 4];
+#pragma unroll
+        for (int i = 0; i < SEG / 4; ++i) va[i] = s4[i];
+        for (int u = 0; u < SC_T / (8 * SEG); u += 2) {
+#pragma unroll
+          for (int i = 0; i < SEG / 4; ++i) vb[i] = s4[STEP4 * (u + 1) + i];
+          scan_segments<SEG>(va, run, seg);
+#pragma unroll
+          for (int i = 0; i < SEG / 4; ++i) s4[STEP4 * u + i] = va[i];
+          if (u + 2 < SC_T / (8 * SEG)) {
+#pragma unroll
+            for (int i = 0; i < SEG / 4; ++i) va[i] = s4[STEP4 * (u + 2) + i];
+          }
+          scan_segments<SEG>(vb, run, seg);
+#pragma unroll
+          for (int i = 0; i < SEG / 4; ++i) s4[STEP4 * (u + 1) + i] = vb[i];
+        }
+      } else if (p < ntiles && seg == 0) {
+        // the short last tile: the segment-0 lane of each row (which holds
+        // the row's running sum) adds it alone
+        const int ncol = (int)(count - p * SC_T);
+        float* rowp = &buf[b][row][0];
         float4* row4 = reinterpret_cast<float4*>(rowp);
         int c = 0;
-        if (ncol == SC_T) {
-          // two register batches in flight: load B, add A, store A, load A', add B ...
-          float4 va[SC_B4], vb[SC_B4];
-#pragma unroll
-          for (int i = 0; i < SC_B4; ++i) va[i] = row4[i];
-          for (int c4 = 0; c4 < SC_T / 4; c4 += 2 * SC_B4) {
-#pragma unroll
-            for (int i = 0; i < SC_B4; ++i) vb[i] = row4[c4 + SC_B4 + i];
-#pragma unroll
-            for (int i = 0; i < SC_B4; ++i) scan4(va[i], run);
-#pragma unroll
-            for (int i = 0; i < SC_B4; ++i) row4[c4 + i] = va[i];
-            if (c4 + 2 * SC_B4 < SC_T / 4) {
-#pragma unroll
-              for (int i = 0; i < SC_B4; ++i) va[i] = row4[c4 + 2 * SC_B4 + i];
-            }
-#pragma unroll
-            for (int i = 0; i < SC_B4; ++i) scan4(vb[i], run);
-#pragma unroll
-            for (int i = 0; i < SC_B4; ++i) row4[c4 + SC_B4 + i] = vb[i];
-          }
-          c = SC_T;
-        }
         for (; c + 4 <= ncol; c += 4) {
           float4 v = row4[c / 4];
           scan4(v, run);
