@@ -111,6 +111,22 @@ __device__ __forceinline__ float key_float(uint32_t k) {
 }
 
 // ---------------------------------------------------------------------------
+// XCD-contiguous block order.  The MI355X hands workgroups to its 8 XCDs
+// round robin by linear id, so blocks next to each other in (x, y) order
+// land on different XCDs and each L2 fetches the halos that neighbouring
+// blocks share.  This returns the (x, y) block to work on such that the XCD
+// running linear id l (l mod 8) walks one contiguous range of the x-fastest
+// order instead: consecutive chunks of consecutive rows.  A bijection on
+// the gridDim.x * gridDim.y blocks of each z plane.
+__device__ __forceinline__ uint2 xcd_contiguous_block() {
+  const unsigned nbx = gridDim.x, nblk = gridDim.x * gridDim.y;
+  const unsigned lin = blockIdx.y * nbx + blockIdx.x;
+  const unsigned q = nblk / 8, r = nblk % 8, x8 = lin % 8;
+  const unsigned l = x8 * q + (x8 < r ? x8 : r) + lin / 8;
+  return make_uint2(l % nbx, l / nbx);
+}
+
+// ---------------------------------------------------------------------------
 // Block-wide helpers.  The block size is a compile-time constant BS and
 // every thread calls these.
 

@@ -105,7 +105,8 @@ __device__ inline void stage_lds(float* dst, const float* __restrict__ src, size
 // the staged entries are every ts-th prefix sum, E[x] = S[off - behind +
 // ts*(pos0 + x)], so that F[pos] = E[pos - pos0 + w/ts] - E[pos - pos0]
 // (w = ahead + behind is a multiple of ts); up to tile_words of them.
-// blockIdx: (chunk, trial, group); rows g*n_dm + m.
+// Grid: (chunks, trials, groups), blocks taken in the XCD-contiguous order;
+// rows g*n_dm + m.
 __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     const float* __restrict__ Sall, size_t spitch, WidthTable wt, const GroupDev* __restrict__ gd,
     size_t np_max, int first_w, int n_group, int tile_ok, unsigned amax, unsigned bmax,
@@ -113,9 +114,10 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
   extern __shared__ float lds[];
   const BlockRmsShape bs = gd[blockIdx.z].bs;
   const unsigned* nwv = gd[blockIdx.z].n_w;
-  const size_t c = blockIdx.x;
+  const uint2 xb = xcd_contiguous_block();  // chunk halos shared within an XCD's L2
+  const size_t c = xb.x;
   if (c >= bs.np) return;
-  const size_t row = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
+  const size_t row = (size_t)blockIdx.z * gridDim.y + xb.y;
   const float* S = Sall + row * spitch;
   const float* winv = winv_all + row * wt.nw;
   const size_t k0 = c * bs.chunk;
@@ -693,10 +695,13 @@ hipError_t launch_block_rms_partials(const float* S, size_t spitch, const WidthT
   }
   const size_t tlen = chunk + amax + bmax;
   const size_t n1 = chunk / 5;
-  // level buffers: block-wide, or one pair per wave (the LDS-tile path)
-  const size_t rest = std::max((n1 ? n1 : 1) + n1 / 5 + 8, (size_t)(FB_BS / 64) * (n1 + n1 / 5 + 2));
+  // level buffers: one pair per wave (the LDS-tile path), else block-wide
+  const size_t rest_block = (n1 ? n1 : 1) + n1 / 5 + 8;
+  const size_t rest_wave = std::max(rest_block, (size_t)(FB_BS / 64) * (n1 + n1 / 5 + 2));
   constexpr size_t kMaxLds = 65536 / sizeof(float);
-  if (tlen + rest > kMaxLds) tile_ok = 0;  // keep LDS <= 64 KiB
+  if (tlen + rest_wave > kMaxLds) tile_ok = 0;  // keep LDS <= 64 KiB
+  // the block-wide form asks for less LDS, so more blocks stay resident
+  const size_t rest = tile_ok ? rest_wave : rest_block;
   // single decimated width: room for the widest range it or an earlier width reads
   size_t tile_words = 0;
   if (!tile_ok && n_group == 1 && chunk > 0) {
