@@ -33,6 +33,13 @@ namespace {
 
 constexpr int FB_BS = 256;
 
+// LDS words of one wave's level buffers on the block-RMS tile path.  With at
+// least 25 leaves per chunk the first two levels run in registers and the
+// buffers hold levels 2 and 3 (n1/5 and n1/25 values), else levels 1 and 2.
+__host__ __device__ constexpr size_t block_rms_wave_words(size_t n1) {
+  return n1 >= 5 ? n1 / 5 + n1 / 25 + 2 : n1 + n1 / 5 + 2;
+}
+
 __device__ __forceinline__ float raw_val(const float* S, const WidthInfo& wi, unsigned off,
                                          size_t i) {
   const size_t base = i * wi.ts;
@@ -182,8 +189,9 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
     // pyramid runs inside the wave, in its own LDS buffers, with no
     // block-wide barrier; the groups of 5 are those of the reference.
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float* WA = A + (size_t)wave * (n1 + n1 / 5 + 2);
-    float* WB = WA + (n1 ? n1 : 1);
+    const size_t n2 = n1 / 5;
+    float* WA = A + (size_t)wave * block_rms_wave_words(n1);
+    float* WB = WA + (n2 ? n2 : (n1 ? n1 : 1));
     for (int g = wave; g < n_group; g += FB_BS / 64) {
       const int j = first_w + g;
       const WidthInfo wi = wt.wi[j];
@@ -192,6 +200,20 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
       float value;
       if (bs.L0 == 0) {
         value = leaf(0);
+      } else if (n2 > 0) {
+        // levels 1 and 2 in registers: a lane reduces 25 consecutive leaves
+        // (its lanes read 25 words apart, so no LDS bank conflicts)
+        for (size_t p = lane; p < n2; p += 64) {
+          const unsigned r = 25 * (unsigned)p;
+          float m[5];
+#pragma unroll
+          for (int i = 0; i < 5; ++i) {
+            const unsigned b = r + 5 * i;
+            m[i] = med5(leaf(b), leaf(b + 1), leaf(b + 2), leaf(b + 3), leaf(b + 4));
+          }
+          WA[p] = med5(m[0], m[1], m[2], m[3], m[4]);
+        }
+        value = wave_pyramid(WA, WB, n2, bs.L0 - 2)[0];
       } else {
         for (size_t q = lane; q < n1; q += 64) {
           const unsigned r = 5 * (unsigned)q;  // pos - pos0 of the group's first leaf
@@ -697,7 +719,7 @@ hipError_t launch_block_rms_partials(const float* S, size_t spitch, const WidthT
   const size_t n1 = chunk / 5;
   // level buffers: one pair per wave (the LDS-tile path), else block-wide
   const size_t rest_block = (n1 ? n1 : 1) + n1 / 5 + 8;
-  const size_t rest_wave = std::max(rest_block, (size_t)(FB_BS / 64) * (n1 + n1 / 5 + 2));
+  const size_t rest_wave = std::max(rest_block, (size_t)(FB_BS / 64) * block_rms_wave_words(n1));
   constexpr size_t kMaxLds = 65536 / sizeof(float);
   if (tlen + rest_wave > kMaxLds) tile_ok = 0;  // keep LDS <= 64 KiB
   // the block-wide form asks for less LDS, so more blocks stay resident
