@@ -125,15 +125,18 @@ __global__ __launch_bounds__(NB_BS) void k_beam_rms(const float* __restrict__ pl
 // Exact because the first L0 levels of the reference pyramid only combine
 // leaves within aligned 5^L0 groups (bc >= 5^L0 keeps every such level in
 // the groups-of-five regime).
-__device__ __forceinline__ float leaf_abs_dev(const float* d, const float* P,
-                                              const SeriesShape& sh, size_t k) {
-  if (sh.nbeams == 1 && sh.bc <= (1u << 24)) {
-    // one beam with exact float positions: the reblocked index is k itself
+
+// Leaf k of the beam's pyramid, |x - baseline|, from x = the sample at the
+// leaf's reblocked position, already loaded.  `direct`: one beam with exact
+// float positions, where the reblocked position is k itself.
+__device__ __forceinline__ float leaf_abs_dev_of(float x, const float* P, const SeriesShape& sh,
+                                                 size_t k, bool direct) {
+  if (direct) {
     const float bl = (sh.sc == 0) ? 0.f : Stretch{sh.step_bl_last}(P, (unsigned)k);
-    return fabsf(d[k] - bl);
+    return fabsf(x - bl);
   }
   const size_t pos = reblock_pos((unsigned)k, sh.bc, sh.rms_delta);
-  return fabsf(d[pos] - baseline_at(P, sh, pos));
+  return fabsf(x - baseline_at(P, sh, pos));
 }
 
 __device__ __forceinline__ int beam_rms_levels(size_t bc) {
@@ -175,8 +178,21 @@ __global__ __launch_bounds__(NB_BS) void k_beam_rms_partials(const float* __rest
     P = sP;
   }
   const size_t k0 = (size_t)b * sh.bc + c * chunk;
-#pragma unroll 4
-  for (size_t i = threadIdx.x; i < chunk; i += NB_BS) A[i] = leaf_abs_dev(d, P, sh, k0 + i);
+  // every load of the thread's leaves in flight before the first use (a
+  // chunk is at most 5^5 = 3125 leaves, 13 per thread)
+  constexpr int NLEAF = (3125 + NB_BS - 1) / NB_BS;
+  const bool direct = sh.nbeams == 1 && sh.bc <= (1u << 24);
+  float x[NLEAF];
+#pragma unroll
+  for (int u = 0; u < NLEAF; ++u) {
+    const size_t i = threadIdx.x + (size_t)u * NB_BS;
+    if (i < chunk) x[u] = d[direct ? k0 + i : reblock_pos((unsigned)(k0 + i), sh.bc, sh.rms_delta)];
+  }
+#pragma unroll
+  for (int u = 0; u < NLEAF; ++u) {
+    const size_t i = threadIdx.x + (size_t)u * NB_BS;
+    if (i < chunk) A[i] = leaf_abs_dev_of(x[u], P, sh, k0 + i, direct);
+  }
   float* res = lds_pyramid<NB_BS>(A, B, (int)chunk, L0);
   if (threadIdx.x == 0) partial[(row * nbm + b) * np_max + c] = res[0];
 }
