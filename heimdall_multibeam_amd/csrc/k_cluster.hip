@@ -54,6 +54,33 @@ __global__ void k_bitonic_step(uint64_t* keys, size_t npad, size_t j, size_t kk)
   }
 }
 
+// The whole network of k_bitonic_step launches in one block, for up to
+// kBitonicBlock keys held in LDS: the same compare-exchange steps (each pair
+// i < i^j handled by one thread, a barrier between steps), so the same
+// sorted order, in one launch instead of log2(npad)(log2(npad)+1)/2.
+constexpr unsigned kBitonicBlock = 4096;
+__global__ __launch_bounds__(1024) void k_bitonic_block(uint64_t* keys, unsigned npad) {
+  __shared__ uint64_t K[kBitonicBlock];
+  for (unsigned i = threadIdx.x; i < npad; i += 1024) K[i] = keys[i];
+  __syncthreads();
+  for (unsigned kk = 2; kk <= npad; kk <<= 1)
+    for (unsigned j = kk >> 1; j > 0; j >>= 1) {
+      for (unsigned i = threadIdx.x; i < npad; i += 1024) {
+        const unsigned l = i ^ j;
+        if (l > i) {
+          const uint64_t a = K[i], b = K[l];
+          const bool up = (i & kk) == 0;
+          if ((a > b) == up) {
+            K[i] = b;
+            K[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (unsigned i = threadIdx.x; i < npad; i += 1024) keys[i] = K[i];
+}
+
 __device__ __forceinline__ unsigned lower_bound_ind(const uint64_t* keys, unsigned n,
                                                     uint64_t v) {
   unsigned lo = 0, hi = n;
@@ -465,10 +492,14 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     hipLaunchKernelGGL(k_sort_init, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s, ind,
                        group, gd, n, cb.sort_keys, npad);
   }
-  for (size_t kk = 2; kk <= npad; kk <<= 1)
-    for (size_t j = kk >> 1; j > 0; j >>= 1)
-      hipLaunchKernelGGL(k_bitonic_step, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s,
-                         cb.sort_keys, npad, j, kk);
+  if (npad <= kBitonicBlock) {
+    hipLaunchKernelGGL(k_bitonic_block, dim3(1), dim3(1024), 0, s, cb.sort_keys, (unsigned)npad);
+  } else {
+    for (size_t kk = 2; kk <= npad; kk <<= 1)
+      for (size_t j = kk >> 1; j > 0; j >>= 1)
+        hipLaunchKernelGGL(k_bitonic_step, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s,
+                           cb.sort_keys, npad, j, kk);
+  }
   if (banded) {
     hipLaunchKernelGGL(k_windows_banded, dim3((n + T - 1) / T), dim3(T), 0, s, ind, filter,
                        group, gd, n, cb.sort_keys, time_tol, (unsigned)filter_tol, max_filter,
