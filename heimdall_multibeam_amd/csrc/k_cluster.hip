@@ -421,6 +421,51 @@ __global__ __launch_bounds__(256) void k_relax_lists(unsigned n,
 
 __global__ void k_clear_flag(unsigned* flag) { *flag = 0u; }
 
+// The same relaxation run to its fixed point inside one 1024-thread block,
+// for up to kRelaxBlock giants: the passes repeat (each the update of
+// k_relax_lists for every giant) until one changes no label, with a
+// block-wide barrier between passes instead of a host check of the flag.
+// Labels only decrease, so the loop ends; the fixed point (each giant
+// labelled with its component's smallest index) does not depend on the
+// order of the updates.  Every wave runs the same number of rounds: the
+// bound is a multiple of the 128 giants a round covers.
+constexpr unsigned kRelaxBlock = 16384;
+__global__ __launch_bounds__(1024) void k_relax_lists_block(unsigned n,
+                                                            const uint32_t* __restrict__ eoff,
+                                                            const uint32_t* __restrict__ adj,
+                                                            int32_t* labels) {
+  constexpr unsigned kPer = 1024 / kListLanes;  // giants per round
+  const unsigned sub = threadIdx.x % kListLanes;
+  const unsigned npad = (n + kPer - 1) / kPer * kPer;
+  for (int pass = 0; pass < 100000; ++pass) {
+    bool changed = false;
+    for (unsigned i = threadIdx.x / kListLanes; i < npad; i += kPer) {
+      const bool live = i < n;
+      const int32_t cur = live ? __atomic_load_n(&labels[i], __ATOMIC_RELAXED) : 0x7fffffff;
+      int32_t best = cur;
+      if (live && sub == 0) {
+        const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
+        if (jump < best) best = jump;
+      }
+      const unsigned p1 = live ? eoff[i + 1] : 0u;
+      for (unsigned p = (live ? eoff[i] : 0u) + sub; p < p1; p += kListLanes) {
+        const int32_t lj = __atomic_load_n(&labels[adj[p]], __ATOMIC_RELAXED);
+        if (lj < best) best = lj;
+      }
+#pragma unroll
+      for (int o = kListLanes / 2; o > 0; o >>= 1) {
+        const int32_t other = __shfl_xor(best, o);
+        best = other < best ? other : best;
+      }
+      if (live && sub == 0 && best < cur) {
+        __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
+        changed = true;
+      }
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+}
+
 __global__ void k_merge_init(unsigned n, const int32_t* __restrict__ labels, uint64_t* best,
                              unsigned* members, unsigned* root_flag) {
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -520,6 +565,7 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
   // (at most kMaxEdges entries; beyond that the passes scan the windows)
   constexpr size_t kMaxEdges = size_t(1) << 26;
   uint32_t* adj = nullptr;
+  size_t n_edges = 0;
   if (banded && cb.ecount && cb.eoff && cb.adj_alloc) {
     hipLaunchKernelGGL(k_edges_banded_wave<0>, dim3((n + 3) / 4), dim3(256), 0, s, ind, filter,
                        dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi, time_tol,
@@ -533,6 +579,7 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     const size_t edges = *h_flag_pinned;
+    n_edges = edges;
     if (edges <= kMaxEdges) adj = cb.adj_alloc(edges);
     if (!adj) (void)hipGetLastError();  // a failed allocation leaves the windows path
     if (adj)
@@ -540,6 +587,14 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
                          filter, dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi,
                          time_tol, filter_tol, dm_tol, nsamps_beam, group, gd, cb.ecount,
                          cb.eoff, adj);
+  }
+  // one block for small searches (lists of up to 2^20 entries: about a
+  // thousand reads per thread and pass)
+  if (adj && n <= kRelaxBlock && n_edges <= (size_t(1) << 20)) {
+    hipLaunchKernelGGL(k_relax_lists_block, dim3(1), dim3(1024), 0, s, n, cb.eoff, adj,
+                       cb.labels);
+    *iterations = 0;  // passes are counted inside the block only
+    return hipGetLastError();
   }
   // relaxation passes run in batches between host checks of the change flag
   // (list passes are cheap next to a check, window passes are not)
