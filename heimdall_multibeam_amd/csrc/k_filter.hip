@@ -522,17 +522,28 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       for (int q = 0; q < TH_PER; ++q) wcnt[q][wave] = (unsigned)__popcll(bq[q]);
     }
     __syncthreads();
-    if (tid == 0) {
-      unsigned run = 0;
-      for (int q = 0; q < TH_PER; ++q)
-        for (int w = 0; w < FB_BS / 64; ++w) {
-          const unsigned n = wcnt[q][w];
-          wcnt[q][w] = run;
-          run += n;
-        }
-      sbase = atomicAdd(gb.counter, run);
-      gb.ccount[key] = run;
-      gb.coff[key] = sbase;
+    if (wave == 0) {
+      // exclusive prefix of the 128 counts in (q, wave) order: lane l takes
+      // entries 2l and 2l+1, the lanes' pair sums are scanned across the
+      // wave, and lane 0 books the total
+      static_assert(TH_PER * (FB_BS / 64) == 128, "two counts per lane");
+      unsigned* e = &wcnt[0][0];
+      const unsigned a = e[2 * lane], b = e[2 * lane + 1];
+      unsigned incl = a + b;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+      }
+      const unsigned excl = incl - (a + b);
+      e[2 * lane] = excl;
+      e[2 * lane + 1] = excl + a;
+      const unsigned total = __shfl(incl, 63);
+      if (lane == 0) {
+        sbase = atomicAdd(gb.counter, total);
+        gb.ccount[key] = total;
+        gb.coff[key] = sbase;
+      }
     }
     __syncthreads();
     const size_t base = sbase;
