@@ -54,22 +54,29 @@ __global__ void k_bitonic_step(uint64_t* keys, size_t npad, size_t j, size_t kk)
   }
 }
 
-// The whole network of k_bitonic_step launches in one block, for up to
-// kBitonicBlock keys held in LDS: the same compare-exchange steps (each pair
-// i < i^j handled by one thread, a barrier between steps), so the same
-// sorted order, in one launch instead of log2(npad)(log2(npad)+1)/2.
+// Steps j < kBitonicBlock of the k_bitonic_step network on tiles of
+// kBitonicBlock keys held in LDS, one block per tile.  kk_lo == 0: every
+// stage kk up to the tile size (npad itself when smaller), which sorts each
+// tile up or down as the network would; otherwise the steps j =
+// kBitonicBlock/2 .. 1 of stage kk_lo, whose pairs i, i^j never leave a
+// tile.  Same compare-exchanges (each pair handled by one thread, a barrier
+// between steps) and directions (from the global index), so the same order.
 constexpr unsigned kBitonicBlock = 4096;
-__global__ __launch_bounds__(1024) void k_bitonic_block(uint64_t* keys, unsigned npad) {
+__global__ __launch_bounds__(1024) void k_bitonic_tile(uint64_t* keys, unsigned npad,
+                                                       unsigned kk_lo) {
   __shared__ uint64_t K[kBitonicBlock];
-  for (unsigned i = threadIdx.x; i < npad; i += 1024) K[i] = keys[i];
+  const unsigned base = blockIdx.x * kBitonicBlock;
+  const unsigned nt = npad < kBitonicBlock ? npad : kBitonicBlock;
+  for (unsigned i = threadIdx.x; i < nt; i += 1024) K[i] = keys[base + i];
   __syncthreads();
-  for (unsigned kk = 2; kk <= npad; kk <<= 1)
-    for (unsigned j = kk >> 1; j > 0; j >>= 1) {
-      for (unsigned i = threadIdx.x; i < npad; i += 1024) {
+  const unsigned k_first = kk_lo ? kk_lo : 2, k_last = kk_lo ? kk_lo : nt;
+  for (unsigned kk = k_first; kk <= k_last; kk <<= 1)
+    for (unsigned j = (kk_lo ? nt : kk) >> 1; j > 0; j >>= 1) {
+      for (unsigned i = threadIdx.x; i < nt; i += 1024) {
         const unsigned l = i ^ j;
         if (l > i) {
           const uint64_t a = K[i], b = K[l];
-          const bool up = (i & kk) == 0;
+          const bool up = ((base + i) & kk) == 0;
           if ((a > b) == up) {
             K[i] = b;
             K[l] = a;
@@ -78,7 +85,7 @@ __global__ __launch_bounds__(1024) void k_bitonic_block(uint64_t* keys, unsigned
       }
       __syncthreads();
     }
-  for (unsigned i = threadIdx.x; i < npad; i += 1024) keys[i] = K[i];
+  for (unsigned i = threadIdx.x; i < nt; i += 1024) keys[base + i] = K[i];
 }
 
 __device__ __forceinline__ unsigned lower_bound_ind(const uint64_t* keys, unsigned n,
@@ -537,8 +544,19 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     hipLaunchKernelGGL(k_sort_init, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s, ind,
                        group, gd, n, cb.sort_keys, npad);
   }
-  if (npad <= kBitonicBlock) {
-    hipLaunchKernelGGL(k_bitonic_block, dim3(1), dim3(1024), 0, s, cb.sort_keys, (unsigned)npad);
+  // the bitonic network: tiles of kBitonicBlock keys sorted in LDS, then per
+  // later stage the steps across tiles one launch each and the rest in LDS
+  if (npad <= (size_t(1) << 31)) {
+    const unsigned ntile = (unsigned)((npad + kBitonicBlock - 1) / kBitonicBlock);
+    hipLaunchKernelGGL(k_bitonic_tile, dim3(ntile), dim3(1024), 0, s, cb.sort_keys,
+                       (unsigned)npad, 0u);
+    for (size_t kk = 2 * kBitonicBlock; kk <= npad; kk <<= 1) {
+      for (size_t j = kk >> 1; j >= kBitonicBlock; j >>= 1)
+        hipLaunchKernelGGL(k_bitonic_step, dim3((unsigned)((npad + T - 1) / T)), dim3(T), 0, s,
+                           cb.sort_keys, npad, j, kk);
+      hipLaunchKernelGGL(k_bitonic_tile, dim3(ntile), dim3(1024), 0, s, cb.sort_keys,
+                         (unsigned)npad, (unsigned)kk);
+    }
   } else {
     for (size_t kk = 2; kk <= npad; kk <<= 1)
       for (size_t j = kk >> 1; j > 0; j >>= 1)
