@@ -394,6 +394,21 @@ __global__ __launch_bounds__(256) void k_edges_banded_wave(
   if (MODE == 0 && lane == 0) ecount[i] = count;
 }
 
+// Pointer jumping: from label cur, follows labels[] while it keeps
+// decreasing (up to 4 steps).  Every label on the way belongs to the same
+// component, so the fixed point (its smallest index) is unchanged; the
+// passes to reach it are fewer.
+__device__ __forceinline__ int32_t follow_labels(const int32_t* labels, int32_t cur) {
+  int32_t r = cur;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int32_t nx = __atomic_load_n(&labels[r], __ATOMIC_RELAXED);
+    if (nx >= r) break;
+    r = nx;
+  }
+  return r;
+}
+
 // The relaxation over the coincidence lists, 8 lanes per giant.
 constexpr int kListLanes = 8;
 __global__ __launch_bounds__(256) void k_relax_lists(unsigned n,
@@ -406,10 +421,7 @@ __global__ __launch_bounds__(256) void k_relax_lists(unsigned n,
   const bool live = i < n;
   const int32_t cur = live ? __atomic_load_n(&labels[i], __ATOMIC_RELAXED) : 0x7fffffff;
   int32_t best = cur;
-  if (live && sub == 0) {
-    const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
-    if (jump < best) best = jump;
-  }
+  if (live && sub == 0) best = follow_labels(labels, cur);
   const unsigned p1 = live ? eoff[i + 1] : 0u;
   for (unsigned p = (live ? eoff[i] : 0u) + sub; p < p1; p += kListLanes) {
     const int32_t lj = __atomic_load_n(&labels[adj[p]], __ATOMIC_RELAXED);
@@ -450,10 +462,7 @@ __global__ __launch_bounds__(1024) void k_relax_lists_block(unsigned n,
       const bool live = i < n;
       const int32_t cur = live ? __atomic_load_n(&labels[i], __ATOMIC_RELAXED) : 0x7fffffff;
       int32_t best = cur;
-      if (live && sub == 0) {
-        const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
-        if (jump < best) best = jump;
-      }
+      if (live && sub == 0) best = follow_labels(labels, cur);
       const unsigned p1 = live ? eoff[i + 1] : 0u;
       for (unsigned p = (live ? eoff[i] : 0u) + sub; p < p1; p += kListLanes) {
         const int32_t lj = __atomic_load_n(&labels[adj[p]], __ATOMIC_RELAXED);
