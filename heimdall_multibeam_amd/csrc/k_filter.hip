@@ -325,6 +325,8 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   extern __shared__ float4 T4[];  // TH_CHUNK + amax + bmax prefix sums (+ alignment)
   __shared__ unsigned wcnt[TH_PER][FB_BS / 64];
   __shared__ unsigned sbase;
+  __shared__ unsigned wmask;  // widths with an output above the threshold in the chunk
+  static_assert(kMaxWidths <= 32, "one bit per width");
   const size_t c = blockIdx.x;
   const size_t row = (size_t)blockIdx.z * gridDim.y + blockIdx.y;
   const size_t stride = gd[blockIdx.z].sh.stride;
@@ -378,6 +380,7 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       if (v0 + u * FB_BS < n4) T4[v0 + u * FB_BS] = r[u];
   }
   const float* T = reinterpret_cast<const float*>(T4) + sh;
+  if (threadIdx.x == 0) wmask = 0u;
   __syncthreads();
   const unsigned tid = threadIdx.x;
   const int lane = (int)(tid & 63), wave = (int)(tid >> 6);
@@ -492,9 +495,11 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   // Screening pass without barriers: most chunks hold no output above the
   // threshold at any width, and then every key of the chunk is empty.
   {
-    bool hit = false;
-    for (int j = 0; j < wt.nw; ++j) hit |= screen_width(j);
-    if (!__syncthreads_or(hit)) {
+    unsigned hw = 0;  // this thread's widths with a hit (nw <= 24 bits)
+    for (int j = 0; j < wt.nw; ++j)
+      if (screen_width(j)) hw |= 1u << j;
+    if (hw) atomicOr(&wmask, hw);
+    if (!__syncthreads_or(hw != 0)) {
       for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
         gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
       return;
@@ -505,7 +510,9 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     float vals[TH_PER];
     unsigned mask;
     size_t i0;
-    if (!eval_width(j, vals, mask, i0)) {
+    // widths the screening found without a hit are empty here (wmask is the
+    // same for the whole block, so every thread skips together)
+    if (!(wmask & (1u << j)) || !eval_width(j, vals, mask, i0)) {
       if (tid == 0) gb.ccount[key] = 0;
       continue;
     }
