@@ -516,11 +516,9 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       if (tid == 0) gb.ccount[key] = 0;
       continue;
     }
-    const int any = __syncthreads_or(mask != 0);
-    if (!any) {
-      if (tid == 0) gb.ccount[key] = 0;
-      continue;
-    }
+    // a flagged width has a hit in some thread (the screening's test is the
+    // same as mask != 0), so no block-wide test here; the previous width's
+    // last barrier already guards wcnt and sbase
     uint64_t bq[TH_PER];
 #pragma unroll
     for (int q = 0; q < TH_PER; ++q) bq[q] = __ballot((mask >> q) & 1u);
