@@ -266,6 +266,8 @@ struct ClusterBuffers {
 };
 // With group (per giant) and gd, giants of different groups never coincide
 // and each group uses its own beam length; otherwise nsamps_beam applies.
+// *iterations: relaxation passes run between host checks of the change flag
+// (0 when a small search ran its passes inside one block).
 hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uint32_t* dm,
                           const uint64_t* members_in, unsigned n, uint64_t time_tol,
                           uint64_t filter_tol, uint64_t dm_tol, uint64_t nsamps_beam,
