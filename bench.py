@@ -24,7 +24,7 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 VALU_INT_PEAK = 256 * 64 * 2.4e9  # int32 lane adds/s (256 CUs x 64 lanes x 2.4 GHz)
 # LDS read rate of the dedispersion inner loop's instruction (ds_read2st64_b32),
-# measured by heimdall_multibeam_amd/bin/probe_lds: 98.9 B/clk/CU at 2.4 GHz
+# measured by a standalone LDS probe (profiles/lds_read_bandwidth_probe.txt): 98.9 B/clk/CU at 2.4 GHz
 LDS_READ2_PEAK = 98.9 * 256 * 2.4e9  # bytes/s
 
 

@@ -4,7 +4,6 @@
   libhd_synth.so         synthetic filterbank generator, g++/OpenMP
   bin/heimdall           gulp driver over sigproc files (app/heimdall_main.cpp)
   bin/generate_dmlist    DM trial list tool (app/generate_dmlist.cpp)
-  bin/probe_lds          LDS read bandwidth by access shape (app/probe_lds.hip)
 
 Every HIP source is compiled for gfx950 only, with -ffp-contract=off (the
 reference performs no fused multiply-adds, and results are compared bit for
@@ -69,10 +68,6 @@ def build(verbose=False):
               os.path.join(HERE, "app", src), "-o", out + ".tmp", "-L" + HERE,
               "-lheimdall_mi355x", "-Wl,-rpath,$ORIGIN/.."])
         os.replace(out + ".tmp", out)
-    out = os.path.join(BIN, "probe_lds")
-    _run([HIPCC] + HIP_FLAGS + ["-x", "hip", os.path.join(HERE, "app", "probe_lds.hip"),
-                                "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB, SYNTH, BIN)
     return LIB
