@@ -1598,12 +1598,14 @@ extern "C" hd_error hd_create_pipeline(hd_pipeline* pipeline_, hd_params params)
     pl->use_graph = g ? atoi(g) != 0 : true;
   }
   DeviceGuard dg(pl->device);
-  if (params.verbosity >= 1) {
+  if (params.verbosity >= 1)
     std::cout << "Process " << params.beam << " using GPU " << params.gpu_id << std::endl;
-    if (params.use_scrunching)
-      std::cout << "Adaptive time scrunching is not supported; searching at full resolution"
-                << std::endl;
-  }
+  // The reference enables dedisp's adaptive time scrunching by default
+  // (default_params.C:34, pipeline.cu:187-196); this search always runs at
+  // full time resolution, so say so whatever the verbosity.
+  if (params.use_scrunching)
+    std::cerr << "heimdall: adaptive time scrunching is not available; searching every DM "
+                 "trial at full time resolution (-scrunching 0 silences this)" << std::endl;
   hipError_t e = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&pl->h_pinned, 64, hipHostMallocDefault);
   for (int k = 0; k <= ST_COUNT && e == hipSuccess; ++k) e = hipEventCreate(&pl->ev[k]);
@@ -1715,6 +1717,63 @@ extern "C" hd_error hd_execute_batch(hd_pipeline pl, const hd_gulp_desc* gulps, 
     if (first == HD_NO_ERROR && go.err != HD_NO_ERROR) first = go.err;
   }
   return first;
+}
+
+// dedisp_execute_adv(plan, nsamps, in, in_nbits, in_stride, out, out_nbits,
+// out_stride, flags) as called at Pipeline/pipeline.cu:452-462, with the
+// pipeline as the plan: its trial list, delay table and killmask.  The
+// unpack and dedispersion kernels are the search's own (same tile choice).
+extern "C" hd_error hd_dedisperse(hd_pipeline pl, hd_size nsamps, const hd_byte* d_in,
+                                  hd_size in_nbits, hd_size in_stride, hd_byte* d_out,
+                                  hd_size out_nbits, hd_size out_stride, unsigned flags) {
+  (void)flags;  // pointers are always device memory here
+  if (!pl) return HD_INVALID_PIPELINE;
+  if (!d_in || !d_out) return HD_INVALID_POINTER;
+  if (!(in_nbits == 1 || in_nbits == 2 || in_nbits == 4 || in_nbits == 8)) return HD_INVALID_NBITS;
+  if (out_nbits != 32) return HD_INVALID_NBITS;  // float output only (pipeline.cu:544-550)
+  const hd_params& p = pl->params;
+  if (in_stride != p.nchans * in_nbits / 8 || (p.nchans * in_nbits) % 8) return HD_INVALID_STRIDE;
+  if (nsamps <= pl->max_delay) return HD_TOO_FEW_NSAMPS;
+  const size_t nout = nsamps - pl->max_delay;
+  if (out_stride < nout * sizeof(float) || out_stride % sizeof(float)) return HD_INVALID_STRIDE;
+  DeviceGuard dgd(pl->device);
+  (void)hipGetLastError();
+  hd_error err = build_live_and_delays(pl);
+  if (err != HD_NO_ERROR) return err;
+  hipStream_t s = pl->stream;
+  const int n_dm = (int)pl->dm_list.size();
+  const int n_live = (int)pl->live_list.size();
+  const size_t ppitch = round_up(nout + 1, (size_t)hd::dedisp_tile_time());
+  const size_t xpitch = round_up(ppitch + pl->max_delay + 2 * pl->wlen + 256, 256);
+  HD_CHECK(pl->x.ensure(std::max(n_live, 1) * xpitch));
+  HD_CHECK(pl->plane.ensure((size_t)n_dm * ppitch * sizeof(float)));
+  if (n_live > 0)
+    HD_CHECK(hd::launch_unpack(d_in, in_stride, (int)in_nbits, nsamps, pl->live.as<int>(), n_live,
+                               pl->x.as<uint8_t>(), xpitch, s, hd::GroupArgs{},
+                               (size_t)n_live == p.nchans));
+  else
+    HD_CHECK(hipMemsetAsync(pl->x.p, 0, xpitch, s));
+  hd::DedispGeom dg;
+  std::memset(&dg, 0, sizeof(dg));
+  dg.n_dm = n_dm;
+  dg.n_live = std::max(n_live, 1);  // no live channel: one zero row sums to 0
+  dg.nout = nout;
+  dg.xpitch = xpitch;
+  dg.ppitch = ppitch;
+  dg.wlen = (int)pl->wlen;
+  const int* pairs = nullptr;
+  if (n_live == 0) {
+    HD_CHECK(hipMemsetAsync(pl->plane.p, 0, (size_t)n_dm * ppitch * sizeof(float), s));
+  } else {
+    err = choose_dedisp_tile(pl, dg, hd::GroupArgs{}, in_nbits, s, &pairs);
+    if (err != HD_NO_ERROR) return err;
+    HD_CHECK(hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pairs, dg,
+                                   pl->plane.as<float>(), s));
+  }
+  HD_CHECK(hipMemcpy2DAsync(d_out, out_stride, pl->plane.p, ppitch * sizeof(float),
+                            nout * sizeof(float), (size_t)n_dm, hipMemcpyDeviceToDevice, s));
+  HD_CHECK(hipStreamSynchronize(s));
+  return HD_NO_ERROR;
 }
 
 extern "C" hd_size hd_get_last_candidate_gulps(hd_pipeline pl, hd_size* gulp_index, hd_size cap) {

@@ -18,6 +18,8 @@
 // group, ordered by root; its fields come from the first (lowest index) giant
 // with the maximum S/N and its member count is the sum.  A 64-bit atomic max
 // over (orderable S/N, ~index) picks that giant.
+#include <algorithm>
+
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
 
@@ -395,9 +397,13 @@ __global__ __launch_bounds__(256) void k_edges_banded_wave(
 }
 
 // Pointer jumping: from label cur, follows labels[] while it keeps
-// decreasing (up to 4 steps).  Every label on the way belongs to the same
-// component, so the fixed point (its smallest index) is unchanged; the
-// passes to reach it are fewer.
+// decreasing (up to 4 steps).  Every label is an index reachable from its
+// giant along a chain of coincident giants with decreasing indices, and so
+// is every label on the way (chains compose), so taking the smallest of them
+// never goes below the fixed point: the smallest index reachable along such
+// decreasing chains (the single-thread result, which can leave a connected
+// group split in two; this is not a union-find).  It only shortens the
+// passes needed to reach it.
 __device__ __forceinline__ int32_t follow_labels(const int32_t* labels, int32_t cur) {
   int32_t r = cur;
 #pragma unroll
@@ -445,10 +451,15 @@ __global__ void k_clear_flag(unsigned* flag) { *flag = 0u; }
 // k_relax_lists for every giant) until one changes no label, with a
 // block-wide barrier between passes instead of a host check of the flag.
 // Labels only decrease, so the loop ends; the fixed point (each giant
-// labelled with its component's smallest index) does not depend on the
-// order of the updates.  Every wave runs the same number of rounds: the
-// bound is a multiple of the 128 giants a round covers.
+// labelled with the smallest index reachable along decreasing coincident
+// chains) does not depend on the order of the updates.  After pass k every
+// giant whose longest decreasing chain has at most k links holds its final
+// label, so at most n + 1 passes run (the last one changes nothing): the
+// pass bound below is never what ends the loop.  Every wave runs the same
+// number of rounds: the bound is a multiple of the 128 giants a round covers.
 constexpr unsigned kRelaxBlock = 16384;
+constexpr int kRelaxBlockPasses = 100000;
+static_assert(kRelaxBlock + 2 <= (unsigned)kRelaxBlockPasses, "pass bound below n + 2");
 __global__ __launch_bounds__(1024) void k_relax_lists_block(unsigned n,
                                                             const uint32_t* __restrict__ eoff,
                                                             const uint32_t* __restrict__ adj,
@@ -456,7 +467,7 @@ __global__ __launch_bounds__(1024) void k_relax_lists_block(unsigned n,
   constexpr unsigned kPer = 1024 / kListLanes;  // giants per round
   const unsigned sub = threadIdx.x % kListLanes;
   const unsigned npad = (n + kPer - 1) / kPer * kPer;
-  for (int pass = 0; pass < 100000; ++pass) {
+  for (int pass = 0; pass < kRelaxBlockPasses; ++pass) {
     bool changed = false;
     for (unsigned i = threadIdx.x / kListLanes; i < npad; i += kPer) {
       const bool live = i < n;
@@ -624,9 +635,13 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     return hipGetLastError();
   }
   // relaxation passes run in batches between host checks of the change flag
-  // (list passes are cheap next to a check, window passes are not)
+  // (list passes are cheap next to a check, window passes are not).  At most
+  // n + 1 passes are needed (see k_relax_lists_block), so a flag still set
+  // after n + 2 passes means the labels are wrong: that is reported.
   const int kBatch = adj ? 2 : 4;
-  for (int it = 0; it < 100000; it += kBatch) {
+  const size_t max_passes = (size_t)n + 2;
+  bool converged = false;
+  for (size_t it = 0; it < max_passes; it += kBatch) {
     hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, cb.flag);
     for (int b = 0; b < kBatch; ++b) {
       if (adj)
@@ -649,9 +664,13 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    *iterations = it + kBatch;
-    if (*h_flag_pinned == 0) break;
+    *iterations = (int)std::min<size_t>(it + kBatch, 0x7fffffff);
+    if (*h_flag_pinned == 0) {
+      converged = true;
+      break;
+    }
   }
+  if (!converged) return hipErrorUnknown;  // cannot happen: see the pass bound above
   return hipGetLastError();
 }
 

@@ -238,7 +238,9 @@ extern "C" hd_error hd_remove_baseline(hd_float* d_data, hd_size beam_stride, hd
   SeriesShape sh;
   if (!baseline_shape(beam_stride, beam_count, smooth_radius, nbeams, series_stride, sh))
     return HD_INVALID_STRIDE;
-  if (series_stride < sh.nout) return HD_INVALID_STRIDE;
+  // the last beam's stretch may read d[nout] (remove_baseline.cu:74-75 with
+  // median_filter.cu:460-462): it must lie inside the series' stride
+  if (series_stride < sh.nout + 1) return HD_INVALID_STRIDE;
   if (sh.sc == 0 || n_series == 0) return HD_NO_ERROR;  // remove_baseline.cu:39-42
   Scratch tmp(s);
   float *pts, *ones;

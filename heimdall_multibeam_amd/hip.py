@@ -62,6 +62,23 @@ class DeviceBuffer:
         check(hip().hipMemcpy(out.ctypes.data, self.ptr, n, D2H), "hipMemcpy D2H")
         return out
 
+    def upload_at(self, offset, array):
+        """Copies a host array to the buffer starting `offset` bytes in."""
+        import numpy as np
+        a = np.ascontiguousarray(array)
+        assert 0 <= offset and offset + a.nbytes <= self.nbytes
+        check(hip().hipMemcpy(C.c_void_p(self.ptr.value + offset), a.ctypes.data, a.nbytes, H2D),
+              "hipMemcpy H2D")
+
+    def download_at(self, offset, nbytes):
+        """nbytes of the buffer from `offset` on, as a uint8 numpy array."""
+        import numpy as np
+        assert 0 <= offset and offset + nbytes <= self.nbytes
+        out = np.empty(nbytes, np.uint8)
+        check(hip().hipMemcpy(out.ctypes.data, C.c_void_p(self.ptr.value + offset), nbytes, D2H),
+              "hipMemcpy D2H")
+        return out
+
     def free(self):
         if self.ptr:
             hip().hipFree(self.ptr)

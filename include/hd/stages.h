@@ -28,8 +28,27 @@ typedef struct hd_giants {
   hd_size   count;
 } hd_giants;
 
+/* replaces dedisp_execute_adv(plan, nsamps, in, in_nbits, in_stride, out,
+ * out_nbits, out_stride, flags) (called at Pipeline/pipeline.cu:452-462), with
+ * the pipeline as the dedisp plan: its DM list, delay table and killmask
+ * (hd_create_pipeline's params).  d_in: nsamps time samples of in_stride =
+ * nchans*in_nbits/8 bytes in device memory (channels packed low bits first);
+ * d_out: one row of nsamps - max_delay floats per DM trial, out_stride BYTES
+ * apart, each value the exact integer sum over the live channels
+ * (out_nbits must be 32).  flags is accepted for the signature and ignored:
+ * both pointers are device memory.  Runs on the pipeline's stream and
+ * returns when the output is written. */
+hd_error hd_dedisperse(hd_pipeline pl, hd_size nsamps, const hd_byte* d_in, hd_size in_nbits,
+                       hd_size in_stride, hd_byte* d_out, hd_size out_nbits,
+                       hd_size out_stride, unsigned flags);
+
 /* replaces RemoveBaselinePlan::exec (Pipeline/remove_baseline.cu:21-160,
- * hd/remove_baseline.h:19-23) */
+ * hd/remove_baseline.h:19-23).  A series holds nout = beam_stride*nbeams -
+ * (beam_stride - beam_count) samples; the last beam's resampling reads the
+ * element one past them, d_data[nout] of each series, as the reference reads
+ * the float after its series (median_filter.cu:460-462), so series_stride
+ * must be at least nout + 1 and that element readable (the search keeps it
+ * 0). */
 hd_error hd_remove_baseline(hd_float* d_data, hd_size beam_stride, hd_size beam_count,
                             hd_size smooth_radius, hd_size nbeams,
                             hd_size n_series, hd_size series_stride, void* stream);

@@ -143,9 +143,19 @@ int orc_dedisperse(const uint8_t* in, size_t nsamps, size_t nbits, size_t nchans
   for (size_t c = 0; c < nchans; ++c)
     if (!killmask || killmask[c]) live[nlive++] = c;
   uint16_t* x = (uint16_t*)malloc(sizeof(uint16_t) * (nlive ? nlive : 1) * nsamps);
-  for (size_t t = 0; t < nsamps; ++t) {
-    const uint8_t* row = in + t * stride;
-    for (size_t l = 0; l < nlive; ++l) x[l * nsamps + t] = (uint16_t)sample_at(row, live[l], nbits);
+  /* transposed in 64 x 64 blocks (a plain copy, blocked for the caches) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads < 1 ? 1 : nthreads)
+#endif
+  for (long tb = 0; tb < (long)((nsamps + 63) / 64); ++tb) {
+    const size_t t0 = (size_t)tb * 64, t1 = (t0 + 64 < nsamps) ? t0 + 64 : nsamps;
+    for (size_t l0 = 0; l0 < nlive; l0 += 64) {
+      const size_t l1 = (l0 + 64 < nlive) ? l0 + 64 : nlive;
+      for (size_t t = t0; t < t1; ++t) {
+        const uint8_t* row = in + t * stride;
+        for (size_t l = l0; l < l1; ++l) x[l * nsamps + t] = (uint16_t)sample_at(row, live[l], nbits);
+      }
+    }
   }
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
@@ -157,11 +167,18 @@ int orc_dedisperse(const uint8_t* in, size_t nsamps, size_t nbits, size_t nchans
 #pragma omp for schedule(dynamic, 1)
 #endif
     for (long d = 0; d < (long)n_dm; ++d) {
-      memset(acc, 0, sizeof(uint32_t) * nout);
-      for (size_t l = 0; l < nlive; ++l) {
-        int off = orc_delay(dm_list[d], delay_table[live[l]]);
-        const uint16_t* xr = x + l * nsamps + off;
-        for (size_t t = 0; t < nout; ++t) acc[t] += xr[t];
+      /* integer sums: the order of the additions does not matter, so time
+       * is walked in tiles that keep the accumulators in the L1 cache */
+      enum { TT = 2048 };
+      for (size_t t0 = 0; t0 < nout; t0 += TT) {
+        const size_t tn = (nout - t0 < TT) ? nout - t0 : TT;
+        uint32_t* a = acc + t0;
+        memset(a, 0, sizeof(uint32_t) * tn);
+        for (size_t l = 0; l < nlive; ++l) {
+          int off = orc_delay(dm_list[d], delay_table[live[l]]);
+          const uint16_t* xr = x + l * nsamps + off + t0;
+          for (size_t t = 0; t < tn; ++t) a[t] += xr[t];
+        }
       }
       float* o = out + (size_t)d * out_stride;
       for (size_t t = 0; t < nout; ++t) o[t] = (float)acc[t];

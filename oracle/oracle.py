@@ -204,6 +204,19 @@ def get_rms_multibeam(d, beam_stride, beam_count, nbeams):
     return rms
 
 
+def normalise_multibeam(d, rms, beam_stride, cur_nsamps):
+    d = np.array(d, np.float32)
+    rms = np.ascontiguousarray(rms, np.float32)
+    lib().orc_normalise_multibeam(_fp(d), _fp(rms), beam_stride, cur_nsamps)
+    return d
+
+
+def normalise_block(d, rms, count):
+    d = np.array(d, np.float32)
+    lib().orc_normalise_block(_fp(d), rms, count)
+    return d
+
+
 def block_rms(d, beam_stride, beam_count, nbeams):
     d = np.ascontiguousarray(d, np.float32)
     return np.float32(lib().orc_block_rms(_fp(d), beam_stride, beam_count, nbeams))

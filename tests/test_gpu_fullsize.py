@@ -45,6 +45,29 @@ def test_full_1024ch_gulp_matches_oracle(oracle):
     _compare(gpu, r, oracle, w)
 
 
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_small_512ch_gulp_matches_oracle(oracle, seed):
+    """The 512-channel gulp (65536 samples of 256 us, 320 trials to DM 500,
+    boxcars 1..256, 10 pulses): one hd_execute call per seed."""
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import synth
+    w = synth.CONFIGS["small_512ch"]
+    fb = synth.make_gulp(w, seed)
+    print("case small_512ch seed %d" % seed, flush=True)
+    pl = H.Pipeline(params_for(w))
+    pl.set_output(False)
+    err, processed = pl.execute(fb, w.nsamps, w.nbits)
+    gpu = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
+               text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
+    pl.close()
+    r = oracle.execute(oracle_params_for(oracle, w, nthreads=_threads()), fb, w.nsamps,
+                       w.nbits, 0, 1)
+    assert err == r.error == H.HD_NO_ERROR
+    assert len(gpu["dms"]) == 320 and gpu["max_delay"] == 1638
+    assert len(r.groups) > 5
+    _compare(gpu, r, oracle, w)
+
+
 def test_full_ragged_batch_matches_oracle(oracle):
     from heimdall_multibeam_amd import synth
     w = synth.CONFIGS["batch_320ch"]

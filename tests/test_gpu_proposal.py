@@ -255,3 +255,68 @@ def test_gated_search_partial_is_exact(H, case):
     assert 0 < sum(int(f.sum()) for f in flags_want) < sum(f.size for f in flags_want)
     P.set_proposal(pl, None, 0)
     pl.close()
+
+
+def _learned():
+    from heimdall_multibeam_amd import synth
+    return synth.CONFIGS["learned_1024ch"]
+
+
+def test_coarse_plane_learned_1024ch_matches_oracle(H, oracle):
+    """The proposal's input on one full gulp of the learned-proposal workload
+    (1024 channels, 262144 samples of 64 us, DM 0-2000, 5% tone channels,
+    zero-DM bursts): the coarse and wide planes, bit for bit."""
+    from heimdall_multibeam_amd import proposal as P, synth
+    w = _learned()
+    fb = synth.make_gulp(w, 5)
+    pl = H.Pipeline(params_for(w, min_tscrunch_width=w.min_tscrunch_width))
+    pl.set_output(False)
+    d = _device(fb)
+    print("case coarse plane learned_1024ch seed 5", flush=True)
+    nu, nv = P.coarse(pl, d.address, w.nsamps, 8)
+    got = P.plane(pl)
+    want, nout, wide = oracle_coarse(oracle, w, fb, 8, pl.dm_list())
+    print("coarse plane", got.shape, "units", nu, "tpos", nv, flush=True)
+    assert got.shape == want.shape
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(P.plane(pl, 1), wide)
+    assert nu == (len(pl.dm_list()) + 127) // 128
+    assert nv == (nout + 31) // 32
+    pl.close()
+
+
+def test_gated_search_learned_1024ch_every_unit_equals_full(H, oracle):
+    """On one full gulp of the learned-proposal workload: the full search
+    (the proposal's labeller and recall judge) against the oracle, then the
+    gated search with every unit proposed against the full search, both bit
+    for bit."""
+    import os
+    from heimdall_multibeam_amd import proposal as P, synth
+    from tests._cases import oracle_params_for
+    from tests.test_gpu_parity import _compare
+    w = _learned()
+    fb = synth.make_gulp(w, 6)
+    pl = H.Pipeline(params_for(w, min_tscrunch_width=w.min_tscrunch_width))
+    pl.set_output(False)
+    print("case full search learned_1024ch seed 6", flush=True)
+    err, processed = pl.execute(fb, w.nsamps, 8)
+    full = dict(err=err, processed=processed, giants=pl.giants(), cands=pl.candidates(),
+                text=pl.cand_text(), dms=pl.dm_list(), max_delay=pl.max_delay())
+    full_g = _giants(pl)
+    r = oracle.execute(oracle_params_for(oracle, w, nthreads=min(16, os.cpu_count() or 1),
+                                         min_tscrunch_width=w.min_tscrunch_width),
+                       fb, w.nsamps, w.nbits, 0, 1)
+    assert len(r.groups) > 5
+    _compare(full, r, oracle, w)
+    print("case gated search learned_1024ch every unit", flush=True)
+    P.set_proposal(pl, _random_params(8), -1e30)
+    pl.execute(fb, w.nsamps, 8)
+    flags = P.last_proposal(pl)
+    assert flags.shape == (1, (len(pl.dm_list()) + 127) // 128) and flags.all()
+    gated_c = pl.candidates()
+    for k in full["cands"]:
+        np.testing.assert_array_equal(gated_c[k], full["cands"][k], err_msg=k)
+    np.testing.assert_array_equal(_giants(pl), full_g)
+    assert pl.cand_text() == full["text"]
+    P.set_proposal(pl, None, 0)
+    pl.close()
