@@ -129,6 +129,16 @@ __device__ __forceinline__ uint2 xcd_contiguous_block() {
 // ---------------------------------------------------------------------------
 // Block-wide helpers.  The block size is a compile-time constant BS and
 // every thread calls these.
+//
+// Every helper that takes a pointer into LDS is __forceinline__.  A helper the
+// compiler leaves as a called function receives its LDS buffers as generic
+// pointers and reaches them with flat loads and stores, and in such an
+// outlined copy of lds_pyramid the barrier that opens each pyramid level was
+// emitted with no wait for the previous level's last flat store: waves could
+// read a level before it was complete, and the baseline points of a few
+// trials changed from run to run.  Inlined, the accesses are ds_read/ds_write
+// with their waits before each barrier.  tests/test_build.py checks that no
+// kernel of the library calls a device function.
 
 // Orders one wave's LDS writes before its later reads (the wave's LDS
 // operations execute in order; this keeps the compiler from moving them).
@@ -162,7 +172,7 @@ __device__ __forceinline__ float* wave_pyramid(float* src, float* dst, size_t n,
 // ponging with `dst`; returns a pointer to the buffer holding the result
 // (n / 5^levels values).  Requires n >= 5^levels.
 template <int BS>
-__device__ float* lds_pyramid(float* src, float* dst, int n, int levels) {
+__device__ __forceinline__ float* lds_pyramid(float* src, float* dst, int n, int levels) {
   for (int l = 0; l < levels; ++l) {
     const int m = n / 5;
     __syncthreads();

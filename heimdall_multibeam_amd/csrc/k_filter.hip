@@ -62,7 +62,7 @@ __device__ __forceinline__ float filtered_at(const float* S, const WidthTable& w
 }
 
 template <int BS>
-__device__ float lds_finish_f(float* A, float* B, size_t n) {
+__device__ __forceinline__ float lds_finish_f(float* A, float* B, size_t n) {
   __shared__ float result;
   while (n > 4) {
     const size_t m = n / 5;
@@ -85,7 +85,7 @@ __device__ float lds_finish_f(float* A, float* B, size_t n) {
 // dst[t] = src[t * stride] for t < n (LDS from global), with SU loads in
 // flight per thread before their LDS writes: a plain loop waits out the
 // memory latency once per element.
-__device__ inline void stage_lds(float* dst, const float* __restrict__ src, size_t n,
+__device__ __forceinline__ void stage_lds(float* dst, const float* __restrict__ src, size_t n,
                                  size_t stride) {
   constexpr int SU = 8;
   for (size_t t0 = threadIdx.x; t0 < n; t0 += (size_t)SU * FB_BS) {
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(64) void k_block_rms_finish(const float* __restrict
 // Block-wide exclusive scan of one unsigned per thread; returns the prefix,
 // writes the total to *total.
 template <int BS>
-__device__ unsigned block_exclusive_scan(unsigned v, unsigned* total) {
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* total) {
   // inclusive scan within each wave (shuffles), then the wave totals
   __shared__ unsigned wsum[BS / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

@@ -36,7 +36,7 @@ __device__ __forceinline__ size_t pow5(int L) {
 // loop `for (size = n; size > 1; size /= 5)`: groups of five while size > 4,
 // then the small-count rule.  Returns the final value to every thread.
 template <int BS>
-__device__ float lds_finish(float* A, float* B, size_t n) {
+__device__ __forceinline__ float lds_finish(float* A, float* B, size_t n) {
   __shared__ float result;
   while (n > 4) {
     const size_t m = n / 5;

@@ -85,7 +85,7 @@ __global__ void k_fill(float* p, size_t n, float v) {
 
 // |d| pooled pyramid chunks of a materialised series (get_rms.cu:104-123)
 template <int BS>
-__device__ float* pyr(float* A, float* B, int n, int levels) {
+__device__ __forceinline__ float* pyr(float* A, float* B, int n, int levels) {
   return lds_pyramid<BS>(A, B, n, levels);
 }
 
