@@ -10,21 +10,26 @@
  * double and size_t arithmetic.  Build with -ffp-contract=off: the reference
  * host build performs no fused multiply-adds.
  *
- * Parity pins (tests/test_oracle.py):
+ * Parity pins (tests/test_oracle.py, tests/test_oracle_pins.py):
  *  - orc_generate_dm_list     : all 1434 trials of Share/dm2000.list (print-exact)
+ *  - orc_delay_table/orc_delay : float64 closed form within half a sample,
+ *                                zero at DM 0 / channel 0, monotone, and the
+ *                                max delays of four setups
  *  - orc_median3/4/5           : order statistics over every tie pattern
  *  - orc_linear_stretch        : closed form on affine inputs, endpoints
  *  - orc_dedisperse            : brute-force triple loop, impulse response,
  *                                DM-0 = channel sum, linearity
  *  - orc_scan / orc_boxcar     : float64 closed forms within rounding, brute force
- *  - orc_remove_baseline       : affine and constant series are removed exactly
- *  - orc_get_rms_multibeam     : 1.4862 x median(|x|) pyramid on Gaussian noise
+ *  - orc_remove_baseline       : affine and constant series are removed exactly;
+ *                                a numpy restatement, bit for bit
+ *  - orc_get_rms_multibeam,    : numpy restatement of both pyramids (tails,
+ *    orc_block_rms               float reblock index above 2^24), bit for bit
  *  - orc_find_giants           : brute-force run detection
  *  - orc_label_clusters        : brute-force DAG reachability definition
  *  - orc_merge_candidates      : brute-force group reduction
- * Parity unpinned (no reference vector exists): the dedisp delay rounding and
- * output scaling (the reference's external dedisp library is absent); this
- * oracle and the product share the convention written in orc_delay().
+ * Parity unpinned (no reference vector exists): whether dedisp rounds delays
+ * as int(dm*k + 0.5f) (the closed form fixes the constant and units, not the
+ * rounding rule), its output scaling, and hash32's output values.
  */
 #include "hd_oracle.h"
 
