@@ -8,6 +8,7 @@
 // and a batch of `n_series` of them is one launch.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "hd/stages.h"
@@ -285,12 +286,12 @@ extern "C" hd_error hd_normalise_multibeam(hd_float* d_data, const hd_float* d_r
   return HD_NO_ERROR;
 }
 
-extern "C" hd_error hd_block_rms(const hd_float* d_data, hd_float* d_block_rms,
-                                 hd_size beam_stride, hd_size beam_count, hd_size nbeams,
-                                 hd_size n_series, hd_size series_stride, void* stream) {
-  if (!d_data || !d_block_rms) return HD_INVALID_POINTER;
-  if (!nbeams || !beam_count || beam_count > beam_stride) return HD_INVALID_STRIDE;
-  hipStream_t s = (hipStream_t)stream;
+namespace hd {
+namespace {
+
+// Pooled block-RMS geometry (get_rms.cu:104-123): n = nbeams*beam_count leaves,
+// chunks of 5^L0 leaves (L0 <= 5) reduced to np partial medians.
+BlockRmsShape block_rms_shape(size_t beam_stride, size_t beam_count, size_t nbeams) {
   BlockRmsShape bs{};
   bs.bc = beam_count;
   bs.n = nbeams * beam_count;
@@ -298,18 +299,95 @@ extern "C" hd_error hd_block_rms(const hd_float* d_data, hd_float* d_block_rms,
   while (bs.L0 < 5 && pow5_host(bs.L0 + 1) <= bs.n) ++bs.L0;
   bs.chunk = pow5_host(bs.L0);
   bs.np = bs.n / bs.chunk;
+  return bs;
+}
+
+// The two launches of the block RMS: partial medians (n_series x np floats)
+// and reciprocals (n_series floats) in caller-given device scratch.
+hipError_t block_rms_launch(const float* d_data, float* d_block_rms, size_t series_stride,
+                            size_t n_series, const BlockRmsShape& bs, float* partial,
+                            float* winv, hipStream_t s) {
+  const size_t n1 = bs.chunk / 5;
+  const size_t lds = (n1 + n1 / 5 + 8) * sizeof(float);
+  hipLaunchKernelGGL(k_block_rms_direct, dim3((unsigned)bs.np, (unsigned)n_series), dim3(ST_BS),
+                     lds, s, d_data, series_stride, bs, partial);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_block_rms_finish(partial, 1, 0, 1, (int)n_series, d_block_rms, winv, s,
+                                 GroupArgs{}, &bs);
+}
+
+}  // namespace
+}  // namespace hd
+
+extern "C" hd_error hd_block_rms(const hd_float* d_data, hd_float* d_block_rms,
+                                 hd_size beam_stride, hd_size beam_count, hd_size nbeams,
+                                 hd_size n_series, hd_size series_stride, void* stream) {
+  if (!d_data || !d_block_rms) return HD_INVALID_POINTER;
+  if (!nbeams || !beam_count || beam_count > beam_stride) return HD_INVALID_STRIDE;
+  hipStream_t s = (hipStream_t)stream;
+  const BlockRmsShape bs = block_rms_shape(beam_stride, beam_count, nbeams);
   if (n_series == 0) return HD_NO_ERROR;
   Scratch tmp(s);
   float *partial, *winv;
   ST_CHECK(tmp.get(&partial, n_series * bs.np));
   ST_CHECK(tmp.get(&winv, n_series));
-  const size_t n1 = bs.chunk / 5;
-  const size_t lds = (n1 + n1 / 5 + 8) * sizeof(float);
-  hipLaunchKernelGGL(k_block_rms_direct, dim3((unsigned)bs.np, (unsigned)n_series), dim3(ST_BS),
-                     lds, s, d_data, series_stride, bs, partial);
-  ST_CHECK(hipGetLastError());
-  ST_CHECK(launch_block_rms_finish(partial, 1, 0, 1, (int)n_series, d_block_rms, winv, s,
-                                   GroupArgs{}, &bs));
+  ST_CHECK(block_rms_launch(d_data, d_block_rms, series_stride, n_series, bs, partial, winv, s));
+  return HD_NO_ERROR;
+}
+
+// ---- hd_block_rms_noalloc: the block RMS with its scratch in a fixed device
+// array instead of stream-ordered allocations (for callers that cannot
+// allocate: code under HIP graph capture, serving loops).  Each GPU holds its
+// own copy of `scratch` until the process ends; its address on a device is
+// taken once, with that device current, by hipGetSymbolAddress (the array's
+// name in host code is not a device address).  Calls on one GPU share the
+// array, so calls that overlap on one GPU race; nothing in it needs clearing
+// (every word a call reads, it has written first).
+static __device__ __align__(256) float scratch[HD_NOALLOC_SCRATCH_WORDS];
+
+namespace hd {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_on_current_device(float** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+}  // namespace
+}  // namespace hd
+
+extern "C" hd_error hd_block_rms_noalloc(const hd_float* d_data, hd_float* d_block_rms,
+                                         hd_size beam_stride, hd_size beam_count,
+                                         hd_size nbeams, hd_size n_series,
+                                         hd_size series_stride, void* stream) {
+  if (!d_data || !d_block_rms) return HD_INVALID_POINTER;
+  if (!nbeams || !beam_count || beam_count > beam_stride) return HD_INVALID_STRIDE;
+  const BlockRmsShape bs = block_rms_shape(beam_stride, beam_count, nbeams);
+  if (n_series == 0) return HD_NO_ERROR;
+  // partials first, the reciprocals 256-byte aligned after them
+  const size_t partial_words = (n_series * bs.np + 63) / 64 * 64;
+  if (n_series > HD_NOALLOC_SCRATCH_WORDS || bs.np > HD_NOALLOC_SCRATCH_WORDS ||
+      partial_words + n_series > HD_NOALLOC_SCRATCH_WORDS)
+    return HD_MEM_ALLOC_FAILED;  // beyond the fixed scratch: nothing is launched
+  float* ptr = nullptr;
+  ST_CHECK(scratch_on_current_device(&ptr));
+  ST_CHECK(block_rms_launch(d_data, d_block_rms, series_stride, n_series, bs, ptr,
+                            ptr + partial_words, (hipStream_t)stream));
   return HD_NO_ERROR;
 }
 

@@ -71,6 +71,22 @@ hd_error hd_block_rms(const hd_float* d_data, hd_float* d_block_rms, hd_size bea
                       hd_size beam_count, hd_size nbeams,
                       hd_size n_series, hd_size series_stride, void* stream);
 
+/* hd_block_rms without allocation.  NO COUNTERPART in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid it).  Same arguments, kernels and
+ * results as hd_block_rms, but its scratch (n_series*np partial medians and
+ * n_series reciprocals, np = floor(nbeams*beam_count / 5^L) with 5^L the
+ * largest power of five <= min(nbeams*beam_count, 3125)) lives in a fixed
+ * device array of HD_NOALLOC_SCRATCH_WORDS floats (4 MiB) that each GPU the
+ * library runs on holds until the process ends.  Returns HD_MEM_ALLOC_FAILED,
+ * launching nothing, when round_up(n_series*np, 64) + n_series exceeds it.
+ * No hipMalloc/hipFree is called.  Calls on one GPU share the array: calls
+ * that overlap on one GPU (concurrent streams or threads) race. */
+#define HD_NOALLOC_SCRATCH_WORDS (1u << 20)
+hd_error hd_block_rms_noalloc(const hd_float* d_data, hd_float* d_block_rms,
+                              hd_size beam_stride, hd_size beam_count, hd_size nbeams,
+                              hd_size n_series, hd_size series_stride, void* stream);
+
 /* replaces normalise_multibeam(d, block_rms, count) (Pipeline/get_rms.cu:189-200),
  * with one block_rms per series read from device memory. */
 hd_error hd_normalise_block(hd_float* d_data, const hd_float* d_block_rms, hd_size count,
