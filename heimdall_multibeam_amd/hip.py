@@ -95,6 +95,39 @@ class DeviceBuffer:
         return self.ptr.value
 
 
+class PinnedBuffer:
+    """Page-locked host memory (hipHostMalloc): host filterbanks that the
+    runtime copies to the device at full link speed."""
+
+    def __init__(self, nbytes):
+        h = hip()
+        h.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        h.hipHostFree.argtypes = [C.c_void_p]
+        self.ptr = C.c_void_p()
+        self.nbytes = nbytes
+        check(h.hipHostMalloc(C.byref(self.ptr), max(nbytes, 1), 0), "hipHostMalloc")
+
+    def array(self):
+        """The buffer as a uint8 numpy array (no copy)."""
+        import numpy as np
+        return np.ctypeslib.as_array(C.cast(self.ptr, C.POINTER(C.c_uint8)), (self.nbytes,))
+
+    def free(self):
+        if self.ptr:
+            hip().hipHostFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+
 def copy_device(dst, src, nbytes):
     """Device-to-device copy (one runtime copy kernel of nbytes)."""
     check(hip().hipMemcpy(C.c_void_p(dst), C.c_void_p(src), nbytes, D2D), "hipMemcpy D2D")
