@@ -1113,9 +1113,12 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   // unpack to channel-major bytes
   const size_t xpitch = round_up(ppitch + max_delay + 2 * pl->wlen + 256, 256);
   HD_CHECK(pl->x.ensure((size_t)G * std::max(n_live, 1) * xpitch));
+  // (the proposal's decimation reads whole rows, so rows are trimmed only
+  // for the plain search of several gulps)
   HD_CHECK(hd::launch_unpack(nullptr, p.nchans * nbits / 8, (int)nbits, 0, pl->live.as<int>(),
                              n_live, pl->x.as<uint8_t>(), xpitch, s, ga,
-                             (size_t)n_live == p.nchans));
+                             (size_t)n_live == p.nchans,
+                             G > 1 && !(pl->gating || pl->coarse_only)));
   record(pl, 10);
   // The search runs over row groups: a whole gulp each, or with gating one
   // per proposed unit of HD_PROPOSAL_UNIT trials (rows past the last trial

@@ -130,10 +130,13 @@ hipError_t run_clean(const CleanPlan& cp, const uint8_t* d_in, uint8_t* d_out, v
 
 // ---- launchers (return hipError_t of the launch) ----
 // live_is_identity: live channel l is channel l (no channel masked).
+// trim_rows: a group's rows are written only as far as its dedispersion reads
+// (nsamps plus two time tiles), not out to xpitch: for ragged batches whose
+// rows nothing else reads.
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
                          hipStream_t s, const GroupArgs& ga = GroupArgs{},
-                         bool live_is_identity = false);
+                         bool live_is_identity = false, bool trim_rows = false);
 
 struct DedispGeom {
   int n_dm, n_live;

@@ -28,13 +28,27 @@ __device__ __forceinline__ unsigned unpack_sample(const uint8_t* row, int c, int
   return (byte >> shift) & ((1u << nbits) - 1u);
 }
 
+// Bytes of a group's unpacked rows that are written: the whole padded row,
+// or with trim > 0 the first nsamps + trim (rounded to 64, at most the
+// row).  The dedispersion tiles of a group read its rows below
+// nsamps + D2_TB + 64 (a tile starts at most at the group's last output,
+// its window spans D2_TB samples past the largest delay plus the staging
+// overrun), so trim = 2 * D2_TB covers every read of a short group of a
+// ragged batch while skipping the rest of its padding.
+__device__ __forceinline__ size_t row_limit(size_t nsamps, size_t xpitch, size_t trim) {
+  if (!trim) return xpitch;
+  const size_t l = (nsamps + trim + 63) / 64 * 64;
+  return l < xpitch ? l : xpitch;
+}
+
 // 64 time x 64 channel tiles: reads along channels, writes along time.
 // blockIdx.z: the group (its input, sample count and rows of x).
 __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ fb,
                                                 size_t stride_bytes, int nbits, size_t nsamps,
                                                 const int* __restrict__ live, int n_live,
                                                 uint8_t* __restrict__ x, size_t xpitch,
-                                                const GroupDev* __restrict__ gd) {
+                                                const GroupDev* __restrict__ gd,
+                                                size_t trim) {
   __shared__ uint8_t tile[64][68];
   if (gd) {
     fb = gd[blockIdx.z].fb;
@@ -42,6 +56,8 @@ __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ fb,
   }
   x += (size_t)blockIdx.z * n_live * xpitch;
   const size_t t0 = (size_t)blockIdx.x * 64;
+  const size_t tlim = row_limit(nsamps, xpitch, trim);
+  if (t0 >= tlim) return;
   const int l0 = blockIdx.y * 64;
   const int tx = threadIdx.x & 63;
   const int ty = threadIdx.x >> 6;
@@ -57,7 +73,7 @@ __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ fb,
   for (int r = ty; r < 64; r += 4) {
     const int lo = l0 + r;
     const size_t t = t0 + tx;
-    if (lo < n_live && t < xpitch) x[(size_t)lo * xpitch + t] = tile[tx][r];
+    if (lo < n_live && t < tlim) x[(size_t)lo * xpitch + t] = tile[tx][r];
   }
 }
 
@@ -68,7 +84,7 @@ constexpr int UP_T = 64, UP_C = 256;
 __global__ __launch_bounds__(256) void k_unpack8(const uint8_t* __restrict__ fb,
                                                  size_t stride_bytes, size_t nsamps, int nchans,
                                                  uint8_t* __restrict__ x, size_t xpitch,
-                                                 const GroupDev* __restrict__ gd) {
+                                                 const GroupDev* __restrict__ gd, size_t trim) {
   __shared__ uint32_t tile[UP_T][UP_C / 4 + 1];
   if (gd) {
     fb = gd[blockIdx.z].fb;
@@ -76,6 +92,8 @@ __global__ __launch_bounds__(256) void k_unpack8(const uint8_t* __restrict__ fb,
   }
   x += (size_t)blockIdx.z * nchans * xpitch;
   const size_t t0 = (size_t)blockIdx.x * UP_T;
+  const size_t tlim = row_limit(nsamps, xpitch, trim);
+  if (t0 >= tlim) return;
   const int c0 = blockIdx.y * UP_C;
   const int tid = threadIdx.x;
   const bool words = (stride_bytes % 4 == 0) && (reinterpret_cast<uintptr_t>(fb) % 4 == 0);
@@ -105,7 +123,7 @@ __global__ __launch_bounds__(256) void k_unpack8(const uint8_t* __restrict__ fb,
     const int cc = idx / (UP_T / 4), q = idx % (UP_T / 4);
     const int c = c0 + cc;
     const size_t t = t0 + 4 * q;
-    if (c >= nchans || t >= xpitch) continue;
+    if (c >= nchans || t >= tlim) continue;
     const uint32_t v = (uint32_t)tb[(4 * q + 0) * RB + cc] |
                        ((uint32_t)tb[(4 * q + 1) * RB + cc] << 8) |
                        ((uint32_t)tb[(4 * q + 2) * RB + cc] << 16) |
@@ -568,17 +586,19 @@ int dedisp_window_dwords(int spread) {
 
 hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size_t nsamps,
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
-                         hipStream_t s, const GroupArgs& ga, bool live_is_identity) {
+                         hipStream_t s, const GroupArgs& ga, bool live_is_identity,
+                         bool trim_rows) {
+  const size_t trim = trim_rows ? (size_t)2 * D2_TB : 0;
   if (nbits == 8 && live_is_identity && xpitch % 4 == 0) {
     dim3 grid((unsigned)((xpitch + UP_T - 1) / UP_T), (unsigned)((n_live + UP_C - 1) / UP_C),
               (unsigned)ga.G);
     hipLaunchKernelGGL(k_unpack8, grid, dim3(256), 0, s, fb, stride_bytes, nsamps, n_live, x,
-                       xpitch, ga.gd);
+                       xpitch, ga.gd, trim);
     return hipGetLastError();
   }
   dim3 grid((unsigned)((xpitch + 63) / 64), (unsigned)((n_live + 63) / 64), (unsigned)ga.G);
   hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, s, fb, stride_bytes, nbits, nsamps, d_live,
-                     n_live, x, xpitch, ga.gd);
+                     n_live, x, xpitch, ga.gd, trim);
   return hipGetLastError();
 }
 

@@ -355,8 +355,13 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const unsigned tlen = (unsigned)(TH_CHUNK + amax + bmax);
   const unsigned sh = (unsigned)((((uintptr_t)S >> 2) + (unsigned long long)tbase) & 3u);
   const long long tal = tbase - sh;               // S + tal is 16-byte aligned
-  const unsigned n4 = (tlen + sh + 3) / 4;
   const long long slen = (long long)s_len;
+  // Every output a width keeps reads prefix sums below s_len, so a row that
+  // ends inside the chunk (short gulps of a batch) stages only that far; the
+  // tile past it is read only for outputs that are never kept.
+  const long long tavail = slen - tbase;
+  const unsigned tneed = tavail <= 0 ? 1u : (tavail < (long long)tlen ? (unsigned)tavail : tlen);
+  const unsigned n4 = (tneed + sh + 3) / 4;
   constexpr int SU = 4;
   for (unsigned v0 = threadIdx.x; v0 < n4; v0 += SU * FB_BS) {
     float4 r[SU];
