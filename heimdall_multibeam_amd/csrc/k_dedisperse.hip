@@ -376,10 +376,11 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
                                                       DedispGeom g, int hw_max, int widen,
                                                       float* __restrict__ out,
                                                       const GroupDev* __restrict__ gd) {
-  extern __shared__ uint32_t lds8[];  // [CC][4][hw_max]
+  extern __shared__ uint32_t lds8[];  // [CC][4][hw_max], then an int2 per live channel
   constexpr int WAVES = BS / 64;
   constexpr int NA = DB / WAVES;
   __shared__ __align__(16) int sdly[DB][CC];
+  int2* ctab = reinterpret_cast<int2*>(lds8 + (size_t)CC * 4 * hw_max);
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
   const int trial0 = gd ? (int)gd[blockIdx.z].trial0 : 0;
   x += (size_t)(gd ? gd[blockIdx.z].xslot : blockIdx.z) * g.n_live * g.xpitch;
@@ -401,6 +402,19 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
   if (t0 > nout) return;
   const int* drow0 = delays + (size_t)(trial0 + m0) * g.n_live;
   const int* drowl = delays + (size_t)(trial0 + ml) * g.n_live;
+  // Per channel, once per block: the window's first sample (the first
+  // trial's delay rounded down to a dword) and its staging items.  The window
+  // spans the block's delays plus 1024 samples; staging item u gives dwords
+  // 4u .. 4u+3 of each of the 4 copies from source dwords 4u .. 4u+4, one
+  // 16-byte LDS write per copy (the copies' rows are 16-byte aligned: hw_max
+  // is a multiple of 64).  The last item may read up to 7 dwords past the
+  // window, inside the row's padding.  Keeping these in LDS takes the delay
+  // table's global-memory latency off the per-chunk staging.
+  for (int c = tid; c < g.n_live; c += BS) {
+    const int b = drow0[c] & ~3;
+    const int ndw = (D2_TB + drowl[c] - b + 3) / 4 + 1;
+    ctab[c] = make_int2(b, (ndw + 3) / 4);
+  }
 
   uint32_t pa[NA][D8_Q];
   uint32_t po[NA][SPLIT ? D8_Q : 1];  // odd bytes in 16-bit lanes (SPLIT)
@@ -414,35 +428,31 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
 #pragma unroll
       for (int e = 0; e < 4; ++e) ua[a][q][e] = 0;
     }
-  int pending = 0;
   const int nchunk = (g.n_live + CC - 1) / CC;
   constexpr int TPC = BS / CC;
   const int scc = tid / TPC;
   const int sjj = tid % TPC;
-  // dwords of each staged copy that channel c needs: its window spans the
-  // block's delays (first trial's rounded down to a dword) plus 1024 samples
-  auto ndw = [&](int c) {
-    const int span = drowl[c] - (drow0[c] & ~3);
-    return (D2_TB + span + 3) / 4 + 1;
-  };
-  // Staging item u of a channel: source dwords 4u .. 4u+4 give dwords
-  // 4u .. 4u+3 of each of the 4 copies, one 16-byte LDS write per copy (the
-  // copies' rows are 16-byte aligned: hw_max is a multiple of 64).  The last
-  // item may read up to 7 dwords past the window, inside the row's padding.
-  auto ngrp = [&](int c) { return (ndw(c) + 3) / 4; };
+  __syncthreads();
   uint32_t pf[D8_PF][5];
+  int pf_ng = 0;  // items of this thread's channel in the prefetched chunk; -1: past the last
   auto prefetch = [&](int k) {
     const int c = k * CC + scc;
-    if (k >= nchunk || c >= g.n_live) return;
-    const int ng = ngrp(c);
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(
-        x + (size_t)c * g.xpitch + t0 + (drow0[c] & ~3));
+    pf_ng = 0;
+    if (k >= nchunk) return;
+    if (c >= g.n_live) {
+      pf_ng = -1;
+      return;
+    }
+    const int2 t = ctab[c];
+    pf_ng = t.y;
+    // item sjj + r * TPC at a constant offset from item sjj
+    const uint32_t* src =
+        reinterpret_cast<const uint32_t*>(x + (size_t)c * g.xpitch + t0 + t.x) + 4 * sjj;
 #pragma unroll
     for (int r = 0; r < D8_PF; ++r) {
-      const int u = sjj + r * TPC;
-      if (u < ng) {
+      if (sjj + r * TPC < t.y) {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) pf[r][i] = src[4 * u + i];
+        for (int i = 0; i < 5; ++i) pf[r][i] = src[4 * r * TPC + i];
       }
     }
   };
@@ -456,52 +466,74 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
           __builtin_amdgcn_alignbyte(d[1], d[0], k), __builtin_amdgcn_alignbyte(d[2], d[1], k),
           __builtin_amdgcn_alignbyte(d[3], d[2], k), __builtin_amdgcn_alignbyte(d[4], d[3], k));
   };
-  prefetch(0);
-  int pf_delay = 0;
-  auto prefetch_delay = [&](int k) {
+  // the raw delay of (trial tid / CC, channel tid % CC) of a chunk, loaded
+  // one chunk ahead of its use
+  int rd = 0;
+  auto load_delay = [&](int k) {
     if (k < nchunk && tid < DB * CC) {
       const int c = min(k * CC + tid % CC, g.n_live - 1);
       const int mm = min(m0 + tid / CC, g.n_dm - 1);
-      const int o = delays[(size_t)(trial0 + mm) * g.n_live + c] - (drow0[c] & ~3);
-      pf_delay = ((tid % CC) * 4 + (o & 3)) * hw_max + (o >> 2);
+      rd = delays[(size_t)(trial0 + mm) * g.n_live + c];
     }
   };
-  prefetch_delay(0);
-  for (int k = 0; k < nchunk; ++k) {
-    const int c0 = k * CC;
-    const int ncc = min(CC, g.n_live - c0);
-    __syncthreads();
-    if (tid < DB * CC) sdly[tid / CC][tid % CC] = pf_delay;
-    if (scc < ncc) {
-      const int ng = ngrp(c0 + scc);
-#pragma unroll
-      for (int r = 0; r < D8_PF; ++r) {
-        const int u = sjj + r * TPC;
-        if (u < ng) put4(u, pf[r]);
+  prefetch(0);
+  load_delay(0);
+  // Chunks in runs of widen / CC between widenings (widen >= CC), so that the
+  // summing loop has no widening branch; a chunk's channels past the last
+  // live one are staged as zeros.
+  const int per_widen = widen / CC;
+  int k = 0;
+  while (k < nchunk) {
+    const int kend = min(nchunk, k + per_widen);
+    for (; k < kend; ++k) {
+      const int c0 = k * CC;
+      __syncthreads();
+      if (tid < DB * CC) {
+        const int c = c0 + tid % CC;
+        int p = (tid % CC) * 4 * hw_max;  // past the last channel: copy 0, zeros
+        if (c < g.n_live) {
+          const int o = rd - ctab[c].x;
+          p += (o & 3) * hw_max + (o >> 2);
+        }
+        sdly[tid / CC][tid % CC] = p;
       }
-      const uint32_t* src = reinterpret_cast<const uint32_t*>(
-          x + (size_t)(c0 + scc) * g.xpitch + t0 + (drow0[c0 + scc] & ~3));
-      for (int u = sjj + D8_PF * TPC; u < ng; u += TPC) {
-        uint32_t d[5];
+      if (pf_ng > 0) {
 #pragma unroll
-        for (int i = 0; i < 5; ++i) d[i] = src[4 * u + i];
-        put4(u, d);
-      }
-    }
-    __syncthreads();
-    prefetch(k + 1);
-    prefetch_delay(k + 1);
-    auto add = [&](int a, const uint32_t* src) {
+        for (int r = 0; r < D8_PF; ++r) {
+          const int u = sjj + r * TPC;
+          if (u < pf_ng) put4(u, pf[r]);
+        }
+        if (pf_ng > D8_PF * TPC) {
+          const uint32_t* src = reinterpret_cast<const uint32_t*>(
+              x + (size_t)(c0 + scc) * g.xpitch + t0 + ctab[c0 + scc].x);
+          for (int u = sjj + D8_PF * TPC; u < pf_ng; u += TPC) {
+            uint32_t d[5];
 #pragma unroll
-      for (int q = 0; q < D8_Q; ++q) {
-        const uint32_t v = src[64 * q];
-        pa[a][q] += v;
-        if (SPLIT) po[a][SPLIT ? q : 0] += __builtin_amdgcn_perm(0u, v, 0x0C030C01u);
+            for (int i = 0; i < 5; ++i) d[i] = src[4 * u + i];
+            put4(u, d);
+          }
+        }
+      } else if (pf_ng < 0) {
+        // the 64 * D8_Q dwords a trial reads at offset 0
+        uint32_t* C = lds8 + (size_t)(scc * 4) * hw_max;
+        for (int u = sjj; u < 64 * D8_Q / 4; u += TPC)
+          *reinterpret_cast<uint4*>(C + 4 * u) = make_uint4(0u, 0u, 0u, 0u);
       }
-    };
-    if (ncc == CC) {
-      // a full chunk, four channels at a time: their offsets for the wave's
-      // trials in one 16-byte read per trial
+      // the next chunk's loads are issued before the barrier, as early as
+      // this chunk's staging registers are free
+      prefetch(k + 1);
+      load_delay(k + 1);
+      __syncthreads();
+      auto add = [&](int a, const uint32_t* src) {
+#pragma unroll
+        for (int q = 0; q < D8_Q; ++q) {
+          const uint32_t v = src[64 * q];
+          pa[a][q] += v;
+          if (SPLIT) po[a][SPLIT ? q : 0] += __builtin_amdgcn_perm(0u, v, 0x0C030C01u);
+        }
+      };
+      // four channels at a time: their offsets for the wave's trials in one
+      // 16-byte read per trial
 #pragma unroll
       for (int v = 0; v < CC / 4; ++v) {
         int off[NA][4];
@@ -518,33 +550,25 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
 #pragma unroll
           for (int a = 0; a < NA; ++a) add(a, lds8 + off[a][cc] + lane);
       }
-    } else {
-      for (int cc = 0; cc < ncc; ++cc)
-#pragma unroll
-        for (int a = 0; a < NA; ++a) add(a, lds8 + sdly[wave + WAVES * a][cc] + lane);
     }
-    pending += ncc;
-    if (pending + CC > widen || c0 + CC >= g.n_live) {
 #pragma unroll
-      for (int a = 0; a < NA; ++a)
+    for (int a = 0; a < NA; ++a)
 #pragma unroll
-        for (int q = 0; q < D8_Q; ++q) {
-          if (SPLIT) {
-            const uint32_t od = po[a][SPLIT ? q : 0];
-            const uint32_t ev = pa[a][q] - (od << 8);
-            ua[a][q][0] += ev & 0xFFFFu;
-            ua[a][q][1] += od & 0xFFFFu;
-            ua[a][q][2] += ev >> 16;
-            ua[a][q][3] += od >> 16;
-            po[a][SPLIT ? q : 0] = 0;
-          } else {
+      for (int q = 0; q < D8_Q; ++q) {
+        if (SPLIT) {
+          const uint32_t od = po[a][SPLIT ? q : 0];
+          const uint32_t ev = pa[a][q] - (od << 8);
+          ua[a][q][0] += ev & 0xFFFFu;
+          ua[a][q][1] += od & 0xFFFFu;
+          ua[a][q][2] += ev >> 16;
+          ua[a][q][3] += od >> 16;
+          po[a][SPLIT ? q : 0] = 0;
+        } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) ua[a][q][e] += (pa[a][q] >> (8 * e)) & 0xFFu;
-          }
-          pa[a][q] = 0;
+          for (int e = 0; e < 4; ++e) ua[a][q][e] += (pa[a][q] >> (8 * e)) & 0xFFu;
         }
-      pending = 0;
-    }
+        pa[a][q] = 0;
+      }
   }
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
@@ -651,10 +675,16 @@ const void* kernel8_wide(int cc, bool split) {
   return split ? ts[k] : t[k];
 }
 
+// Dynamic LDS of the byte kernel: the staged copies, then each live
+// channel's window (int2).
+static size_t dedisp8_dynamic_lds(const DedispGeom& g) {
+  return (size_t)g.cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t) + (size_t)g.n_live * 8;
+}
+
 // LDS of a tile: the staged copies (dynamic) plus the trial offsets (static).
 size_t dedisp_lds_bytes(const DedispGeom& g) {
   const size_t offs = (size_t)g.db * g.cc * sizeof(int);
-  if (g.widen8 > 0) return (size_t)g.cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t) + offs;
+  if (g.widen8 > 0) return dedisp8_dynamic_lds(g) + offs;
   return (size_t)g.cc * 2 * (size_t)g.wlen * sizeof(uint32_t) + offs;
 }
 
@@ -671,9 +701,10 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
   // the delay span of a DM block too wide for this tile's staged copies
   if (dedisp_lds_bytes(g) > 160 * 1024) return hipErrorInvalidValue;
   if (g.widen8 > 0) {
-    if (g.widen8 < cc || g.wlen8 <= 0) return hipErrorInvalidValue;
+    // (a copy row holds at least the 64 * D8_Q dwords a trial reads)
+    if (g.widen8 < cc || g.wlen8 < 64 * D8_Q || g.wlen8 % 64) return hipErrorInvalidValue;
     if (g.split8 && g.widen8 > 256) return hipErrorInvalidValue;   // 16-bit lanes would carry
-    const size_t lds8 = (size_t)cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t);
+    const size_t lds8 = dedisp8_dynamic_lds(g);
     const bool wide = g.wide && db == 32;
     const void* f8 = wide ? kernel8_wide(cc, g.split8 != 0) : kernel8(slot, g.split8 != 0);
     const hipError_t e =
