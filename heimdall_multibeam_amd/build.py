@@ -5,6 +5,7 @@
   bin/heimdall           gulp driver over sigproc files (app/heimdall_main.cpp)
   bin/generate_dmlist    DM trial list tool (app/generate_dmlist.cpp)
   bin/coincidencer       multibeam coincidence of candidate files / TCP messages
+  bin/candidate_profiler DM-time / frequency-time plots around one candidate
 
 Every HIP source is compiled for gfx950 only, with -ffp-contract=off (the
 reference performs no fused multiply-adds, and results are compared bit for
@@ -22,12 +23,12 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libheimdall_mi355x.so")
 SYNTH = os.path.join(HERE, "libhd_synth.so")
 
-HIP_SOURCES = ["hd_api.cpp", "formats.cpp", "k_dedisperse.hip", "k_normalise.hip",
+HIP_SOURCES = ["hd_api.cpp", "dedisp_plan.cpp", "formats.cpp", "k_dedisperse.hip", "k_normalise.hip",
                "k_filter.hip", "k_cluster.hip", "k_stages.hip", "k_clean.hip",
                "k_proposal.hip", "k_cnn.hip", "k_synth.hip"]
 # command-line tools linked against the library: bin/<name> from app/<name>.cpp
 APPS = {"heimdall": "heimdall_main.cpp", "generate_dmlist": "generate_dmlist.cpp",
-        "coincidencer": "coincidencer.cpp"}
+        "coincidencer": "coincidencer.cpp", "candidate_profiler": "candidate_profiler.cpp"}
 BIN = os.path.join(HERE, "bin")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

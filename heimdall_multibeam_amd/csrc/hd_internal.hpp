@@ -137,6 +137,11 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
                          const int* d_live, int n_live, uint8_t* x, size_t xpitch,
                          hipStream_t s, const GroupArgs& ga = GroupArgs{},
                          bool live_is_identity = false, bool trim_rows = false);
+// Halves the time resolution of unpacked rows: y[l][t] = (x[l][2t] +
+// x[l][2t+1]) >> 1 for t < nsamps / 2, zero out to the pitch (both pitches
+// xpitch, a multiple of 256 and at least nsamps + 8).
+hipError_t launch_scrunch2(const uint8_t* x, int n_live, size_t xpitch, size_t nsamps,
+                           uint8_t* y, hipStream_t s);
 
 struct DedispGeom {
   int n_dm, n_live;

@@ -59,4 +59,41 @@ inline int delay(float dm, float kc) {
   return int(x + 0.5f);
 }
 
+// Adaptive time resolution (dedisp_enable_adaptive_dt, requested at
+// Pipeline/pipeline.cu:187-196 with dm_pulse_width and scrunch_tol).  The
+// dedisp library is not part of the reference tree; this restates its
+// published rule.  The effective width of a pulse at DM with trial step dDM,
+// in float throughout (f0, df in MHz, sampling time and width in seconds):
+//   BW = nchans |df|, fc = f0 - BW / 2,
+//   t_DM = 8.3 BW DM / fc^3, t_dDM = 8.3 / 4 BW nchans dDM / fc^3,
+//   W_eff = sqrt(dt^2 + W^2 + t_DM^2 + t_dDM^2).
+inline float smearing(float dt, float width, float f0, std::size_t nchans, float df, float dm,
+                      float ddm) {
+  const float bw = float(nchans) * std::fabs(df);
+  const float fc = f0 - bw / 2.f;
+  const float inv_fc3 = 1.f / (fc * fc * fc);
+  const float t_dm = 8.3f * bw * dm * inv_fc3;
+  const float t_ddm = 8.3f / 4.f * bw * float(nchans) * ddm * inv_fc3;
+  return std::sqrt(dt * dt + width * width + t_dm * t_dm + t_ddm * t_ddm);
+}
+
+// Time-scrunch factor of each trial: 1 for the first; each next trial doubles
+// its predecessor's factor when doubling the sampling time would raise the
+// effective width by less than a factor tol, else keeps it.  Factors are
+// powers of two and never decrease along the list.
+inline std::vector<std::size_t> scrunch_list(const std::vector<float>& dms, float dt_s,
+                                             float pulse_width_us, float f0, float df,
+                                             std::size_t nchans, float tol) {
+  std::vector<std::size_t> s(dms.size(), 1);
+  const float w = pulse_width_us * 1e-6f;
+  for (std::size_t d = 1; d < dms.size(); ++d) {
+    const float ddm = dms[d] - dms[d - 1];
+    const float cur = float(s[d - 1]) * dt_s;
+    const float a = smearing(cur, w, f0, nchans, df, dms[d], ddm);
+    const float b = smearing(2.f * cur, w, f0, nchans, df, dms[d], ddm);
+    s[d] = (b / a < tol) ? 2 * s[d - 1] : s[d - 1];
+  }
+  return s;
+}
+
 }  // namespace hdplan

@@ -590,7 +590,38 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
   }
 }
 
+// Adaptive time resolution: one thread per 4 output samples of a row, from
+// one 8-byte load (xpitch is a multiple of 256, so every load is aligned and
+// inside the row).  The pairwise mean rounds down, so samples keep their bit
+// width.
+__global__ __launch_bounds__(256) void k_scrunch2(const uint8_t* __restrict__ x, size_t xpitch,
+                                                  size_t nsamps, uint8_t* __restrict__ y) {
+  const size_t t4 = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+  const size_t row = blockIdx.y;
+  if (t4 >= xpitch) return;
+  const size_t nh = nsamps / 2;
+  uint32_t out = 0;
+  if (t4 < nh) {
+    const uint2 w = *reinterpret_cast<const uint2*>(x + row * xpitch + 2 * t4);
+    const uint32_t b[8] = {w.x & 0xffu, (w.x >> 8) & 0xffu, (w.x >> 16) & 0xffu, w.x >> 24,
+                           w.y & 0xffu, (w.y >> 8) & 0xffu, (w.y >> 16) & 0xffu, w.y >> 24};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (t4 + k < nh) out |= ((b[2 * k] + b[2 * k + 1]) >> 1) << (8 * k);
+  }
+  *reinterpret_cast<uint32_t*>(y + row * xpitch + t4) = out;
+}
+
 }  // namespace
+
+hipError_t launch_scrunch2(const uint8_t* x, int n_live, size_t xpitch, size_t nsamps,
+                           uint8_t* y, hipStream_t s) {
+  // (a thread's 8-byte load ends at most 5 bytes past the row's last sample)
+  if (xpitch % 256 || nsamps + 8 > xpitch || n_live <= 0) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((xpitch / 4 + 255) / 256), (unsigned)n_live);
+  hipLaunchKernelGGL(k_scrunch2, grid, dim3(256), 0, s, x, xpitch, nsamps, y);
+  return hipGetLastError();
+}
 
 int dedisp_tile_time() { return D2_TB; }
 // Default trials per block (the pipeline's autotuner may choose 8 or 32).
