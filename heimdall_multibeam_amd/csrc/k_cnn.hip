@@ -1,17 +1,19 @@
 // k_cnn.hip -- the proposal network: forward, backward and Adam on gfx950.
 //
 // Every matrix stage is one instantiation of k_mm, a wave-tiled bfloat16
-// MFMA product C[M][N] = sum_k A[m][k] B[k][n] (v_mfma_f32_16x16x16bf16_1k,
+// MFMA product C[M][N] = sum_k A[m][k] B[k][n] (v_mfma_f32_16x16x32_bf16,
 // float32 accumulation).  The operands are not materialised: each stage's
-// A and B functors gather them (implicit im2col for the convolutions, the
-// transposed convolution for the data gradients), and its C functor applies
-// the epilogue (bias + ReLU + bfloat16 store, ReLU-masked gradient store, or
-// a float32 split-K partial).
+// A and B functors gather them, four consecutive k at a time (implicit
+// im2col for the convolutions, the transposed convolution for the data
+// gradients), and its C functor applies the epilogue (bias + ReLU +
+// bfloat16 store, ReLU-masked gradient store, or a float32 split-K partial).
+// Where every wave of a block shares its B columns (the convolutions'
+// weights), the block stages B in LDS once.
 //
-// Register layout of v_mfma_f32_16x16x16bf16_1k for lane l (0..63), with
-// r = l % 16 and q = 4 * (l / 16):
-//   A operand: A[r][q..q+3]      B operand: B[q..q+3][r]
-//   result:    C[q..q+3][r]
+// Register layout of v_mfma_f32_16x16x32_bf16 for lane l (0..63), with
+// r = l % 16 and g = l / 16 (checked against a CPU product on gfx950):
+//   A operand: A[r][8g..8g+7]    B operand: B[8g..8g+7][r]
+//   result:    C[4g..4g+3][r]
 //
 // Activations are NHWC (channels contiguous), bfloat16.
 #include <hip/hip_runtime.h>
@@ -71,39 +73,84 @@ __device__ __forceinline__ bf16x4 load4(const uint16_t* p) {
 constexpr uint16_t BF_ONE = 0x3F80;
 
 // ---------------------------------------------------------------------------
-// The generic product.  A block is four waves; wave w owns output tile
-// blockIdx.x*4 + w of (16*TM) x (16*TN) elements, tiles ordered n fastest.
-// blockIdx.y selects a K chunk [y*kchunk, min((y+1)*kchunk, K)) (split-K).
-// K and kchunk are multiples of 16; the functors return zeros past their
-// data, and the C functor drops rows >= M and columns >= N.
-template <int TM, int TN, class P>
+// The generic product, on gfx950's K = 32 step (v_mfma_f32_16x16x32_bf16):
+// lane l = 16 g + r holds A[r][8g .. 8g+7] and B[8g .. 8g+7][r] of the step,
+// and gets C[4g .. 4g+3][r] (the result layout of the K = 16 form).  The
+// 8-element operands are two 4-element functor gathers.
+//
+// A block is four waves; wave w owns output tile blockIdx.x*4 + w of
+// (16*TM) x (16*TN) elements, tiles ordered n fastest.  blockIdx.y selects a
+// K chunk [y*kchunk, min((y+1)*kchunk, K)) (split-K).  K and kchunk are
+// multiples of 16 (the second half of a step past the chunk's end is zero,
+// and its gathers are not made); the functors return zeros past their data,
+// and the C functor drops rows >= M and columns >= N.
+//
+// STAGE_B: every wave of the block has the same columns (N <= 16*TN), so the
+// block gathers its B chunk once into LDS as [column][k] rows of kchunk + 8
+// elements (16-byte aligned rows; 16-byte reads of 16 rows hit distinct
+// banks) and the waves read their B operands there: the convolutions'
+// weights, shared by the block's 4 * TM row tiles.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+  const s16x8 v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+template <int TM, int TN, bool STAGE_B, class P>
 __global__ __launch_bounds__(256) void k_mm(P p) {
+  extern __shared__ uint16_t bsh[];  // STAGE_B: [16*TN][kchunk + 8]
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int r = lane & 15, q = (lane >> 4) * 4;
+  const int r = lane & 15, g = lane >> 4;
   const long tiles_n = (p.N + 16 * TN - 1) / (16 * TN);
+  const long kb = (long)blockIdx.y * p.kchunk;
+  const long ke = kb + p.kchunk < p.K ? kb + p.kchunk : p.K;
+  const int ldb = (int)p.kchunk + 8;
+  if (STAGE_B) {  // the block's columns are 0 .. 16*TN-1 (tiles_n == 1)
+    const int items = 16 * TN * (int)(p.kchunk / 4);
+    for (int it = threadIdx.x; it < items; it += 256) {
+      const int n = it % (16 * TN), kq = it / (16 * TN);
+      const long k = kb + 4 * kq;
+      *reinterpret_cast<bf16x4*>(bsh + n * ldb + 4 * kq) = k < ke ? p.B(k, n) : zero4();
+    }
+    __syncthreads();
+  }
   const long tile = (long)blockIdx.x * 4 + wave;
   const long tm = tile / tiles_n, tn = tile - tm * tiles_n;
   const long m0 = tm * 16 * TM, n0 = tn * 16 * TN;
-  if (m0 >= p.M) return;  // wave-uniform
+  if (m0 >= p.M) return;  // wave-uniform, after the block's only barrier
   f32x4 acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const long kb = (long)blockIdx.y * p.kchunk;
-  const long ke = kb + p.kchunk < p.K ? kb + p.kchunk : p.K;
-  for (long k0 = kb; k0 < ke; k0 += 16) {
-    bf16x4 a[TM], b[TN];
+  for (long k0 = kb; k0 < ke; k0 += 32) {
+    const long k = k0 + 8 * g;
+    const bool in = k < ke;  // ke - k0 is a multiple of 16: all 8 or none
+    bf16x8 a[TM], b[TN];
 #pragma unroll
-    for (int i = 0; i < TM; ++i) a[i] = p.A(m0 + 16 * i + r, k0 + q);
+    for (int i = 0; i < TM; ++i) {
+      const long m = m0 + 16 * i + r;
+      a[i] = in ? cat8(p.A(m, k), p.A(m, k + 4)) : cat8(zero4(), zero4());
+    }
 #pragma unroll
-    for (int j = 0; j < TN; ++j) b[j] = p.B(k0 + q, n0 + 16 * j + r);
+    for (int j = 0; j < TN; ++j) {
+      if (STAGE_B) {
+        b[j] = in ? __builtin_bit_cast(bf16x8, *reinterpret_cast<const s16x8*>(
+                                                   bsh + (16 * j + r) * ldb + (k - kb)))
+                  : cat8(zero4(), zero4());
+      } else {
+        const long n = n0 + 16 * j + r;
+        b[j] = in ? cat8(p.B(k, n), p.B(k + 4, n)) : cat8(zero4(), zero4());
+      }
+    }
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -111,7 +158,7 @@ __global__ __launch_bounds__(256) void k_mm(P p) {
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        p.C(m0 + 16 * i + q + e, n0 + 16 * j + r, acc[i][j][e], (int)blockIdx.y);
+        p.C(m0 + 16 * i + 4 * g + e, n0 + 16 * j + r, acc[i][j][e], (int)blockIdx.y);
 }
 
 template <int TM, int TN, class P>
@@ -119,8 +166,14 @@ hipError_t run_mm(const P& p, hipStream_t s) {
   const long tiles = ((p.M + 16 * TM - 1) / (16 * TM)) * ((p.N + 16 * TN - 1) / (16 * TN));
   const long splits = (p.K + p.kchunk - 1) / p.kchunk;
   if (tiles <= 0 || splits <= 0) return hipSuccess;
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mm<TM, TN, P>), dim3((unsigned)((tiles + 3) / 4),
-                     (unsigned)splits), dim3(256), 0, s, p);
+  if (p.K % 16 || p.kchunk % 16) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)splits);
+  const size_t lds = (size_t)16 * TN * (p.kchunk + 8) * sizeof(uint16_t);
+  // (one step of K = 32 reads each B element once: staging it would only add a barrier)
+  if (p.N <= 16 * TN && p.kchunk >= 64 && lds <= 64 * 1024)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mm<TM, TN, true, P>), grid, dim3(256), lds, s, p);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mm<TM, TN, false, P>), grid, dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
