@@ -11,9 +11,8 @@
 //   normalise  normalise_multibeam (get_rms.cu:202-212): z = (x-baseline)/rms,
 //              in place (stage entry point), or fused into the scan's loads.
 //   scan       MatchedFilterPlan::prep (matched_filter.cu:21-32): the exclusive
-//              prefix sum in the reference's sequential order, 8 lanes per
-//              series taking turns over row tiles staged through LDS so that
-//              HBM reads and writes stay coalesced.
+//              prefix sum in a fixed three-level order (segments of 16,
+//              chunks of 1024, chunks in turn), four waves per series.
 #include <algorithm>
 
 #include "hd_device.hpp"
@@ -232,266 +231,149 @@ __global__ __launch_bounds__(256) void k_normalise(float* __restrict__ plane,
   }
 }
 
-// Sequential exclusive scan per series: S[0] = 0, S[k] = S[k-1] + z[k-1].
-//
-// A block owns SC_ROWS series.  Wave 0 runs the dependent adds over a
-// SC_T-sample tile held in LDS while waves 1..8 store the previous tile's
-// prefix sums and load the next tile (double buffer), so the add chain never
-// waits on HBM latency.  Wave 0 gives each series 8 lanes: the tile is walked
-// in steps of 8 * SC_SEG samples, lane (series, seg) holds samples
-// seg*SC_SEG .. seg*SC_SEG + SC_SEG-1 of the step in registers, and the 8
-// segments take turns adding theirs onto the running sum, which a DPP move
-// hands from each lane to the next.  Every sample is still added in order,
-// one float add at a time, and the chain pays one LDS read and one write per
-// SC_SEG samples instead of per 4.
-//
-// When rms is given the loader normalises on the fly, z = (x - baseline)/rms
-// exactly as k_normalise does, so the pipeline never writes the normalised
-// plane back to HBM.
-constexpr int SC_ROWS = 8;
-constexpr int SC_T = 1024;
-constexpr int SC_PAD = 4;
-constexpr int SC_MOVERS = 512;                  // waves 1..8
-constexpr int SC_THREADS = 64 + SC_MOVERS;
-constexpr int SC_VEC = SC_ROWS * SC_T / 4;       // float4 per tile
-constexpr int SC_PER = SC_VEC / SC_MOVERS;       // float4 per mover thread
-// samples per lane and step (32 measured slower)
-constexpr int SC_SEG = 16;
-static_assert(SC_T % (16 * SC_SEG) == 0, "two steps per loop trip");
-
-__device__ __forceinline__ void scan4(float4& v, float& run) {
-  run = run + v.x;
-  v.x = run;
-  run = run + v.y;
-  v.y = run;
-  run = run + v.z;
-  v.z = run;
-  run = run + v.w;
-  v.w = run;
-}
-
-// DPP row moves (within each 16-lane row): lane i takes lane i-1's value
-// (row_shr:1), or lane (i-9) mod 16's (row_ror:9), which for the two 8-lane
-// groups of a row is each group's lane 7 into its lane 0.
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float v) {
-  const int i = __float_as_int(v);
-  return __int_as_float(__builtin_amdgcn_update_dpp(i, i, CTRL, 0xF, 0xF, false));
-}
-
-// One step of the segmented chain: segment k (lane seg == k of each 8-lane
-// group) adds its SEG samples onto run, then run moves one lane on; after
-// the last segment it wraps to the group's lane 0, ready for the next step.
-template <int SEG>
-__device__ __forceinline__ void scan_segments(float4 (&v)[SEG / 4], float& run, int seg) {
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (seg == k) {
-#pragma unroll
-      for (int i = 0; i < SEG / 4; ++i) scan4(v[i], run);
-    }
-    run = k < 7 ? dpp_move<0x111>(run) : dpp_move<0x129>(run);
-  }
-}
-
 constexpr int SC_PMAX = 1024;                    // staged baseline points per block
 constexpr int SC_RMAX = 256;                     // staged beam RMS values per block
 
-// STAGED: the block's baseline points and beam RMS values live in LDS (the
-// normalising loader reads them for every sample); otherwise they are read
-// from global memory.
+// Exclusive scan per series in a fixed three-level order (the oracle's
+// orc_scan follows the same order):
+//   samples in segments of 16, summed in order inside a segment (P, the
+//   inclusive sum up to the sample);
+//   segments in chunks of 64 (1024 samples), their totals summed in order
+//   inside the chunk (Q, the sum of the chunk's earlier segments);
+//   chunks in order along the series (O, the sum of the earlier chunks);
+//   S[0] = 0 and S[k + 1] = O + (Q + P) for sample k.
+// One block of SQ_WAVES waves per series; the waves take the chunks in turn
+// (wave w: chunks w, w + SQ_WAVES, ...).  Lane j holds segment j of the
+// wave's chunk in registers, so P is 16 adds per lane and Q a 64-step chain
+// of wave-uniform adds over the lanes' totals (v_readlane).  Q restarts in
+// every chunk, so the waves' chunks are independent up to O: after each
+// round of SQ_WAVES chunks the waves exchange their chunk totals in LDS (one
+// barrier) and each forms its chunk's O from the round's carry and the
+// earlier chunks' totals, in order.  The dependent chain of a series is a
+// few adds per chunk instead of 1024, and with four waves per series the
+// kernel is bound by its HBM traffic (the plane read once, S written once);
+// each wave loads its next chunk while it sums this one.
+//
+// When rms is given the loads are normalised on the fly, z = (x -
+// baseline)/rms exactly as k_normalise does, so the pipeline never writes the
+// normalised plane back to HBM.
+constexpr int SQ_SEG = 16;
+constexpr int SQ_CHUNK = 64 * SQ_SEG;
+constexpr int SQ_WAVES = 4;  // waves per series
+
 template <bool STAGED>
-__global__ __launch_bounds__(SC_THREADS) void k_scan(const float* __restrict__ z, size_t zpitch,
-                                                     size_t count_arg, int n_series,
-                                                     float* __restrict__ S, size_t spitch,
-                                                     int s_vec_ok, const float* __restrict__ pts,
-                                                     const float* __restrict__ rms,
-                                                     SeriesShape sh_arg,
-                                                     const GroupDev* __restrict__ gd,
-                                                     size_t prow_arg, unsigned nbm_arg) {
-  __shared__ float buf[2][SC_ROWS][SC_T + SC_PAD];
+__global__ __launch_bounds__(64 * SQ_WAVES) void k_scan_seg(
+    const float* __restrict__ z, size_t zpitch, size_t count_arg, int n_series,
+    float* __restrict__ S, size_t spitch, int s_vec_ok, const float* __restrict__ pts,
+    const float* __restrict__ rms, SeriesShape sh_arg, const GroupDev* __restrict__ gd,
+    size_t prow_arg, unsigned nbm_arg) {
   __shared__ float sP[STAGED ? SC_PMAX + 1 : 1];
   __shared__ float sR[STAGED ? SC_RMAX : 1];
+  __shared__ float sT[2][SQ_WAVES];  // chunk totals of a round (two rounds alternate)
   const int tid = threadIdx.x;
-  // the add chain is the critical path: its wave goes first when it shares
-  // a SIMD with a mover wave
-  if (tid < 64) __builtin_amdgcn_s_setprio(3);
-  // blocks: (group, SC_ROWS-row slice of its n_series rows); rows are
-  // g*n_series + r in z, S, pts and rms
-  const int bpg = (n_series + SC_ROWS - 1) / SC_ROWS;
-  const int gz = blockIdx.x / bpg;
-  const int r0 = (blockIdx.x % bpg) * SC_ROWS;
+  const int wave = tid >> 6, lane = tid & 63;
+  // blocks: (group, series); rows are g*n_series + r in z, S, pts and rms
+  const int gz = blockIdx.x / n_series;
+  const size_t rg = (size_t)gz * n_series + blockIdx.x % n_series;
   const SeriesShape sh = gd ? gd[gz].sh : sh_arg;
   const size_t count = gd ? sh.nout : count_arg;
-  const size_t rowbase = (size_t)gz * n_series;
-  z += rowbase * zpitch;
-  S += rowbase * spitch;
   const size_t prow = gd ? prow_arg : (size_t)sh.nbeams * sh.sc;  // baseline points per row
   const unsigned nbm = gd ? nbm_arg : sh.nbeams;                   // RMS values per row
-  if (rms) {
-    pts += rowbase * prow;
-    rms += rowbase * nbm;
-  }
-  const size_t ntiles = (count + SC_T - 1) / SC_T;
-  const bool z_vec = (zpitch % 4 == 0) && ((reinterpret_cast<uintptr_t>(z) & 15) == 0);
+  const float* zr = z + rg * zpitch;
+  float* Sr = S + rg * spitch;
+  const float* Pg = rms ? pts + rg * prow : nullptr;
+  const float* Rg = rms ? rms + rg * nbm : nullptr;
   if (STAGED) {
-    // rows r0.. of pts (plus the float after them, which the stretch may
-    // touch at the very end, as it would in global memory) and of rms
-    const int nrow = min(SC_ROWS, n_series - r0);
-    const size_t np = (size_t)nrow * prow + 1;
-    for (size_t i = tid; i < np; i += SC_THREADS) sP[i] = pts[(size_t)r0 * prow + i];
-    for (size_t i = tid; i < (size_t)nrow * nbm; i += SC_THREADS) sR[i] = rms[(size_t)r0 * nbm + i];
-  }
-
-  auto norm = [&](int row, size_t t, float x) -> float {
-    const float* P = STAGED ? sP + (size_t)row * prow : pts + (size_t)(r0 + row) * prow;
-    const float* R = STAGED ? sR + (size_t)row * nbm : rms + (size_t)(r0 + row) * nbm;
-    return normalised_at(x, P, R, sh, t);
-  };
-  // Tile loads go into registers (nxt) one iteration before they are
-  // normalised and written to LDS, so the barrier between iterations (which
-  // waits for LDS traffic only) does not wait for HBM: the loads of tile p+2
-  // are in flight while the scan lane works through tile p.
-  float4 nxt[SC_PER];
-  auto load_raw = [&](size_t tile) {
-    const int mt = tid - 64;
-    const bool full = z_vec && (tile + 1) * SC_T <= count;
-    // rows past the last series load the last series (zeroed in commit)
-    if (full) {
-#pragma unroll
-      for (int q = 0; q < SC_PER; ++q) {
-        const int idx = mt + SC_MOVERS * q;
-        const int row = idx / (SC_T / 4);
-        const int c4 = (idx % (SC_T / 4)) * 4;
-        const int rr = min(r0 + row, n_series - 1);
-        nxt[q] = *reinterpret_cast<const float4*>(z + (size_t)rr * zpitch + tile * SC_T + c4);
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < SC_PER; ++q) {
-        const int idx = mt + SC_MOVERS * q;
-        const int row = idx / (SC_T / 4);
-        const int c4 = (idx % (SC_T / 4)) * 4;
-        const size_t t = tile * SC_T + c4;
-        nxt[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (r0 + row < n_series) {
-          const float* src = z + (size_t)(r0 + row) * zpitch + t;
-          if (t + 0 < count) nxt[q].x = src[0];
-          if (t + 1 < count) nxt[q].y = src[1];
-          if (t + 2 < count) nxt[q].z = src[2];
-          if (t + 3 < count) nxt[q].w = src[3];
-        }
-      }
-    }
-  };
-  // nxt (tile `tile`) normalised into buf[b]
-  auto commit = [&](size_t tile, int b) {
-    const int mt = tid - 64;
-#pragma unroll
-    for (int q = 0; q < SC_PER; ++q) {
-      const int idx = mt + SC_MOVERS * q;
-      const int row = idx / (SC_T / 4);
-      const int c4 = (idx % (SC_T / 4)) * 4;
-      const size_t t = tile * SC_T + c4;
-      float4 v = nxt[q];
-      if (r0 + row >= n_series) {
-        v = make_float4(0.f, 0.f, 0.f, 0.f);
-      } else if (rms) {
-        if (t + 0 < count) v.x = norm(row, t + 0, v.x);
-        if (t + 1 < count) v.y = norm(row, t + 1, v.y);
-        if (t + 2 < count) v.z = norm(row, t + 2, v.z);
-        if (t + 3 < count) v.w = norm(row, t + 3, v.w);
-      }
-      *reinterpret_cast<float4*>(&buf[b][row][c4]) = v;
-    }
-  };
-  auto store_tile = [&](size_t tile, int b) {
-    const int mt = tid - 64;
-    float4 v[SC_PER];
-#pragma unroll
-    for (int q = 0; q < SC_PER; ++q) {
-      const int idx = mt + SC_MOVERS * q;
-      v[q] = *reinterpret_cast<const float4*>(&buf[b][idx / (SC_T / 4)][(idx % (SC_T / 4)) * 4]);
-    }
-#pragma unroll
-    for (int q = 0; q < SC_PER; ++q) {
-      const int idx = mt + SC_MOVERS * q;
-      const int row = idx / (SC_T / 4);
-      const int c4 = (idx % (SC_T / 4)) * 4;
-      const size_t t = tile * SC_T + c4;
-      if (r0 + row >= n_series) continue;
-      float* dst = S + (size_t)(r0 + row) * spitch + t + 1;
-      if (s_vec_ok && t + 3 < count) {
-        *reinterpret_cast<float4*>(dst) = v[q];
-      } else {
-        if (t + 0 < count) dst[0] = v[q].x;
-        if (t + 1 < count) dst[1] = v[q].y;
-        if (t + 2 < count) dst[2] = v[q].z;
-        if (t + 3 < count) dst[3] = v[q].w;
-      }
-    }
-  };
-
-  if (tid < SC_ROWS && r0 + tid < n_series) S[(size_t)(r0 + tid) * spitch] = 0.f;
-  if (STAGED) __syncthreads();
-  if (tid >= 64 && ntiles > 0) {
-    load_raw(0);
-    commit(0, 0);
-    if (ntiles > 1) load_raw(1);
-  }
-  __syncthreads();
-  float run = 0.f;
-  for (size_t p = 0; p <= ntiles; ++p) {
-    const int b = (int)(p & 1);
-    if (tid < 64) {
-      const int row = tid >> 3, seg = tid & 7;
-      if (p < ntiles && (size_t)SC_T * (p + 1) <= count) {
-        // full tile: lane (row, seg) holds samples 16*seg .. 16*seg + 15 of
-        // each 128-sample step of its row in registers; two steps in
-        // flight (load B, add A, store A, load A', add B ...)
-        constexpr int SEG = SC_SEG;
-        float4* s4 = reinterpret_cast<float4*>(&buf[b][row][SEG * seg]);
-        constexpr int STEP4 = SEG * 8 / 4;  // float4 per step and row
-        float4 va[SEG / 4], vb[SEG / 4];
-#pragma unroll
-        for (int i = 0; i < SEG / 4; ++i) va[i] = s4[i];
-        for (int u = 0; u < SC_T / (8 * SEG); u += 2) {
-#pragma unroll
-          for (int i = 0; i < SEG / 4; ++i) vb[i] = s4[STEP4 * (u + 1) + i];
-          scan_segments<SEG>(va, run, seg);
-#pragma unroll
-          for (int i = 0; i < SEG / 4; ++i) s4[STEP4 * u + i] = va[i];
-          if (u + 2 < SC_T / (8 * SEG)) {
-#pragma unroll
-            for (int i = 0; i < SEG / 4; ++i) va[i] = s4[STEP4 * (u + 2) + i];
-          }
-          scan_segments<SEG>(vb, run, seg);
-#pragma unroll
-          for (int i = 0; i < SEG / 4; ++i) s4[STEP4 * (u + 1) + i] = vb[i];
-        }
-      } else if (p < ntiles && seg == 0) {
-        // the short last tile: the segment-0 lane of each row (which holds
-        // the row's running sum) adds it alone
-        const int ncol = (int)(count - p * SC_T);
-        float* rowp = &buf[b][row][0];
-        float4* row4 = reinterpret_cast<float4*>(rowp);
-        int c = 0;
-        for (; c + 4 <= ncol; c += 4) {
-          float4 v = row4[c / 4];
-          scan4(v, run);
-          row4[c / 4] = v;
-        }
-        for (; c < ncol; ++c) {
-          run = run + rowp[c];
-          rowp[c] = run;
-        }
-      }
-    } else {
-      if (p >= 1) store_tile(p - 1, b ^ 1);
-      if (p + 1 < ntiles) commit(p + 1, b ^ 1);
-      if (p + 2 < ntiles) load_raw(p + 2);
-    }
+    // the row's points (plus the float after them, which the stretch may
+    // touch at the very end, as it would in global memory) and RMS values
+    for (size_t i = tid; i < prow + 1; i += 64 * SQ_WAVES) sP[i] = Pg[i];
+    for (size_t i = tid; i < nbm; i += 64 * SQ_WAVES) sR[i] = Rg[i];
     __syncthreads();
+  }
+  const float* P = STAGED ? sP : Pg;
+  const float* R = STAGED ? sR : Rg;
+  const bool z_vec = (zpitch % 4 == 0) && ((reinterpret_cast<uintptr_t>(z) & 15) == 0);
+  if (tid == 0) Sr[0] = 0.f;
+  const size_t nchunks = (count + SQ_CHUNK - 1) / SQ_CHUNK;
+  const size_t nrounds = (nchunks + SQ_WAVES - 1) / SQ_WAVES;
+  // this lane's segment of chunk c: samples t0 .. t0 + 15
+  auto load = [&](size_t c, float4 (&v)[SQ_SEG / 4]) {
+    const size_t t0 = c * SQ_CHUNK + (size_t)lane * SQ_SEG;
+    if (z_vec && t0 + SQ_SEG <= count) {
+#pragma unroll
+      for (int i = 0; i < SQ_SEG / 4; ++i)
+        v[i] = *reinterpret_cast<const float4*>(zr + t0 + 4 * i);
+    } else {
+#pragma unroll
+      for (int i = 0; i < SQ_SEG / 4; ++i) {
+        const size_t t = t0 + 4 * i;
+        v[i].x = t + 0 < count ? zr[t + 0] : 0.f;
+        v[i].y = t + 1 < count ? zr[t + 1] : 0.f;
+        v[i].z = t + 2 < count ? zr[t + 2] : 0.f;
+        v[i].w = t + 3 < count ? zr[t + 3] : 0.f;
+      }
+    }
+  };
+  float4 va[SQ_SEG / 4], vb[SQ_SEG / 4];
+  if ((size_t)wave < nchunks) load(wave, va);
+  float O = 0.f;  // the chunks before this round, in order (the same in every wave)
+  for (size_t r = 0; r < nrounds; ++r) {
+    const size_t c = r * SQ_WAVES + wave;
+    if (c + SQ_WAVES < nchunks) load(c + SQ_WAVES, vb);
+    const size_t t0 = c * SQ_CHUNK + (size_t)lane * SQ_SEG;
+    const int nv = (c < nchunks && t0 < count) ? (int)min((size_t)SQ_SEG, count - t0) : 0;
+    float x[SQ_SEG];
+#pragma unroll
+    for (int i = 0; i < SQ_SEG / 4; ++i) {
+      x[4 * i + 0] = va[i].x;
+      x[4 * i + 1] = va[i].y;
+      x[4 * i + 2] = va[i].z;
+      x[4 * i + 3] = va[i].w;
+    }
+    // P: in order inside the segment (samples past the series are not added)
+    float p = 0.f;
+#pragma unroll
+    for (int i = 0; i < SQ_SEG; ++i) {
+      if (i < nv) {
+        const float v = rms ? normalised_at(x[i], P, R, sh, t0 + i) : x[i];
+        p = p + v;
+        x[i] = p;
+      }
+    }
+    // Q: the chunk's segment totals in lane order, as a wave-uniform chain
+    float q = 0.f, mine = 0.f;
+#pragma unroll
+    for (int s = 0; s < 64; ++s) {
+      const float ts = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p), s));
+      if (lane == s) mine = q;
+      q = q + ts;
+    }
+    // O of this chunk: the round's carry plus the earlier chunks of the round
+    // in order; a chunk past the series adds a zero total after every real one
+    if (lane == 0) sT[r & 1][wave] = q;
+    __syncthreads();
+    float o = O;
+#pragma unroll
+    for (int w = 0; w < SQ_WAVES; ++w) {
+      const float tw = sT[r & 1][w];
+      if (w < wave) o = o + tw;
+      O = O + tw;
+    }
+#pragma unroll
+    for (int i = 0; i < SQ_SEG; ++i) x[i] = o + (mine + x[i]);
+    float* dst = Sr + t0 + 1;
+    if (s_vec_ok && nv == SQ_SEG) {
+#pragma unroll
+      for (int i = 0; i < SQ_SEG / 4; ++i)
+        *reinterpret_cast<float4*>(dst + 4 * i) =
+            make_float4(x[4 * i], x[4 * i + 1], x[4 * i + 2], x[4 * i + 3]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < SQ_SEG; ++i)
+        if (i < nv) dst[i] = x[i];
+    }
+#pragma unroll
+    for (int i = 0; i < SQ_SEG / 4; ++i) va[i] = vb[i];
   }
 }
 
@@ -589,18 +471,16 @@ hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series
   SeriesShape none{};
   if (rms && !sh && !ga.gd) return hipErrorInvalidValue;
   const SeriesShape one = sh ? *sh : none;
-  const int bpg = (n_series + SC_ROWS - 1) / SC_ROWS;
-  const dim3 grid((unsigned)(bpg * ga.G));
+  const dim3 grid((unsigned)(n_series * ga.G));  // one block per series
   const size_t prow = ga.gd ? ga.prow : (size_t)one.nbeams * one.sc;
   const unsigned nbm = ga.gd ? ga.nbmax : one.nbeams;
-  const bool staged = rms && (size_t)SC_ROWS * prow + 1 <= (size_t)SC_PMAX &&
-                      (size_t)SC_ROWS * nbm <= (size_t)SC_RMAX;
+  const bool staged = rms && prow + 1 <= (size_t)SC_PMAX && nbm <= (size_t)SC_RMAX;
   if (staged)
-    hipLaunchKernelGGL(k_scan<true>, grid, dim3(SC_THREADS), 0, s, z, zpitch, count, n_series, S,
-                       spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
+    hipLaunchKernelGGL(k_scan_seg<true>, grid, dim3(64 * SQ_WAVES), 0, s, z, zpitch, count,
+                       n_series, S, spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
   else
-    hipLaunchKernelGGL(k_scan<false>, grid, dim3(SC_THREADS), 0, s, z, zpitch, count, n_series,
-                       S, spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
+    hipLaunchKernelGGL(k_scan_seg<false>, grid, dim3(64 * SQ_WAVES), 0, s, z, zpitch, count,
+                       n_series, S, spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
   return hipGetLastError();
 }
 

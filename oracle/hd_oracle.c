@@ -19,7 +19,8 @@
  *  - orc_linear_stretch        : closed form on affine inputs, endpoints
  *  - orc_dedisperse            : brute-force triple loop, impulse response,
  *                                DM-0 = channel sum, linearity
- *  - orc_scan / orc_boxcar     : float64 closed forms within rounding, brute force
+ *  - orc_scan / orc_boxcar     : float64 closed forms within rounding, brute force;
+ *                                the three-level order by an independent loop
  *  - orc_remove_baseline       : affine and constant series are removed exactly;
  *                                a numpy restatement, bit for bit
  *  - orc_get_rms_multibeam,    : numpy restatement of both pyramids (tails,
@@ -410,14 +411,30 @@ void orc_normalise_block(float* d, float block_rms, size_t count) {
   for (size_t i = 0; i < count; ++i) d[i] = d[i] * inv;
 }
 
-/* MatchedFilterPlan::prep (matched_filter.cu:21-32): sequential exclusive
- * scan over count+1 inputs; only the first count inputs reach the output. */
+/* MatchedFilterPlan::prep (matched_filter.cu:21-32): exclusive scan over
+ * count+1 inputs; only the first count inputs reach the output.  The
+ * reference's thrust::exclusive_scan on the device adds in a parallel order
+ * that Thrust does not specify; the order adopted here (and by the search's
+ * k_scan_seg) has three levels: samples in segments of 16, summed in order
+ * (P); segments in chunks of 64, their totals summed in order inside the
+ * chunk (Q, the chunk's earlier segments); chunks in order (O, the earlier
+ * chunks); scanned[k + 1] = O + (Q + P) for sample k. */
 void orc_scan(const float* x, size_t count, float* scanned) {
-  float s = 0.f;
-  scanned[0] = s;
-  for (size_t k = 1; k <= count; ++k) {
-    s = s + x[k - 1];
-    scanned[k] = s;
+  scanned[0] = 0.f;
+  float O = 0.f;
+  for (size_t c0 = 0; c0 < count; c0 += 1024) {
+    const size_t ce = c0 + 1024 < count ? c0 + 1024 : count;
+    float Q = 0.f;
+    for (size_t s0 = c0; s0 < ce; s0 += 16) {
+      const size_t se = s0 + 16 < ce ? s0 + 16 : ce;
+      float P = 0.f;
+      for (size_t k = s0; k < se; ++k) {
+        P = P + x[k];
+        scanned[k + 1] = O + (Q + P);
+      }
+      Q = Q + P;
+    }
+    O = O + Q;
   }
 }
 

@@ -148,9 +148,38 @@ def test_scan_and_boxcar_exact_on_integers(oracle):
                 assert out[k] == x[c - behind:c + ahead].sum()
 
 
+def test_scan_three_level_order(oracle):
+    """The adopted addition order, written out independently: segments of 16
+    summed in order, segment totals summed in order inside chunks of 1024,
+    chunk totals in order; S[k+1] = O + (Q + P).  A plainly sequential scan
+    gives other bits on these inputs (checked), so the order itself is pinned,
+    including short last segments and chunks."""
+    f32 = np.float32
+    rng = np.random.default_rng(11)
+    for n in (5000, 1024, 1040, 17, 3):
+        x = rng.standard_normal(n).astype(f32) * f32(100)
+        want = np.zeros(n + 1, f32)
+        O = f32(0)
+        for c0 in range(0, n, 1024):
+            Q = f32(0)
+            for s0 in range(c0, min(c0 + 1024, n), 16):
+                P = f32(0)
+                for k in range(s0, min(s0 + 16, min(c0 + 1024, n))):
+                    P = f32(P + x[k])
+                    want[k + 1] = f32(O + f32(Q + P))
+                Q = f32(Q + P)
+            O = f32(O + Q)
+        got = oracle.scan(x)
+        np.testing.assert_array_equal(got, want)
+        if n == 5000:
+            seq = np.cumsum(x, dtype=f32)
+            assert not np.array_equal(got[1:], seq)
+
+
 def test_scan_sequential_rounding(oracle):
-    """Float32 prefix sums in sequential order stay within the classic
-    accumulated-rounding bound of the exact sums."""
+    """Float32 prefix sums stay within the classic accumulated-rounding bound
+    of the exact sums (the sequential bound holds a fortiori for the
+    three-level order)."""
     x = np.random.default_rng(3).standard_normal(200000).astype(np.float32)
     S = oracle.scan(x).astype(np.float64)
     exact = np.concatenate([[0.0], np.cumsum(x.astype(np.float64))])
