@@ -242,60 +242,71 @@ constexpr int SC_RMAX = 256;                     // staged beam RMS values per b
 //   inside the chunk (Q, the sum of the chunk's earlier segments);
 //   chunks in order along the series (O, the sum of the earlier chunks);
 //   S[0] = 0 and S[k + 1] = O + (Q + P) for sample k.
-// One block of SQ_WAVES waves per series; the waves take the chunks in turn
-// (wave w: chunks w, w + SQ_WAVES, ...).  Lane j holds segment j of the
-// wave's chunk in registers, so P is 16 adds per lane and Q a 64-step chain
-// of wave-uniform adds over the lanes' totals (v_readlane).  Q restarts in
-// every chunk, so the waves' chunks are independent up to O: after each
-// round of SQ_WAVES chunks the waves exchange their chunk totals in LDS (one
-// barrier) and each forms its chunk's O from the round's carry and the
-// earlier chunks' totals, in order.  The dependent chain of a series is a
-// few adds per chunk instead of 1024, and with four waves per series the
+// With few series, one block of SQ_WAVES waves per series; the waves take
+// the chunks in turn (wave w: chunks w, w + SQ_WAVES, ...).  Lane j holds
+// segment j of the wave's chunk in registers, so P is 16 adds per lane and Q
+// a 64-step chain of wave-uniform adds over the lanes' totals (v_readlane).
+// Q restarts in every chunk, so the waves' chunks are independent up to O:
+// after each round of SQ_WAVES chunks the waves exchange their chunk totals
+// in LDS (one barrier) and each forms its chunk's O from the round's carry
+// and the earlier chunks' totals, in order.  The dependent chain of a series
+// is a few adds per chunk instead of 1024, and with four waves per series the
 // kernel is bound by its HBM traffic (the plane read once, S written once);
 // each wave loads its next chunk while it sums this one.
 //
 // When rms is given the loads are normalised on the fly, z = (x -
 // baseline)/rms exactly as k_normalise does, so the pipeline never writes the
 // normalised plane back to HBM.
+// With many series (batches of beams) one wave per series keeps the GPU as
+// busy without the barriers: WPS waves per series, SQ_WAVES / WPS series per
+// block; the order of the additions is the same either way.
 constexpr int SQ_SEG = 16;
 constexpr int SQ_CHUNK = 64 * SQ_SEG;
-constexpr int SQ_WAVES = 4;  // waves per series
+constexpr int SQ_WAVES = 4;  // waves per block
+constexpr size_t SQ_WAVE_SERIES = 4096;  // from this many series on, one wave per series
 
-template <bool STAGED>
+template <bool STAGED, int WPS>
 __global__ __launch_bounds__(64 * SQ_WAVES) void k_scan_seg(
     const float* __restrict__ z, size_t zpitch, size_t count_arg, int n_series,
     float* __restrict__ S, size_t spitch, int s_vec_ok, const float* __restrict__ pts,
     const float* __restrict__ rms, SeriesShape sh_arg, const GroupDev* __restrict__ gd,
     size_t prow_arg, unsigned nbm_arg) {
+  static_assert(WPS == 1 || WPS == SQ_WAVES, "a block's waves share one series or none");
+  constexpr int NSER = SQ_WAVES / WPS;  // series per block
   __shared__ float sP[STAGED ? SC_PMAX + 1 : 1];
   __shared__ float sR[STAGED ? SC_RMAX : 1];
   __shared__ float sT[2][SQ_WAVES];  // chunk totals of a round (two rounds alternate)
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  // blocks: (group, series); rows are g*n_series + r in z, S, pts and rms
-  const int gz = blockIdx.x / n_series;
-  const size_t rg = (size_t)gz * n_series + blockIdx.x % n_series;
+  const int ls = wave / WPS, lw = wave % WPS;  // the wave's series in the block, its turn
+  // blocks: (group, NSER-row slice of its n_series rows); rows are
+  // g*n_series + r in z, S, pts and rms
+  const int bpg = (n_series + NSER - 1) / NSER;
+  const int gz = blockIdx.x / bpg;
+  const int r0 = (blockIdx.x % bpg) * NSER;
+  const bool valid = r0 + ls < n_series;
+  const size_t rg = (size_t)gz * n_series + min(r0 + ls, n_series - 1);
   const SeriesShape sh = gd ? gd[gz].sh : sh_arg;
-  const size_t count = gd ? sh.nout : count_arg;
+  const size_t count = valid ? (gd ? sh.nout : count_arg) : 0;
   const size_t prow = gd ? prow_arg : (size_t)sh.nbeams * sh.sc;  // baseline points per row
   const unsigned nbm = gd ? nbm_arg : sh.nbeams;                   // RMS values per row
   const float* zr = z + rg * zpitch;
   float* Sr = S + rg * spitch;
-  const float* Pg = rms ? pts + rg * prow : nullptr;
-  const float* Rg = rms ? rms + rg * nbm : nullptr;
   if (STAGED) {
-    // the row's points (plus the float after them, which the stretch may
-    // touch at the very end, as it would in global memory) and RMS values
-    for (size_t i = tid; i < prow + 1; i += 64 * SQ_WAVES) sP[i] = Pg[i];
-    for (size_t i = tid; i < nbm; i += 64 * SQ_WAVES) sR[i] = Rg[i];
+    // rows r0.. of pts (plus the float after them, which the stretch may
+    // touch at the very end, as it would in global memory) and of rms
+    const size_t rb = (size_t)gz * n_series + r0;
+    const int nrow = min(NSER, n_series - r0);
+    for (size_t i = tid; i < (size_t)nrow * prow + 1; i += 64 * SQ_WAVES) sP[i] = pts[rb * prow + i];
+    for (size_t i = tid; i < (size_t)nrow * nbm; i += 64 * SQ_WAVES) sR[i] = rms[rb * nbm + i];
     __syncthreads();
   }
-  const float* P = STAGED ? sP : Pg;
-  const float* R = STAGED ? sR : Rg;
+  const float* P = STAGED ? sP + (size_t)ls * prow : (rms ? pts + rg * prow : nullptr);
+  const float* R = STAGED ? sR + (size_t)ls * nbm : (rms ? rms + rg * nbm : nullptr);
   const bool z_vec = (zpitch % 4 == 0) && ((reinterpret_cast<uintptr_t>(z) & 15) == 0);
-  if (tid == 0) Sr[0] = 0.f;
+  if (tid % (64 * WPS) == 0 && valid) Sr[0] = 0.f;
   const size_t nchunks = (count + SQ_CHUNK - 1) / SQ_CHUNK;
-  const size_t nrounds = (nchunks + SQ_WAVES - 1) / SQ_WAVES;
+  const size_t nrounds = (nchunks + WPS - 1) / WPS;
   // this lane's segment of chunk c: samples t0 .. t0 + 15
   auto load = [&](size_t c, float4 (&v)[SQ_SEG / 4]) {
     const size_t t0 = c * SQ_CHUNK + (size_t)lane * SQ_SEG;
@@ -315,11 +326,11 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_scan_seg(
     }
   };
   float4 va[SQ_SEG / 4], vb[SQ_SEG / 4];
-  if ((size_t)wave < nchunks) load(wave, va);
+  if ((size_t)lw < nchunks) load(lw, va);
   float O = 0.f;  // the chunks before this round, in order (the same in every wave)
   for (size_t r = 0; r < nrounds; ++r) {
-    const size_t c = r * SQ_WAVES + wave;
-    if (c + SQ_WAVES < nchunks) load(c + SQ_WAVES, vb);
+    const size_t c = r * WPS + lw;
+    if (c + WPS < nchunks) load(c + WPS, vb);
     const size_t t0 = c * SQ_CHUNK + (size_t)lane * SQ_SEG;
     const int nv = (c < nchunks && t0 < count) ? (int)min((size_t)SQ_SEG, count - t0) : 0;
     float x[SQ_SEG];
@@ -350,14 +361,18 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_scan_seg(
     }
     // O of this chunk: the round's carry plus the earlier chunks of the round
     // in order; a chunk past the series adds a zero total after every real one
-    if (lane == 0) sT[r & 1][wave] = q;
-    __syncthreads();
     float o = O;
+    if (WPS == 1) {
+      O = O + q;
+    } else {  // the block's waves share one series and take the same rounds
+      if (lane == 0) sT[r & 1][lw] = q;
+      __syncthreads();
 #pragma unroll
-    for (int w = 0; w < SQ_WAVES; ++w) {
-      const float tw = sT[r & 1][w];
-      if (w < wave) o = o + tw;
-      O = O + tw;
+      for (int w = 0; w < WPS; ++w) {
+        const float tw = sT[r & 1][w];
+        if (w < lw) o = o + tw;
+        O = O + tw;
+      }
     }
 #pragma unroll
     for (int i = 0; i < SQ_SEG; ++i) x[i] = o + (mine + x[i]);
@@ -471,16 +486,26 @@ hipError_t launch_scan(const float* z, size_t zpitch, size_t count, int n_series
   SeriesShape none{};
   if (rms && !sh && !ga.gd) return hipErrorInvalidValue;
   const SeriesShape one = sh ? *sh : none;
-  const dim3 grid((unsigned)(n_series * ga.G));  // one block per series
   const size_t prow = ga.gd ? ga.prow : (size_t)one.nbeams * one.sc;
   const unsigned nbm = ga.gd ? ga.nbmax : one.nbeams;
-  const bool staged = rms && prow + 1 <= (size_t)SC_PMAX && nbm <= (size_t)SC_RMAX;
-  if (staged)
-    hipLaunchKernelGGL(k_scan_seg<true>, grid, dim3(64 * SQ_WAVES), 0, s, z, zpitch, count,
-                       n_series, S, spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
+  // enough series to fill the GPU with one wave each: no barriers in the loop
+  const bool one_wave = (size_t)n_series * ga.G >= SQ_WAVE_SERIES;
+  const int nser = one_wave ? SQ_WAVES : 1;
+  const bool staged =
+      rms && nser * prow + 1 <= (size_t)SC_PMAX && nser * (size_t)nbm <= (size_t)SC_RMAX;
+  const dim3 grid((unsigned)(((n_series + nser - 1) / nser) * ga.G));
+  const dim3 block(64 * SQ_WAVES);
+#define HD_SCAN_ARGS z, zpitch, count, n_series, S, spitch, s_vec_ok, pts, rms, one, ga.gd, \
+                     ga.prow, ga.nbmax
+  if (one_wave && staged)
+    hipLaunchKernelGGL((k_scan_seg<true, 1>), grid, block, 0, s, HD_SCAN_ARGS);
+  else if (one_wave)
+    hipLaunchKernelGGL((k_scan_seg<false, 1>), grid, block, 0, s, HD_SCAN_ARGS);
+  else if (staged)
+    hipLaunchKernelGGL((k_scan_seg<true, SQ_WAVES>), grid, block, 0, s, HD_SCAN_ARGS);
   else
-    hipLaunchKernelGGL(k_scan_seg<false>, grid, dim3(64 * SQ_WAVES), 0, s, z, zpitch, count,
-                       n_series, S, spitch, s_vec_ok, pts, rms, one, ga.gd, ga.prow, ga.nbmax);
+    hipLaunchKernelGGL((k_scan_seg<false, SQ_WAVES>), grid, block, 0, s, HD_SCAN_ARGS);
+#undef HD_SCAN_ARGS
   return hipGetLastError();
 }
 
