@@ -12,6 +12,34 @@
 #include "hd_internal.hpp"
 #include "hd_plan.hpp"
 
+namespace hd {
+
+bool pick_byte_tile(const std::vector<int>& d, size_t rows, size_t nl, size_t nbits,
+                    DedispGeom& dg) {
+  static const int tiles[][4] = {{32, 8, 1, 0}, {32, 4, 1, 0}, {16, 4, 0, 0}, {8, 4, 0, 0}};
+  for (const auto& t : tiles) {
+    const int db = t[0];
+    int spread = 0;
+    for (size_t m0 = 0; m0 < rows; m0 += db) {
+      const size_t ml = std::min(m0 + db, rows) - 1;
+      for (size_t l = 0; l < nl; ++l)
+        spread = std::max(spread, d[ml * nl + l] - (d[m0 * nl + l] & ~3));
+    }
+    dg.db = db;
+    dg.cc = t[1];
+    dg.wide = t[2];
+    dg.remap = 0;
+    dg.widen8 = nbits <= 4 ? 255 / ((1 << nbits) - 1) : 256;
+    dg.split8 = nbits == 8 ? 1 : 0;
+    dg.wlen8 = dedisp8_window_dwords(spread);
+    dg.wlen = 0;
+    if (dedisp_lds_bytes(dg) <= 160 * 1024) return true;
+  }
+  return false;
+}
+
+}  // namespace hd
+
 namespace {
 
 struct Buf {
@@ -92,31 +120,6 @@ struct DeviceGuard {
   }
 };
 
-// A byte-kernel tile whose staged copies fit the LDS for delays d (rows x
-// nl) of one class: 32 trials x 8 channels on 1024 threads first.
-bool pick_tile(const std::vector<int>& d, size_t rows, size_t nl, size_t nbits,
-               hd::DedispGeom& dg) {
-  static const int tiles[][4] = {{32, 8, 1, 0}, {32, 4, 1, 0}, {16, 4, 0, 0}, {8, 4, 0, 0}};
-  for (const auto& t : tiles) {
-    const int db = t[0];
-    int spread = 0;
-    for (size_t m0 = 0; m0 < rows; m0 += db) {
-      const size_t ml = std::min(m0 + db, rows) - 1;
-      for (size_t l = 0; l < nl; ++l)
-        spread = std::max(spread, d[ml * nl + l] - (d[m0 * nl + l] & ~3));
-    }
-    dg.db = db;
-    dg.cc = t[1];
-    dg.wide = t[2];
-    dg.remap = 0;
-    dg.widen8 = nbits <= 4 ? 255 / ((1 << nbits) - 1) : 256;
-    dg.split8 = nbits == 8 ? 1 : 0;
-    dg.wlen8 = hd::dedisp8_window_dwords(spread);
-    dg.wlen = 0;
-    if (hd::dedisp_lds_bytes(dg) <= 160 * 1024) return true;
-  }
-  return false;
-}
 
 }  // namespace
 
@@ -301,7 +304,7 @@ extern "C" hd_error hd_dedisp_execute(hd_dedisp_plan p, hd_size nsamps, const hd
       dg.xpitch = xpitch;
       dg.ppitch = ppitch;
       const std::vector<int> dc(d.begin() + a * live.size(), d.begin() + b * live.size());
-      if (!pick_tile(dc, b - a, live.size(), in_nbits, dg)) return HD_INVALID_STRIDE;
+      if (!hd::pick_byte_tile(dc, b - a, live.size(), in_nbits, dg)) return HD_INVALID_STRIDE;
       PL_CHECK(hd::launch_dedisperse(p->x[cur].as<uint8_t>(),
                                      p->delays.as<int>() + a * live.size(), nullptr, dg,
                                      p->plane.as<float>() + a * ppitch, s));

@@ -102,6 +102,18 @@ struct hd_pipeline_t {
   std::vector<float> dm_list;
   std::vector<float> kc;
   size_t max_delay = 0;
+  // adaptive time resolution (use_scrunching): each trial's scrunch factor,
+  // the runs of equal factor (first trial, count, factor), and, when any
+  // factor exceeds 1, the delay table with each trial's delays in samples of
+  // its own resolution (the slope divided by the factor)
+  std::vector<size_t> factors;
+  struct ScrunchClass {
+    size_t first, count, factor;
+  };
+  std::vector<ScrunchClass> classes;
+  bool adaptive = false;
+  DevBuf adelays;
+  std::vector<int> adelays_h;
   // device buffers
   DevBuf fb, x, live, delays, plane, pts, rms, brms, S, partial, wrms, winv;
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
@@ -258,7 +270,7 @@ extern "C" void hd_print_usage(void) {
             << p.dm_pulse_width << "]\n"
             << "    -dm_nbits num            number of bits per sample in dedispersed time series ["
             << p.dm_nbits << "]\n"
-            << "    -scrunching num          1 = use adaptive dm-dependent time scrunching (not supported; treated as 0)\n"
+            << "    -scrunching num          1 = use adaptive dm-dependent time scrunching, 0 = off\n"
             << "    -scrunching_tol num      smearing tolerance factor for time scrunching\n"
             << "    -rfi_tol num             RFI exicision threshold limits [" << p.rfi_tol << "]\n"
             << "    -rfi_min_beams num       minimum beams for coincident RFI [" << p.rfi_min_beams
@@ -469,6 +481,19 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   HD_CHECK(pl->delays.ensure(d.size() * sizeof(int)));
   HD_CHECK(hipMemcpy(pl->live.p, pl->live_list.data(), nl * sizeof(int), hipMemcpyHostToDevice));
   HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (pl->adaptive) {
+    // each trial's delays in samples of its own resolution (the slope over a
+    // power of two is exact in float)
+    pl->adelays_h.assign(n_dm * std::max<size_t>(nl, 1), 0);
+    for (size_t m = 0; m < n_dm; ++m) {
+      const float inv = 1.f / (float)pl->factors[m];
+      for (size_t l = 0; l < nl; ++l)
+        pl->adelays_h[m * nl + l] = hdplan::delay(pl->dm_list[m], pl->kc[pl->live_list[l]] * inv);
+    }
+    HD_CHECK(pl->adelays.ensure(pl->adelays_h.size() * sizeof(int)));
+    HD_CHECK(hipMemcpy(pl->adelays.p, pl->adelays_h.data(), pl->adelays_h.size() * sizeof(int),
+                       hipMemcpyHostToDevice));
+  }
   pl->wlen = 0;
   for (int k = 0; k < 9; ++k) {
     int hw = 2, spread = 0;
@@ -521,6 +546,7 @@ hd_error build_coarse(hd_pipeline pl) {
 }
 
 // Shapes of one call (pipeline.cu:488-499, 572-573, 626-692; remove_baseline.cu:30-46)
+// at scrunch factor f (the trial's series holds nsamps_computed / f samples).
 struct CallShape {
   SeriesShape sh;
   WidthTable wt;
@@ -529,30 +555,31 @@ struct CallShape {
   bool skip;  // DM loop breaks before the first trial (no giants)
 };
 
-hd_error make_shapes(hd_pipeline pl, size_t nsamps, size_t nbeams, CallShape& cs) {
+hd_error make_shapes(hd_pipeline pl, size_t nsamps, size_t nbeams, CallShape& cs, size_t f = 1) {
   const hd_params& p = pl->params;
   std::memset(&cs, 0, sizeof(cs));
-  const size_t max_delay = pl->max_delay;
-  const size_t W = p.boxcar_max;
+  const size_t delay = pl->max_delay / f;  // cur_dm_delay (:493-497)
+  const size_t W = p.boxcar_max / f;       // rel_boxcar_max (:626)
   SeriesShape& sh = cs.sh;
   sh.nbeams = (unsigned)nbeams;
-  sh.stride = nsamps / nbeams;
-  sh.nout = nsamps - max_delay;
-  cs.skip = (max_delay >= sh.stride) || (max_delay + W > sh.stride);
+  sh.stride = nsamps / nbeams / f;          // cur_beam_stride (:498)
+  sh.nout = (nsamps - pl->max_delay) / f;   // cur_nsamps (:490)
+  cs.skip = (delay >= sh.stride) || (delay + W > sh.stride);  // :501-506, 633-638
   if (cs.skip) return HD_NO_ERROR;
-  sh.bc = sh.stride - max_delay;
+  sh.bc = sh.stride - delay;
   const int TB = hd::dedisp_tile_time();
   sh.ppitch = round_up(sh.nout + 1, (size_t)TB);
   sh.spitch = round_up(sh.nout + 1, 64);
   {
-    const int e = hd::series_shape(p.dt * 1.0f, p.baseline_length, sh);
+    const int e = hd::series_shape(p.dt * (float)f, p.baseline_length, sh);  // cur_dt (:491)
     if (e != HD_NO_ERROR) return (hd_error)e;
   }
-  // widths (pipeline.cu:657-692)
+  // widths (pipeline.cu:657-692): filter_width = f, 2f, .. <= boxcar_max, that
+  // is relative widths w = 1, 2, .. <= boxcar_max / f
   WidthTable& wt = cs.wt;
   wt.off = (unsigned)(W / 2);
   const size_t max_nsamps_filtered = sh.nout + 1 - W;
-  const size_t rel_min_ts = std::max<size_t>(p.min_tscrunch_width, 1);
+  const size_t rel_min_ts = std::max<size_t>(p.min_tscrunch_width / f, 1);
   int nw = 0;
   for (size_t w = 1; w <= W; w *= 2) {
     if (nw >= hd::kMaxWidths) return HD_INVALID_STRIDE;
@@ -563,15 +590,16 @@ hd_error make_shapes(hd_pipeline pl, size_t nsamps, size_t nbeams, CallShape& cs
     wi.ahead = (unsigned)((w - 1) / 2 + 1);
     wi.behind = (unsigned)(w / 2);
     wi.n_w = (unsigned)((max_nsamps_filtered - 1) / ts + 1);
-    wi.filter_idx = log2_pow2((unsigned)w);
+    wi.filter_idx = log2_pow2((unsigned)(w * f));
     wi.merge_dist = (unsigned)(p.cand_sep_time * (w / ts));
     wi.rel_off = (unsigned)((W / 2) / ts);
+    wi.cs = (unsigned)(ts * f);
   }
   wt.nw = nw;
   // The first width must produce fresh output at every block-RMS position;
   // otherwise its RMS would read the previous trial's buffer contents.
   if (nw == 0 || wt.wi[0].ts != 1) return HD_INVALID_STRIDE;
-  cs.bc692 = sh.stride - (max_delay + W);
+  cs.bc692 = sh.stride - (delay + W);
   BlockRmsShape& bs = cs.bs;
   bs.bc = cs.bc692;
   bs.n = nbeams * cs.bc692;
@@ -1109,16 +1137,34 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   ga.prow = std::max<size_t>(prow, 1);
   const SeriesShape& sh0 = pl->hgd[0].sh;
 
+  // Adaptive time resolution searches each run of equal scrunch factor from
+  // the input at that resolution; the gated search has no such path.
+  if (pl->adaptive && (pl->gating || pl->coarse_only)) return HD_INVALID_PIPELINE;
+  const bool adaptive = pl->adaptive;
+  int n_levels = 1;  // input resolutions: level l holds the input halved l times
+  if (adaptive)
+    for (const auto& k : pl->classes)
+      if (k.factor <= W) n_levels = std::max(n_levels, (int)log2_pow2((unsigned)k.factor) + 1);
+
   record(pl, 0);
   // unpack to channel-major bytes
   const size_t xpitch = round_up(ppitch + max_delay + 2 * pl->wlen + 256, 256);
-  HD_CHECK(pl->x.ensure((size_t)G * std::max(n_live, 1) * xpitch));
+  HD_CHECK(pl->x.ensure((size_t)n_levels * G * std::max(n_live, 1) * xpitch));
   // (the proposal's decimation reads whole rows, so rows are trimmed only
   // for the plain search of several gulps)
   HD_CHECK(hd::launch_unpack(nullptr, p.nchans * nbits / 8, (int)nbits, 0, pl->live.as<int>(),
                              n_live, pl->x.as<uint8_t>(), xpitch, s, ga,
                              (size_t)n_live == p.nchans,
                              G > 1 && !(pl->gating || pl->coarse_only)));
+  // level l, slots l*G .. l*G + G - 1: level l - 1 halved (pairwise means
+  // rounded down, zeros past the halved samples)
+  for (int l = 1; l < n_levels && n_live > 0; ++l)
+    for (int g = 0; g < G; ++g) {
+      const size_t slot = (size_t)n_live * xpitch;
+      HD_CHECK(hd::launch_scrunch2(pl->x.as<uint8_t>() + ((size_t)(l - 1) * G + g) * slot, n_live,
+                                   xpitch, pl->hgd[g].nsamps >> (l - 1),
+                                   pl->x.as<uint8_t>() + ((size_t)l * G + g) * slot, s));
+    }
   record(pl, 10);
   // The search runs over row groups: a whole gulp each, or with gating one
   // per proposed unit of HD_PROPOSAL_UNIT trials (rows past the last trial
@@ -1151,11 +1197,21 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     hrg = pl->hgd;
   }
   record(pl, 1);
-  const int Gr = (int)hrg.size();
-  if (Gr == 0) {  // nothing proposed: no giants, no candidates
+  if (hrg.empty() && !adaptive) {  // nothing proposed: no giants, no candidates
     for (int ev = 2; ev <= 8; ++ev) record(pl, ev);
     return HD_NO_ERROR;
   }
+  // The stage chain (dedispersion .. thresholding) and the giants of row
+  // groups hrg of R rows each: width table wt (shared by the groups but for
+  // their fresh counts), baseline points per row prow, unpacked input xb,
+  // delay table dly, launch shape sh0; the dedispersion tile fixed (tile) or
+  // the autotuner's.  Leaves the giants in g_* and their count in n_giants.
+  auto search_rows = [&](std::vector<hd::GroupDev>& hrg, const int R, const WidthTable& wt,
+                         const size_t prow, const uint8_t* xb, const int* dly,
+                         const hd::DedispGeom* tile, const SeriesShape& sh0,
+                         size_t& n_giants) -> hd_error {
+  const int Gr = (int)hrg.size();
+  n_giants = 0;
   HD_CHECK(pl->rdev.ensure((size_t)Gr * sizeof(hd::GroupDev)));
   HD_CHECK(hipMemcpyAsync(pl->rdev.p, hrg.data(), (size_t)Gr * sizeof(hd::GroupDev),
                           hipMemcpyHostToDevice, s));
@@ -1163,6 +1219,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   gr.gd = pl->rdev.as<hd::GroupDev>();
   gr.host = hrg.data();
   gr.G = Gr;
+  gr.prow = prow;
   const size_t rows = (size_t)Gr * R;
   // Every buffer of the stage chain (dedispersion .. thresholding) is sized
   // first: the chain may be captured as a graph, which must not allocate.
@@ -1184,7 +1241,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   hd::GiantBuffers gb{};
   gb.chunk = hd::threshold_chunk();
   size_t n_w0 = 0;
-  for (int g = 0; g < G; ++g) n_w0 = std::max<size_t>(n_w0, pl->hgd[g].n_w[0]);
+  for (const hd::GroupDev& d : hrg) n_w0 = std::max<size_t>(n_w0, d.n_w[0]);
   gb.nchunks_per_w = (n_w0 + gb.chunk - 1) / gb.chunk;
   const size_t nkeys = rows * wt.nw * gb.nchunks_per_w;
   HD_CHECK(pl->ccount.ensure(nkeys * sizeof(unsigned)));
@@ -1215,7 +1272,16 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   dg.ppitch = ppitch;
   dg.wlen = (int)pl->wlen;
   const int* pairs = nullptr;
-  {
+  if (tile) {
+    dg.db = tile->db;
+    dg.cc = tile->cc;
+    dg.remap = tile->remap;
+    dg.wlen = tile->wlen;
+    dg.widen8 = tile->widen8;
+    dg.wlen8 = tile->wlen8;
+    dg.split8 = tile->split8;
+    dg.wide = tile->wide;
+  } else {
     hd_error te = choose_dedisp_tile(pl, dg, gr, nbits, s, &pairs);  // may time candidates
     if (te != HD_NO_ERROR) return te;
   }
@@ -1228,8 +1294,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   };
   // The chain: kernel launches only.  With `timed`, stage events in between.
   auto chain = [&](bool timed) -> hipError_t {
-    hipError_t e = hd::launch_dedisperse(pl->x.as<uint8_t>(), pl->delays.as<int>(), pairs, dg,
-                                         pl->plane.as<float>(), s, gr);
+    hipError_t e = hd::launch_dedisperse(xb, dly, pairs, dg, pl->plane.as<float>(), s, gr);
     if (e != hipSuccess) return e;
     if (timed) record(pl, 2);
     e = hd::launch_baseline_points(pl->plane.as<float>(), sh0, R, pl->pts.as<float>(), s, gr);
@@ -1260,7 +1325,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     if (timed) record(pl, 6);
     return threshold();
   };
-  if (pl->use_graph && !(pl->gating || pl->coarse_only)) {
+  if (pl->use_graph && !tile && !(pl->gating || pl->coarse_only)) {
     // Graph of the chain, captured once per set of launch arguments (shapes,
     // tile, buffers, the row-group table without its input pointers) and
     // replayed while they repeat.  The row-group table itself is uploaded
@@ -1331,7 +1396,6 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     pl->sample_cap = round_up((size_t)nsamp + nsamp / 4, 1 << 20);
   }
   if (nsamp > pl->sample_cap) return HD_MEM_ALLOC_FAILED;
-  size_t n_giants = 0;
   if (nsamp > 0) {
     HD_CHECK(hd::launch_exclusive_scan_u32(gb.ccount, gb.cpos, nkeys, gb.scan_tmp, s));
     HD_CHECK(pl->okey.ensure((size_t)nsamp * sizeof(uint64_t)));
@@ -1367,6 +1431,105 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     HD_CHECK(pl->g_best.ensure(std::max<size_t>(n_giants, 1) * sizeof(uint64_t)));
     HD_CHECK(hd::launch_segment_reduce(gb, wt, nsamp, (unsigned)n_giants, R, go,
                                        pl->g_best.as<uint64_t>(), s, gr.gd));
+  }
+  return HD_NO_ERROR;
+  };
+  size_t n_giants = 0;
+  if (!adaptive) {
+    const hd_error e = search_rows(hrg, R, wt, ga.prow, pl->x.as<uint8_t>(),
+                                   pl->delays.as<int>(), nullptr, sh0, n_giants);
+    if (e != HD_NO_ERROR) return e;
+  } else {
+    // Adaptive time resolution (pipeline.cu:488-790): the runs of trials of
+    // equal scrunch factor in turn, each over every gulp with the shapes of
+    // its factor, from the input at its resolution.  The giants are gathered
+    // per gulp in trial order, as the reference's DM loop appends them.
+    std::vector<float> a_peak;
+    std::vector<uint64_t> a_ind, a_begin, a_end;
+    std::vector<uint32_t> a_filter, a_dm, a_group;
+    const size_t nl = std::max(n_live, 1);
+    for (const auto& k : pl->classes) {
+      if (k.factor > W) break;  // no width to search (:657-659), here or later
+      std::vector<hd::GroupDev> crg;
+      WidthTable cwt{};
+      size_t cprow = 1;
+      for (int g = 0; g < G; ++g) {
+        CallShape cs;
+        const hd_error e =
+            make_shapes(pl, pl->hgd[g].nsamps, pl->hgd[g].sh.nbeams, cs, k.factor);
+        if (e != HD_NO_ERROR) return e;
+        if (cs.skip) continue;
+        hd::GroupDev d = pl->hgd[g];
+        d.sh = cs.sh;
+        d.sh.ppitch = ppitch;
+        d.sh.spitch = spitch;
+        d.bs = cs.bs;
+        d.bc692 = cs.bc692;
+        for (int j = 0; j < hd::kMaxWidths; ++j) d.n_w[j] = j < cs.wt.nw ? cs.wt.wi[j].n_w : 0;
+        d.trial0 = (uint32_t)k.first;
+        d.rows_valid = (uint32_t)k.count;
+        if (crg.empty()) cwt = cs.wt;
+        cprow = std::max(cprow, (size_t)cs.sh.nbeams * cs.sh.sc);
+        crg.push_back(d);
+      }
+      if (crg.empty()) continue;
+      hd::DedispGeom tile;
+      std::memset(&tile, 0, sizeof(tile));
+      const std::vector<int> dc(pl->adelays_h.begin() + k.first * nl,
+                                pl->adelays_h.begin() + (k.first + k.count) * nl);
+      if (!hd::pick_byte_tile(dc, k.count, nl, nbits, tile)) return HD_INVALID_STRIDE;
+      tile.remap = 1;
+      const uint8_t* xb =
+          pl->x.as<uint8_t>() + (size_t)log2_pow2((unsigned)k.factor) * G * nl * xpitch;
+      size_t ng = 0;
+      const hd_error e = search_rows(crg, (int)k.count, cwt, cprow, xb, pl->adelays.as<int>(),
+                                     &tile, crg[0].sh, ng);
+      if (e != HD_NO_ERROR) return e;
+      if (ng == 0) continue;
+      const size_t o = a_peak.size();
+      a_peak.resize(o + ng); a_ind.resize(o + ng); a_begin.resize(o + ng); a_end.resize(o + ng);
+      a_filter.resize(o + ng); a_dm.resize(o + ng); a_group.resize(o + ng);
+      HD_CHECK(hipMemcpyAsync(a_peak.data() + o, pl->g_peak.p, ng * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_ind.data() + o, pl->g_ind.p, ng * 8, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_begin.data() + o, pl->g_begin.p, ng * 8, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_end.data() + o, pl->g_end.p, ng * 8, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_filter.data() + o, pl->g_filter.p, ng * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_dm.data() + o, pl->g_dm.p, ng * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_group.data() + o, pl->g_group.p, ng * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipStreamSynchronize(s));
+    }
+    // the stage events of the runs are not split: the whole search counts as
+    // dedispersion
+    for (int ev = 2; ev <= 6; ++ev) record(pl, ev);
+    n_giants = a_peak.size();
+    if (n_giants > 0) {
+      // gulp by gulp (stable: each gulp's giants stay in trial order)
+      std::vector<size_t> ord(n_giants);
+      for (size_t i = 0; i < n_giants; ++i) ord[i] = i;
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](size_t a, size_t b) { return a_group[a] < a_group[b]; });
+      auto permute = [&](auto& v) {
+        auto c = v;
+        for (size_t i = 0; i < n_giants; ++i) v[i] = c[ord[i]];
+      };
+      permute(a_peak); permute(a_ind); permute(a_begin); permute(a_end);
+      permute(a_filter); permute(a_dm); permute(a_group);
+      HD_CHECK(pl->g_peak.ensure(n_giants * sizeof(float)));
+      HD_CHECK(pl->g_ind.ensure(n_giants * sizeof(uint64_t)));
+      HD_CHECK(pl->g_begin.ensure(n_giants * sizeof(uint64_t)));
+      HD_CHECK(pl->g_end.ensure(n_giants * sizeof(uint64_t)));
+      HD_CHECK(pl->g_filter.ensure(n_giants * sizeof(uint32_t)));
+      HD_CHECK(pl->g_dm.ensure(n_giants * sizeof(uint32_t)));
+      HD_CHECK(pl->g_group.ensure(n_giants * sizeof(uint32_t)));
+      HD_CHECK(hipMemcpyAsync(pl->g_peak.p, a_peak.data(), n_giants * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_ind.p, a_ind.data(), n_giants * 8, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_begin.p, a_begin.data(), n_giants * 8, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_end.p, a_end.data(), n_giants * 8, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_filter.p, a_filter.data(), n_giants * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_dm.p, a_dm.data(), n_giants * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipMemcpyAsync(pl->g_group.p, a_group.data(), n_giants * 4, hipMemcpyHostToDevice, s));
+      HD_CHECK(hipStreamSynchronize(s));
+    }
   }
   pl->n_giants = n_giants;
   record(pl, 7);
@@ -1603,12 +1766,8 @@ extern "C" hd_error hd_create_pipeline(hd_pipeline* pipeline_, hd_params params)
   DeviceGuard dg(pl->device);
   if (params.verbosity >= 1)
     std::cout << "Process " << params.beam << " using GPU " << params.gpu_id << std::endl;
-  // The reference enables dedisp's adaptive time scrunching by default
-  // (default_params.C:34, pipeline.cu:187-196); this search always runs at
-  // full time resolution, so say so whatever the verbosity.
-  if (params.use_scrunching)
-    std::cerr << "heimdall: adaptive time scrunching is not available; searching every DM "
-                 "trial at full time resolution (-scrunching 0 silences this)" << std::endl;
+  if (params.use_scrunching && params.verbosity >= 2)
+    std::cout << "Enabling adaptive dt" << std::endl;  // pipeline.cu:187-196
   hipError_t e = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void**)&pl->h_pinned, 64, hipHostMallocDefault);
   for (int k = 0; k <= ST_COUNT && e == hipSuccess; ++k) e = hipEventCreate(&pl->ev[k]);
@@ -1623,6 +1782,25 @@ extern "C" hd_error hd_create_pipeline(hd_pipeline* pipeline_, hd_params params)
   for (size_t c = 0; c < params.nchans; ++c) {
     const int d = hdplan::delay(pl->dm_list.back(), pl->kc[c]);
     if (d > 0 && (size_t)d > pl->max_delay) pl->max_delay = (size_t)d;
+  }
+  // scrunch factors (dedisp_enable_adaptive_dt with dm_pulse_width and
+  // scrunch_tol, pipeline.cu:187-196, 344-360)
+  if (params.use_scrunching)
+    pl->factors = hdplan::scrunch_list(pl->dm_list, params.dt, params.dm_pulse_width, params.f0,
+                                       params.df, params.nchans, params.scrunch_tol);
+  else
+    pl->factors.assign(pl->dm_list.size(), 1);
+  for (size_t a = 0; a < pl->factors.size();) {
+    size_t b = a;
+    while (b < pl->factors.size() && pl->factors[b] == pl->factors[a]) ++b;
+    pl->classes.push_back({a, b - a, pl->factors[a]});
+    a = b;
+  }
+  pl->adaptive = pl->classes.size() > 1;
+  if (params.verbosity >= 2) {
+    std::cout << "Scrunch factors:" << std::endl;
+    for (size_t f : pl->factors) std::cout << f << " ";
+    std::cout << std::endl;
   }
   *pipeline_ = pl;
   return HD_NO_ERROR;

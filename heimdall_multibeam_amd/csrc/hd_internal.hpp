@@ -44,6 +44,7 @@ struct WidthInfo {
   unsigned filter_idx; // log2 w
   unsigned merge_dist; // cand_sep_time * (w/ts)
   unsigned rel_off;    // (boxcar_max/2)/ts
+  unsigned cs;         // cur_scrunch = scrunch factor * ts: output index to sample index
 };
 
 constexpr int kMaxWidths = 24;
@@ -166,6 +167,12 @@ int dedisp_tile_dms();      // default DMs per block
 int dedisp_window_dwords(int spread);  // staged dwords per copy for a delay span
 int dedisp8_window_dwords(int spread); // the same for the 8-bit-lane kernel
 int dedisp_chunk_channels();           // default channels staged per step
+// A byte-staged tile (widen8 > 0, remap 0) whose staged copies fit the LDS for
+// the delays d (rows x nl, blocks from row 0) of one launch: 32 trials x 8
+// channels on 1024 threads first, then 32 x 4, 16 x 4 and 8 x 4.  False when
+// none fits.
+bool pick_byte_tile(const std::vector<int>& d, size_t rows, size_t nl, size_t nbits,
+                    DedispGeom& dg);
 
 hipError_t launch_baseline_points(const float* plane, const SeriesShape& sh, int n_dm,
                                   float* pts, hipStream_t s, const GroupArgs& ga = GroupArgs{});

@@ -683,7 +683,7 @@ __global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp
   const uint64_t dmw = key >> 32;
   const unsigned j = (unsigned)(dmw % (uint64_t)wt.nw);
   const WidthInfo wi = wt.wi[j];
-  const uint64_t ro = wi.rel_off, sc = wi.ts;
+  const uint64_t ro = wi.rel_off, sc = wi.cs;
   const uint64_t i = key & 0xffffffffull;
   if (h) {
     const uint64_t row = dmw / (uint64_t)wt.nw;  // group * n_dm + trial
@@ -704,7 +704,7 @@ __global__ void k_segment_finish(GiantBuffers gb, WidthTable wt, unsigned ngiant
   const uint64_t key = gb.okey[k];
   const WidthInfo wi = wt.wi[(unsigned)((key >> 32) % (uint64_t)wt.nw)];
   go.peak[g] = gb.oval[k];
-  go.ind[g] = ((key & 0xffffffffull) + wi.rel_off) * (uint64_t)wi.ts;
+  go.ind[g] = ((key & 0xffffffffull) + wi.rel_off) * (uint64_t)wi.cs;
 }
 
 __global__ void k_zero_u64(uint64_t* p, size_t n) {

@@ -488,6 +488,7 @@ extern "C" hd_error hd_find_giants(const hd_float* d_data, hd_size count, hd_siz
   wt.off = 0;
   wt.wi[0].w = 1;
   wt.wi[0].ts = 1;
+  wt.wi[0].cs = 1;
   wt.wi[0].merge_dist = (unsigned)merge_dist;
   ST_CHECK(launch_segment_heads(gb, wt, nsamp, s));
   ST_CHECK(launch_exclusive_scan_u32(gb.head, gb.segid, nsamp, gb.scan_tmp, s));

@@ -28,9 +28,16 @@
  *  - orc_find_giants           : brute-force run detection
  *  - orc_label_clusters        : brute-force DAG reachability definition
  *  - orc_merge_candidates      : brute-force group reduction
+ *  - orc_scrunch_factors,      : equal to the numpy restatement of
+ *    orc_dedisperse_adaptive     oracle/adaptive_dt.py, whose invariants
+ *                                (doubling criterion in float64, brute-force
+ *                                halved sums, all-ones = orc_dedisperse) are
+ *                                tested; factor-1 trials of orc_execute with
+ *                                scrunching equal the plain search's
  * Parity unpinned (no reference vector exists): whether dedisp rounds delays
  * as int(dm*k + 0.5f) (the closed form fixes the constant and units, not the
- * rounding rule), its output scaling, and hash32's output values.
+ * rounding rule), its output scaling, its adaptive-dt rule and halving
+ * (dedisp is absent), and hash32's output values.
  */
 #include "hd_oracle.h"
 
@@ -194,6 +201,82 @@ int orc_dedisperse(const uint8_t* in, size_t nsamps, size_t nbits, size_t nchans
   free(x);
   free(live);
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adaptive time resolution: dedisp's smearing rule and halving (dedisp is
+ * absent from the reference tree; pipeline.cu:187-196 enables it and
+ * pipeline.cu:489-498 consumes its factors).  Parity unpinned: the numpy
+ * restatement in oracle/adaptive_dt.py and the invariants of
+ * tests/test_adaptive_dt.py are its checks. */
+static float smearing(float dt, float width, float f0, size_t nchans, float df, float dm,
+                      float ddm) {
+  const float bw = (float)nchans * fabsf(df);
+  const float fc = f0 - bw / 2.f;
+  const float inv_fc3 = 1.f / (fc * fc * fc);
+  const float t_dm = 8.3f * bw * dm * inv_fc3;
+  const float t_ddm = 8.3f / 4.f * bw * (float)nchans * ddm * inv_fc3;
+  return sqrtf(dt * dt + width * width + t_dm * t_dm + t_ddm * t_ddm);
+}
+
+void orc_scrunch_factors(const float* dms, size_t n, float dt, float pulse_width_us, float f0,
+                         float df, size_t nchans, float tol, size_t* out) {
+  if (n == 0) return;
+  const float w = pulse_width_us * 1e-6f;
+  out[0] = 1;
+  for (size_t d = 1; d < n; ++d) {
+    const float ddm = dms[d] - dms[d - 1];
+    const float cur = (float)out[d - 1] * dt;
+    const float a = smearing(cur, w, f0, nchans, df, dms[d], ddm);
+    const float b = smearing(2.f * cur, w, f0, nchans, df, dms[d], ddm);
+    out[d] = (b / a < tol) ? 2 * out[d - 1] : out[d - 1];
+  }
+}
+
+int orc_dedisperse_adaptive(const uint8_t* in, size_t nsamps, size_t nbits, size_t nchans,
+                            const int* killmask, const float* dm_list, size_t n_dm,
+                            const float* delay_table, size_t max_delay, const size_t* factors,
+                            float* out, size_t out_stride, int nthreads) {
+  if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8)) return -1;
+  if (max_delay >= nsamps) return -2;
+  enum { PAD = 4 };
+  const size_t nout = nsamps - max_delay;
+  const size_t stride = nchans * nbits / 8;
+  /* unpacked samples, [time][channel], PAD zero samples after the last */
+  uint8_t* x = (uint8_t*)calloc((nsamps + PAD) * (nchans ? nchans : 1), 1);
+  float* kcs = (float*)malloc(sizeof(float) * (nchans ? nchans : 1));
+  for (size_t t = 0; t < nsamps; ++t)
+    for (size_t c = 0; c < nchans; ++c)
+      x[t * nchans + c] = (uint8_t)sample_at(in + t * stride, c, nbits);
+  size_t level = 1, ns = nsamps;
+  int rc = 0;
+  for (size_t a = 0; a < n_dm && rc == 0;) {
+    const size_t f = factors[a];
+    size_t b = a;
+    while (b < n_dm && factors[b] == f) ++b;
+    while (level < f) { /* halve: pairwise means rounded down */
+      const size_t h = ns / 2;
+      for (size_t t = 0; t < h; ++t)
+        for (size_t c = 0; c < nchans; ++c)
+          x[t * nchans + c] =
+              (uint8_t)(((unsigned)x[2 * t * nchans + c] + x[(2 * t + 1) * nchans + c]) >> 1);
+      memset(x + h * nchans, 0, (ns - h + PAD) * nchans);
+      ns = h;
+      level *= 2;
+    }
+    const size_t nout_s = nout / f;
+    for (size_t d = a; d < b; ++d) memset(out + d * out_stride, 0, sizeof(float) * nout);
+    if (nout_s > 0) {
+      /* the slope over a power of two is exact in float */
+      for (size_t c = 0; c < nchans; ++c) kcs[c] = delay_table[c] / (float)f;
+      rc = orc_dedisperse(x, ns + PAD, 8, nchans, killmask, dm_list + a, b - a, kcs,
+                          ns + PAD - nout_s, out + a * out_stride, out_stride, nthreads);
+    }
+    a = b;
+  }
+  free(kcs);
+  free(x);
+  return rc;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -588,6 +671,8 @@ void orc_default_params(orc_params* p) {
   p->min_tscrunch_width = 4096;
   p->nthreads = 1;
   p->rfi_tol = 5.0f;
+  p->use_scrunching = 0;
+  p->scrunch_tol = 1.15f;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -785,13 +870,16 @@ struct dm_ctx {
   size_t nsamps, nbeams, nsamps_computed, max_delay, W;
 };
 
-/* One DM trial of the loop at pipeline.cu:488-790 (scrunch factor 1).
- * Appends its giants to r; returns 1 when the giant-rate limit trips. */
+/* One DM trial of the loop at pipeline.cu:488-790 at scrunch factor
+ * cur_dm_scrunch (the trial's row holds nsamps_computed / cur_dm_scrunch
+ * samples).  Appends its giants to r; returns 1 when the giant-rate limit
+ * trips. */
 static int process_dm(const struct dm_ctx* c, struct dm_state* st, size_t dm_idx,
-                      const float* row, orc_result* r, size_t* gcap) {
+                      const float* row, orc_result* r, size_t* gcap, size_t cur_dm_scrunch) {
   const orc_params* p = c->p;
   const size_t W = c->W, nbeams = c->nbeams, nsamps = c->nsamps;
-  size_t cur_dm_scrunch = 1;
+  /* A factor above boxcar_max searches no width (:657-659): no giants */
+  if (cur_dm_scrunch > W) return 0;
   size_t cur_nsamps = c->nsamps_computed / cur_dm_scrunch;  /* :490 */
   float cur_dt = p->dt * (float)cur_dm_scrunch;             /* :491 */
   size_t cur_dm_delay = c->max_delay / cur_dm_scrunch;      /* :493-497 */
@@ -849,8 +937,8 @@ static int process_dm(const struct dm_ctx* c, struct dm_state* st, size_t dm_idx
   return 0;
 }
 
-/* hd_execute (pipeline.cu:233-1087) without the file/TCP output, with the
- * scrunch factor fixed at 1 (adaptive dt off). */
+/* hd_execute (pipeline.cu:233-1087) without the file/TCP output; with
+ * use_scrunching, at the adaptive time resolution of orc_scrunch_factors. */
 int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
                 size_t nbits, size_t first_idx, size_t nbeams, orc_result* r) {
   (void)first_idx;
@@ -890,6 +978,22 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
   float* sub_dm = (float*)malloc(sizeof(float) * (n_sub ? n_sub : 1));
   n_sub = 0;
   for (size_t d = dm_first; d < dm_end; d += step) { sub_idx[n_sub] = d; sub_dm[n_sub++] = dm_list[d]; }
+  /* scrunch factors of the whole list (pipeline.cu:187-196, 344-345), then
+   * of the trials processed */
+  size_t* sub_fac = (size_t*)malloc(sizeof(size_t) * (n_sub ? n_sub : 1));
+  int adaptive = 0;
+  {
+    size_t* fac = (size_t*)malloc(sizeof(size_t) * n_dm);
+    for (size_t d = 0; d < n_dm; ++d) fac[d] = 1;
+    if (p->use_scrunching)
+      orc_scrunch_factors(dm_list, n_dm, p->dt, p->dm_pulse_width, p->f0, p->df, nchans,
+                          p->scrunch_tol, fac);
+    for (size_t k = 0; k < n_sub; ++k) {
+      sub_fac[k] = fac[sub_idx[k]];
+      if (sub_fac[k] != 1) adaptive = 1;
+    }
+    free(fac);
+  }
   float* plane = (float*)malloc(sizeof(float) * (n_sub ? n_sub : 1) * series_stride);
   /* CLEANING build (pipeline.cu:263-299): zero-DM cleaning, boxcars to 64 */
   uint8_t* cleaned = NULL;
@@ -902,14 +1006,20 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
     if (rc != 0) {
       r->error = rc == -1 ? 7 : 11;
       free(cleaned); free(plane); free(dm_list); free(kc); free(sub_idx); free(sub_dm);
+      free(sub_fac);
       return r->error;
     }
     fb = cleaned;
   }
-  if (orc_dedisperse(fb, nsamps, nbits, nchans, p->killmask, sub_dm, n_sub, kc, max_delay,
-                     plane, series_stride, p->nthreads) != 0) {
+  const int drc =
+      adaptive ? orc_dedisperse_adaptive(fb, nsamps, nbits, nchans, p->killmask, sub_dm, n_sub,
+                                         kc, max_delay, sub_fac, plane, series_stride,
+                                         p->nthreads)
+               : orc_dedisperse(fb, nsamps, nbits, nchans, p->killmask, sub_dm, n_sub, kc,
+                                max_delay, plane, series_stride, p->nthreads);
+  if (drc != 0) {
     r->error = 8; free(plane); free(dm_list); free(kc); free(sub_idx); free(sub_dm);
-    free(cleaned);
+    free(sub_fac); free(cleaned);
     return r->error;
   }
   free(cleaned);
@@ -925,7 +1035,8 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
     struct dm_state* st = dm_state_create(series_stride, nbeams);
     size_t gcap = 0;
     for (size_t k = 0; k < n_sub && !too_many; ++k) {
-      too_many = process_dm(&ctx, st, sub_idx[k], plane + k * series_stride, r, &gcap);
+      too_many = process_dm(&ctx, st, sub_idx[k], plane + k * series_stride, r, &gcap,
+                            sub_fac[k]);
     }
     dm_state_destroy(st);
   } else if (!skip) {
@@ -941,7 +1052,8 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
 #endif
       for (long k = 0; k < (long)n_sub; ++k) {
         size_t gcap = 0;
-        process_dm(&ctx, st, sub_idx[k], plane + (size_t)k * series_stride, &part[k], &gcap);
+        process_dm(&ctx, st, sub_idx[k], plane + (size_t)k * series_stride, &part[k], &gcap,
+                   sub_fac[k]);
       }
       dm_state_destroy(st);
     }
@@ -957,6 +1069,7 @@ int orc_execute(const orc_params* p, const uint8_t* fb, size_t nsamps,
   }
   free(sub_idx);
   free(sub_dm);
+  free(sub_fac);
 
   if (!too_many && r->n_giants > 0) {
     uint64_t* labels = (uint64_t*)malloc(sizeof(uint64_t) * r->n_giants);

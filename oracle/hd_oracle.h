@@ -127,6 +127,27 @@ int orc_clean_filterbank_rfi(const uint8_t* in, size_t nsamps, size_t nbits, siz
                              float rfi_tol, size_t boxcar_max, size_t nbeams, uint8_t* out,
                              int* mask_out);
 
+/* ---- adaptive time resolution (dedisp's, as pipeline.cu:187-196 enables it) ----
+ * Parity unpinned: dedisp is absent from the reference tree.  The rule and the
+ * halving follow dedisp's published method; oracle/adaptive_dt.py states the
+ * same rules in numpy, and the tests pin the two against each other and
+ * against the invariants. */
+/* Scrunch factor of each trial: 1 for the first; a trial doubles its
+ * predecessor's factor when doubling the sampling time raises the effective
+ * pulse width (float32) by less than a factor tol. */
+void orc_scrunch_factors(const float* dms, size_t n, float dt, float pulse_width_us, float f0,
+                         float df, size_t nchans, float tol, size_t* out);
+/* Dedispersion at the trials' own resolutions: a trial of factor s sums the
+ * unpacked input halved log2(s) times (pairwise means rounded down, four
+ * zero samples appended) with delays orc_delay(dm, k_c / s), into the first
+ * (nsamps - max_delay) / s floats of its row; the rest of the row is zero.
+ * factors must not decrease along the list.  Returns 0, -1 (nbits) or -2
+ * (max_delay >= nsamps). */
+int orc_dedisperse_adaptive(const uint8_t* in, size_t nsamps, size_t nbits, size_t nchans,
+                            const int* killmask, const float* dm_list, size_t n_dm,
+                            const float* delay_table, size_t max_delay, const size_t* factors,
+                            float* out, size_t out_stride, int nthreads);
+
 /* ---- whole-gulp restatement of hd_execute (pipeline.cu:233-1087) ---- */
 typedef struct {
   size_t nchans;
@@ -145,6 +166,8 @@ typedef struct {
   size_t dm_stride;            /* process every dm_stride-th trial of the subset (0 = 1) */
   int    cleaning;             /* run clean_filterbank_rfi first (the CLEANING build) */
   float  rfi_tol;              /* sigma cut of the cleaning (default 5) */
+  int    use_scrunching;       /* adaptive time resolution (pipeline.cu:187-196; default 0 here) */
+  float  scrunch_tol;          /* its smearing tolerance (default_params.C: 1.15) */
 } orc_params;
 
 typedef struct {

@@ -38,7 +38,8 @@ class Params(C.Structure):
                 ("min_tscrunch_width", C.c_size_t), ("beam", C.c_size_t),
                 ("killmask", C.POINTER(C.c_int)), ("nthreads", C.c_int),
                 ("dm_first", C.c_size_t), ("dm_count", C.c_size_t), ("dm_stride", C.c_size_t),
-                ("cleaning", C.c_int), ("rfi_tol", C.c_float)]
+                ("cleaning", C.c_int), ("rfi_tol", C.c_float),
+                ("use_scrunching", C.c_int), ("scrunch_tol", C.c_float)]
 
 
 class Result(C.Structure):
@@ -118,6 +119,12 @@ def lib():
         L.orc_clean_filterbank_rfi.argtypes = [C.c_void_p, sz, sz, sz, C.c_float, C.c_float,
                                                C.c_float, C.c_float, C.c_float, C.c_float, sz,
                                                sz, C.c_void_p, C.c_void_p]
+        L.orc_scrunch_factors.restype = None
+        L.orc_scrunch_factors.argtypes = [fp, sz, C.c_float, C.c_float, C.c_float, C.c_float,
+                                          sz, C.c_float, C.POINTER(C.c_size_t)]
+        L.orc_dedisperse_adaptive.restype = C.c_int
+        L.orc_dedisperse_adaptive.argtypes = [C.c_void_p, sz, sz, sz, C.c_void_p, fp, sz, fp, sz,
+                                              C.POINTER(C.c_size_t), fp, sz, C.c_int]
         L.orc_default_params.restype = None
         L.orc_default_params.argtypes = [C.POINTER(Params)]
         L.orc_execute.restype = C.c_int
@@ -180,6 +187,35 @@ def dedisperse(fb, nsamps, nbits, nchans, dms, kc, max_delay, killmask=None, nth
                               _fp(kc), max_delay, _fp(out), nout, nthreads)
     if rc != 0:
         raise ValueError("orc_dedisperse failed %d" % rc)
+    return out
+
+
+def scrunch_factors(dms, dt, pulse_width_us, f0, df, nchans, tol):
+    dms = np.ascontiguousarray(dms, np.float32)
+    out = np.zeros(len(dms), np.uintp)
+    lib().orc_scrunch_factors(_fp(dms), len(dms), dt, pulse_width_us, f0, df, nchans, tol,
+                              out.ctypes.data_as(C.POINTER(C.c_size_t)))
+    return out.astype(np.int64)
+
+
+def dedisperse_adaptive(fb, nsamps, nbits, nchans, dms, kc, max_delay, factors, killmask=None,
+                        nthreads=1):
+    fb = np.ascontiguousarray(fb, dtype=np.uint8)
+    dms = np.ascontiguousarray(dms, np.float32)
+    kc = np.ascontiguousarray(kc, np.float32)
+    fac = np.ascontiguousarray(factors, np.uintp)
+    nout = nsamps - max_delay
+    out = np.zeros((len(dms), nout), np.float32)
+    km = None
+    if killmask is not None:
+        killmask = np.ascontiguousarray(killmask, np.int32)
+        km = killmask.ctypes.data
+    rc = lib().orc_dedisperse_adaptive(fb.ctypes.data, nsamps, nbits, nchans, km, _fp(dms),
+                                       len(dms), _fp(kc), max_delay,
+                                       fac.ctypes.data_as(C.POINTER(C.c_size_t)), _fp(out), nout,
+                                       nthreads)
+    if rc != 0:
+        raise ValueError("orc_dedisperse_adaptive failed %d" % rc)
     return out
 
 

@@ -116,3 +116,93 @@ def test_adaptive_special_cases():
     nout = nsamps - maxd
     for i, s in enumerate(f):
         assert np.all(got[i, :nout // s] == 77 * nchans), i
+
+
+# ---- the C restatement (oracle/hd_oracle.c) used by the whole-gulp oracle ----
+
+@pytest.mark.parametrize("name", sorted(synth.CONFIGS))
+def test_c_scrunch_factors_equal_numpy(name):
+    w = synth.CONFIGS[name]
+    dms = synth.trial_dms(w)
+    for tol in (1.05, 1.15, 1.5):
+        a = A.scrunch_list(dms, w.dt, 40.0, w.f0, w.df, w.nchans, tol)
+        b = O.scrunch_factors(dms, w.dt, 40.0, w.f0, w.df, w.nchans, tol)
+        assert np.array_equal(a, b), (name, tol)
+
+
+@pytest.mark.parametrize("nbits", [8, 2])
+def test_c_adaptive_dedispersion_equals_numpy(nbits):
+    rng = np.random.default_rng(5 + nbits)
+    nchans, nsamps, dt, f0, df = 48, 3001, 5e-4, 600.0, -3.0
+    kc = O.delay_table(nchans, f32(dt), f32(f0), f32(df))
+    dms = np.linspace(0, 60, 14).astype(f32)
+    factors = np.array([1, 1, 1, 2, 2, 2, 4, 4, 4, 8, 8, 16, 16, 32])
+    maxd = int(_delays(dms, kc).max())
+    fb = rng.integers(0, 256, nsamps * nchans * nbits // 8, dtype=np.uint8)
+    km = np.ones(nchans, np.int32)
+    km[[0, 7, 30]] = 0
+    want = A.dedisperse_adaptive(fb, nsamps, nbits, nchans, dms, kc, maxd, factors, killmask=km)
+    got = O.dedisperse_adaptive(fb, nsamps, nbits, nchans, dms, kc, maxd, factors, killmask=km,
+                                nthreads=4)
+    assert np.array_equal(got, want)
+
+
+def _same_giants(a, b):
+    assert len(a) == len(b)
+    for k in a.dtype.names:  # field by field (the records have padding bytes)
+        assert np.array_equal(a[k], b[k]), k
+
+
+def _exec(w, seed, **kw):
+    from tests._cases import oracle_params_for
+    fb = synth.make_gulp(w, seed)
+    p = oracle_params_for(O, w, **kw)
+    return O.execute(p, fb, w.nsamps, w.nbits, 0, 1)
+
+
+def test_execute_all_factors_one_is_the_plain_search():
+    """tiny_64ch's list never doubles: scrunching on changes nothing."""
+    from tests._cases import TINY
+    assert np.all(O.scrunch_factors(synth.trial_dms(TINY), TINY.dt, 40.0, TINY.f0, TINY.df,
+                                    TINY.nchans, 1.15) == 1)
+    a = _exec(TINY, 1, use_scrunching=0)
+    b = _exec(TINY, 1, use_scrunching=1)
+    assert len(a.giants) > 0
+    _same_giants(a.giants, b.giants)
+    assert a.text == b.text
+
+
+@pytest.mark.parametrize("case", ["SCRUNCH_128", "SCRUNCH_256"])
+def test_execute_adaptive_trials(case):
+    """With scrunching, the trials whose factor stays 1 give exactly the
+    plain search's giants (the loop at pipeline.cu:488-790 treats them
+    alike); a trial of factor s reports samples of the full resolution
+    (multiples of s) and widths from s up (filter >= log2 s); the strongest
+    injected pulses are found either way."""
+    import tests._cases as C
+    w = getattr(C, case)
+    dms = synth.trial_dms(w)
+    f = O.scrunch_factors(dms, w.dt, 40.0, w.f0, w.df, w.nchans, 1.15)
+    assert f.max() >= 8
+    seed = {"SCRUNCH_128": 3, "SCRUNCH_256": 5}[case]
+    off = _exec(w, seed, use_scrunching=0)
+    on = _exec(w, seed, use_scrunching=1)
+    one_off = off.giants[f[off.giants["dm"]] == 1]
+    one_on = on.giants[f[on.giants["dm"]] == 1]
+    assert len(one_on) > 0
+    _same_giants(one_on, one_off)
+    sc = on.giants[f[on.giants["dm"]] > 1]
+    assert len(sc) > 0
+    s = f[sc["dm"]].astype(np.uint64)
+    for k in ("ind", "begin", "end"):
+        assert np.all(sc[k] % s == 0), k
+    assert np.all((1 << sc["filter"].astype(np.uint64)) >= s)
+    # the brightest candidates of the plain search at least as wide as their
+    # trial's factor are detected as giants (clustering may fold them into
+    # another group's peak)
+    strong = off.groups[(off.groups["peak"] > 12) &
+                        ((1 << off.groups["filter"].astype(np.int64)) >= f[off.groups["dm"]])]
+    assert len(strong) > 0
+    for g in strong:
+        near = on.giants[(np.abs(on.giants["ind"].astype(np.int64) - int(g["ind"])) < 64)]
+        assert len(near) > 0 and near["peak"].max() > 0.8 * g["peak"], g
