@@ -17,6 +17,8 @@ Prints one JSON line:
   kernels_hbm    per-kernel HBM GB/s from the committed rocprofv3 counters
   host_path      hd_execute from page-locked host memory (the rate including
                  the PCIe copy; never the headline)
+  adaptive_dt    the same gulp with use_scrunching on (the reference's default
+                 search: trials at their scrunch factor)
   batch          the ragged 352-beam batch (320 channels, 655.36 us) in one
                  hd_execute_batch per step, with its own roofline
   cpu_baseline   the CPU oracle on a bounded sample of the same gulp;
@@ -84,6 +86,8 @@ def parse():
                     help="skip the proposal-gated vs full search comparison")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the ragged-batch block of the default line")
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the adaptive time resolution block")
     ap.add_argument("--no-host-path", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) timing")
     ap.add_argument("--all-ops", action="store_true",
@@ -521,6 +525,43 @@ def host_path(pl, w, fb, steps, beam_s):
                     "every call (reported beside value, which has the input in HBM)"}
 
 
+def adaptive_path(w, dbuf, steps, warmup, beam_s):
+    """The same HBM-resident gulp searched with adaptive time resolution
+    (use_scrunching, the reference's default): each trial at its scrunch
+    factor (1, 2, 4, ...)."""
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import hip
+    from heimdall_multibeam_amd.dedisp_plan import DedispPlan
+    params = H.default_params(nchans=w.nchans, dt=w.dt, f0=w.f0, df=w.df, dm_min=w.dm_min,
+                              dm_max=w.dm_max, boxcar_max=w.boxcar_max,
+                              detect_thresh=w.detect_thresh, use_scrunching=True,
+                              min_tscrunch_width=w.min_tscrunch_width)
+    pl = H.Pipeline(params)
+    pl.set_output(False)
+    for _ in range(warmup):
+        pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
+        n_cand = len(pl.candidates()["peak"])
+    hip.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    n_giants = len(pl.giants()["peak"])
+    pl.close()
+    plan = DedispPlan(w.nchans, w.dt, w.f0, w.df)
+    plan.generate_dm_list(w.dm_min, w.dm_max, 40.0, 1.25)
+    plan.enable_adaptive_dt(40.0, params.scrunch_tol)
+    f, c = np.unique(plan.dt_factors(), return_counts=True)
+    plan.close()
+    return {"value": round(beam_s / t, 3), "unit": "beam-seconds searched/s",
+            "ms_per_step": round(t * 1e3, 3), "candidates": n_cand, "giants": n_giants,
+            "trials_per_factor": {str(int(k)): int(v) for k, v in zip(f, c)},
+            "note": "use_scrunching on (scrunch_tol %.2f): the reference's default search, "
+                    "trials at their scrunch factor; value keeps the full-resolution search"
+                    % params.scrunch_tol}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -616,12 +657,18 @@ def main():
                                         pl.dedisp_tile(), t_step, rows),
         "kernels_hbm": table,
         "host_path": None,
+        "adaptive_dt": None,
         "cpu_baseline": None,
         "cpu_baseline_small": None,
     }
     if not a.no_host_path:
         out["host_path"] = host_path(pl, w, fb, a.steps, beam_s)
     pl.close()
+    if not a.no_adaptive:
+        try:
+            out["adaptive_dt"] = adaptive_path(w, dbuf, a.steps, a.warmup, beam_s)
+        except Exception as e:  # reported, never hides the main measurement
+            out["adaptive_dt"] = {"error": "%s: %s" % (type(e).__name__, e)}
     dbuf.free()
     if not a.no_cpu_baseline:
         print("timing CPU baselines", file=sys.stderr, flush=True)

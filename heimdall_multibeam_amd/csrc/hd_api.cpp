@@ -1157,13 +1157,17 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
                              (size_t)n_live == p.nchans,
                              G > 1 && !(pl->gating || pl->coarse_only)));
   // level l, slots l*G .. l*G + G - 1: level l - 1 halved (pairwise means
-  // rounded down, zeros past the halved samples)
+  // rounded down), then zeros as far as the dedispersion windows reach: a
+  // tile starting at t0 <= nout / f reads bytes below t0 + (max delay of its
+  // trials) + tile + 52 <= nsamps / f + tile + 53
   for (int l = 1; l < n_levels && n_live > 0; ++l)
     for (int g = 0; g < G; ++g) {
       const size_t slot = (size_t)n_live * xpitch;
+      const size_t ns = pl->hgd[g].nsamps >> (l - 1);
+      const size_t limit = round_up(ns / 2 + 2 * (size_t)hd::dedisp_tile_time(), 256);
       HD_CHECK(hd::launch_scrunch2(pl->x.as<uint8_t>() + ((size_t)(l - 1) * G + g) * slot, n_live,
-                                   xpitch, pl->hgd[g].nsamps >> (l - 1),
-                                   pl->x.as<uint8_t>() + ((size_t)l * G + g) * slot, s));
+                                   xpitch, ns, pl->x.as<uint8_t>() + ((size_t)l * G + g) * slot, s,
+                                   limit));
     }
   record(pl, 10);
   // The search runs over row groups: a whole gulp each, or with gating one
@@ -1281,6 +1285,10 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     dg.wlen8 = tile->wlen8;
     dg.split8 = tile->split8;
     dg.wide = tile->wide;
+    // the groups' own series lengths set the grid's time tiles
+    size_t nmax = 0;
+    for (const hd::GroupDev& d : hrg) nmax = std::max(nmax, d.sh.nout);
+    dg.tiles = nmax / hd::dedisp_tile_time() + 1;
   } else {
     hd_error te = choose_dedisp_tile(pl, dg, gr, nbits, s, &pairs);  // may time candidates
     if (te != HD_NO_ERROR) return te;

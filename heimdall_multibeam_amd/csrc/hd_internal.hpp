@@ -139,10 +139,11 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
                          hipStream_t s, const GroupArgs& ga = GroupArgs{},
                          bool live_is_identity = false, bool trim_rows = false);
 // Halves the time resolution of unpacked rows: y[l][t] = (x[l][2t] +
-// x[l][2t+1]) >> 1 for t < nsamps / 2, zero out to the pitch (both pitches
-// xpitch, a multiple of 256 and at least nsamps + 8).
+// x[l][2t+1]) >> 1 for t < nsamps / 2, zero from there to byte `limit` of the
+// row (0: the pitch; a multiple of 4, at least nsamps / 2); both pitches
+// xpitch, a multiple of 256 and at least nsamps + 8.
 hipError_t launch_scrunch2(const uint8_t* x, int n_live, size_t xpitch, size_t nsamps,
-                           uint8_t* y, hipStream_t s);
+                           uint8_t* y, hipStream_t s, size_t limit = 0);
 
 struct DedispGeom {
   int n_dm, n_live;
@@ -157,6 +158,8 @@ struct DedispGeom {
   int split8 = 0;      // byte-staged kernel for 8-bit samples (even/odd byte sums)
   int cc = 4;          // channels staged per step (4, 8 or 16; the staging pairs are planned for it)
   int wide = 0;        // byte staging at 32 trials: 1024 threads (2 trials per wave) instead of 512
+  size_t tiles = 0;    // time tiles of the grid (0: ppitch / tile time); must reach every
+                       // group's tile holding its sample nout
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,

@@ -593,12 +593,13 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
 // Adaptive time resolution: one thread per 4 output samples of a row, from
 // one 8-byte load (xpitch is a multiple of 256, so every load is aligned and
 // inside the row).  The pairwise mean rounds down, so samples keep their bit
-// width.
+// width.  Bytes limit .. xpitch of the output rows are left as they are.
 __global__ __launch_bounds__(256) void k_scrunch2(const uint8_t* __restrict__ x, size_t xpitch,
-                                                  size_t nsamps, uint8_t* __restrict__ y) {
+                                                  size_t nsamps, uint8_t* __restrict__ y,
+                                                  size_t limit) {
   const size_t t4 = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
   const size_t row = blockIdx.y;
-  if (t4 >= xpitch) return;
+  if (t4 >= limit) return;
   const size_t nh = nsamps / 2;
   uint32_t out = 0;
   if (t4 < nh) {
@@ -615,11 +616,13 @@ __global__ __launch_bounds__(256) void k_scrunch2(const uint8_t* __restrict__ x,
 }  // namespace
 
 hipError_t launch_scrunch2(const uint8_t* x, int n_live, size_t xpitch, size_t nsamps,
-                           uint8_t* y, hipStream_t s) {
+                           uint8_t* y, hipStream_t s, size_t limit) {
   // (a thread's 8-byte load ends at most 5 bytes past the row's last sample)
   if (xpitch % 256 || nsamps + 8 > xpitch || n_live <= 0) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((xpitch / 4 + 255) / 256), (unsigned)n_live);
-  hipLaunchKernelGGL(k_scrunch2, grid, dim3(256), 0, s, x, xpitch, nsamps, y);
+  if (limit == 0 || limit > xpitch) limit = xpitch;
+  if (limit % 4 || limit < nsamps / 2) return hipErrorInvalidValue;
+  dim3 grid((unsigned)((limit / 4 + 255) / 256), (unsigned)n_live);
+  hipLaunchKernelGGL(k_scrunch2, grid, dim3(256), 0, s, x, xpitch, nsamps, y, limit);
   return hipGetLastError();
 }
 
@@ -726,7 +729,11 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
   const int db = g.db, cc = g.cc;
   if (db != 8 && db != 16 && db != 32) return hipErrorInvalidValue;
   if (cc != 4 && cc != 8 && cc != 16) return hipErrorInvalidValue;
-  dim3 grid((unsigned)((g.n_dm + db - 1) / db), (unsigned)(g.ppitch / D2_TB),
+  // time tiles: the row pitch's, or fewer when every group's series ends
+  // sooner (the later tiles would only return)
+  const size_t tiles =
+      (g.tiles && g.tiles < g.ppitch / D2_TB) ? g.tiles : g.ppitch / D2_TB;
+  dim3 grid((unsigned)((g.n_dm + db - 1) / db), (unsigned)tiles,
             (unsigned)ga.G);
   const int slot = tile_slot(db, cc);
   // the delay span of a DM block too wide for this tile's staged copies
