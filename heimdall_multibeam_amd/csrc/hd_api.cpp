@@ -1993,7 +1993,7 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
                       &pl->c_sfd, &pl->c_ecount, &pl->c_eoff, &pl->c_adj, &pl->rdev, &pl->co_delays, &pl->co_pairs, &pl->co_xc,
                       &pl->co_plane, &pl->co_pts, &pl->co_rms, &pl->co_brms, &pl->co_bf, &pl->co_bw,
                       &pl->co_gdev, &pl->co_tiles, &pl->co_scores, &pl->co_toff, &pl->co_ntpos,
-                      &pl->co_ncols, &pl->co_flags, &pl->co_pos};
+                      &pl->co_ncols, &pl->co_flags, &pl->co_pos, &pl->adelays};
     for (DevBuf* b : bufs) b->release();
     for (DevBuf& b : pl->pairs_db) b.release();
     if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
