@@ -340,8 +340,10 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     // record empty counts (and padding rows of a gated group)
     bool any = false;
     if (blockIdx.y < gd[blockIdx.z].rows_valid)
-      for (int j = 0; j < wt.nw; ++j)
-        if ((p0 + wt.wi[j].ts - 1) / wt.wi[j].ts < nwv[j]) any = true;
+      for (int j = 0; j < wt.nw; ++j) {
+        const unsigned ts = wt.wi[j].ts;
+        if ((ts == 1 ? p0 : (p0 + ts - 1) / ts) < nwv[j]) any = true;
+      }
     if (!any) {
       for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
         gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
@@ -362,7 +364,7 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const long long tavail = slen - tbase;
   const unsigned tneed = tavail <= 0 ? 1u : (tavail < (long long)tlen ? (unsigned)tavail : tlen);
   const unsigned n4 = (tneed + sh + 3) / 4;
-  constexpr int SU = 4;
+  constexpr int SU = 12;  // (the widest tile, 8192 + 4096 floats, in one round)
   for (unsigned v0 = threadIdx.x; v0 < n4; v0 += SU * FB_BS) {
     float4 r[SU];
 #pragma unroll
@@ -391,6 +393,34 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const int lane = (int)(tid & 63), wave = (int)(tid >> 6);
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;  // lanes under this one
   const unsigned tmax = tlen - 1;
+  // Per-width index arithmetic on the scalar unit: 64-bit divisions are long
+  // software sequences there, so the undecimated widths (output i at
+  // position i) take the chunk's own range and beam period, found once:
+  // positions p0 .. p0 + TH_CHUNK - 1 in period pbeam (find_giants.cu:55-58
+  // keeps i with (i mod stride) < bc692), which ends at pnext.
+  const size_t pbeam = (size_t)(unsigned)p0 / stride;
+  const size_t pnext = (pbeam + 1) * stride;
+  const size_t plim = pbeam * stride + bc692;
+  // outputs i with i*ts in the chunk, i0 .. i1 - 1 (i1 not yet clipped to
+  // the width's count), whether they lie in one beam period, and its bound
+  auto chunk_outputs = [&](const WidthInfo& wi, size_t& i0, size_t& i1) {
+    if (wi.ts == 1) {
+      i0 = p0;
+      i1 = p0 + TH_CHUNK;
+    } else {
+      i0 = (p0 + wi.ts - 1) / wi.ts;
+      i1 = (p0 + TH_CHUNK + wi.ts - 1) / wi.ts;
+    }
+  };
+  auto beam_period = [&](const WidthInfo& wi, size_t i0, size_t i1, size_t& lim) -> bool {
+    if (wi.ts == 1) {  // i0 == p0
+      lim = plim;
+      return i1 - 1 < pnext && i1 - 1 <= 0xffffffffull;
+    }
+    const size_t beam0 = (size_t)(unsigned)i0 / stride;
+    lim = beam0 * stride + bc692;
+    return (size_t)(unsigned)(i1 - 1) / stride == beam0 && i1 - 1 <= 0xffffffffull;
+  };
   // Outputs p = q*FB_BS + tid (q < TH_PER) of width j in this chunk: their
   // values, and the bits of those kept and above the threshold.  Returns
   // false when the chunk holds no output of the width (outputs i0 .. i1-1).
@@ -398,8 +428,8 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     const WidthInfo wi = wt.wi[j];
     const float inv = winv_all[row * wt.nw + j];
     // outputs i with i*ts in [p0, p0 + TH_CHUNK)
-    i0 = (p0 + wi.ts - 1) / wi.ts;
-    size_t i1 = (p0 + TH_CHUNK + wi.ts - 1) / wi.ts;
+    size_t i1;
+    chunk_outputs(wi, i0, i1);
     const size_t n_w = nwv[j];
     if (i1 > n_w) i1 = n_w;
     mask = 0;
@@ -407,10 +437,8 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     const unsigned nout = (unsigned)(i1 - i0);
     // find_giants.cu:55-58 keeps i with (i mod stride) < bc692.  When the
     // chunk's outputs all lie in one beam period that is a single bound.
-    const size_t beam0 = (size_t)(unsigned)i0 / stride;
-    const bool one_period = (size_t)(unsigned)(i1 - 1) / stride == beam0 &&
-                            i1 - 1 <= 0xffffffffull;
-    const size_t lim = beam0 * stride + bc692;
+    size_t lim;
+    const bool one_period = beam_period(wi, i0, i1, lim);
     const unsigned rbase = (unsigned)(i0 * wi.ts - p0 + bmax);   // T index of output i0
     if (one_period) {
       // kept: outputs p < pk (i < min(i1, lim)); the others are evaluated
@@ -463,13 +491,12 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   // either form); other widths evaluate every output.
   auto screen_width = [&](int j) -> bool {
     const WidthInfo wi = wt.wi[j];
-    const size_t i0 = (p0 + wi.ts - 1) / wi.ts;
-    size_t i1 = (p0 + TH_CHUNK + wi.ts - 1) / wi.ts;
+    size_t i0, i1;
+    chunk_outputs(wi, i0, i1);
     if (i1 > nwv[j]) i1 = nwv[j];
     if (i0 >= i1) return false;
-    const size_t beam0 = (size_t)(unsigned)i0 / stride;
-    const bool one_period = (size_t)(unsigned)(i1 - 1) / stride == beam0 &&
-                            i1 - 1 <= 0xffffffffull;
+    size_t lim;
+    const bool one_period = beam_period(wi, i0, i1, lim);
     if (wi.ts != 1 || !one_period) {
       float vals[TH_PER];
       unsigned mask;
@@ -478,7 +505,6 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
       return mask != 0;
     }
     const float inv = winv_all[row * wt.nw + j];
-    const size_t lim = beam0 * stride + bc692;
     const size_t kend = i1 < lim ? i1 : lim;
     const unsigned pk = kend > i0 ? (unsigned)(kend - i0) : 0u;
     const unsigned rbase = (unsigned)(i0 - p0 + bmax);
@@ -489,8 +515,11 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
 #pragma unroll
       for (int q = 0; q < TH_PER; ++q) mx = fmaxf(mx, Ta[q * FB_BS] - Tb[q * FB_BS]);
     } else {
+      // a row ending in the chunk (short gulps of a batch): the steps past
+      // its last kept output are skipped, block-uniformly
 #pragma unroll
       for (int q = 0; q < TH_PER; ++q) {
+        if ((unsigned)q * FB_BS >= pk) break;
         const float d = Ta[q * FB_BS] - Tb[q * FB_BS];
         if ((unsigned)q * FB_BS + tid < pk) mx = fmaxf(mx, d);
       }
