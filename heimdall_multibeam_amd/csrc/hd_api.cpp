@@ -1243,7 +1243,7 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   int n_fresh = 0;
   while (n_fresh < wt.nw && wt.wi[n_fresh].ts == 1) ++n_fresh;
   hd::GiantBuffers gb{};
-  gb.chunk = hd::threshold_chunk();
+  gb.chunk = hd::threshold_chunk(wt);
   size_t n_w0 = 0;
   for (const hd::GroupDev& d : hrg) n_w0 = std::max<size_t>(n_w0, d.n_w[0]);
   gb.nchunks_per_w = (n_w0 + gb.chunk - 1) / gb.chunk;

@@ -224,7 +224,8 @@ struct GiantBuffers {
   size_t nchunks_per_w;
   size_t chunk;        // outputs per chunk
 };
-size_t threshold_chunk();  // prefix-sum positions per launch_threshold block (one key each)
+// prefix-sum positions per launch_threshold block (one key each) for width table wt
+size_t threshold_chunk(const WidthTable& wt);
 hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
                             const float* winv, int n_dm, float thresh, GiantBuffers& gb,
                             hipStream_t s, const GroupArgs& ga);

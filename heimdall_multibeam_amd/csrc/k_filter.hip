@@ -16,10 +16,11 @@
 //              the latest earlier width that reached that position
 //              (pipeline.cu:680-712 reuse d_filtered_series).
 //   threshold  GiantFinder's selection (find_giants.cu:47-60,137-183): per
-//              (DM, width, 8192-position chunk) the samples above threshold are
-//              compacted in time order and appended to an unordered buffer;
-//              the per-chunk counts are scanned so that a second pass lays the
-//              samples out in the reference's order (DM, width, time).
+//              (DM, width, chunk of 8192 or 4096 positions) the samples above
+//              threshold are compacted in time order and appended to an
+//              unordered buffer; the per-chunk counts are scanned so that a
+//              second pass lays the samples out in the reference's order
+//              (DM, width, time).
 //   segments   head flags where a new giant starts (adjacent gap > merge_dist,
 //              find_giants.cu:79-87,197-210), their scan, and one thread per
 //              giant reducing its run to the first maximum, begin and end
@@ -302,8 +303,12 @@ __device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* t
   return pre + x - v;
 }
 
-constexpr int TH_PER = 32;
-constexpr int TH_CHUNK = FB_BS * TH_PER;  // 8192 prefix-sum positions per block
+// Positions per block: FB_BS * TH_PER, 8192 by default; 4096 when the widest
+// boxcar's halo is at most 2048 (short rows and boxcars: twice the blocks
+// resident per CU hide the staging better, for a halo that stays small).
+constexpr int TH_PER_LONG = 32;
+constexpr int TH_PER_SHORT = 16;
+constexpr unsigned TH_SHORT_HALO = 2048;
 
 // One block per (chunk of TH_CHUNK positions i*ts, DM): the chunk's prefix
 // sums (plus the widest boxcar's halo) are staged in LDS once and every width
@@ -318,10 +323,12 @@ constexpr int TH_CHUNK = FB_BS * TH_PER;  // 8192 prefix-sum positions per block
 // the q*FB_BS offsets are immediates).  Hits are written in time order:
 // q-major, then wave, then lane (one ballot per q and wave, wave counts
 // prefixed in LDS).
+template <int TH_PER>
 __global__ __launch_bounds__(FB_BS) void k_threshold(
     const float* __restrict__ Sall, size_t spitch, const GroupDev* __restrict__ gd,
     WidthTable wt, unsigned amax, unsigned bmax, const float* __restrict__ winv_all,
     float thresh, GiantBuffers gb) {
+  constexpr int TH_CHUNK = FB_BS * TH_PER;
   extern __shared__ float4 T4[];  // TH_CHUNK + amax + bmax prefix sums (+ alignment)
   __shared__ unsigned wcnt[TH_PER][FB_BS / 64];
   __shared__ unsigned sbase;
@@ -562,21 +569,31 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     }
     __syncthreads();
     if (wave == 0) {
-      // exclusive prefix of the 128 counts in (q, wave) order: lane l takes
-      // entries 2l and 2l+1, the lanes' pair sums are scanned across the
-      // wave, and lane 0 books the total
-      static_assert(TH_PER * (FB_BS / 64) == 128, "two counts per lane");
+      // exclusive prefix of the TH_PER * 4 counts in (q, wave) order: lane l
+      // takes entries EP*l .. EP*l + EP - 1, the lanes' sums are scanned
+      // across the wave, and lane 0 books the total
+      constexpr int EP = TH_PER * (FB_BS / 64) / 64;
+      static_assert(EP >= 1 && EP * 64 == TH_PER * (FB_BS / 64), "whole counts per lane");
       unsigned* e = &wcnt[0][0];
-      const unsigned a = e[2 * lane], b = e[2 * lane + 1];
-      unsigned incl = a + b;
+      unsigned cv[EP];
+      unsigned own = 0;
+#pragma unroll
+      for (int k = 0; k < EP; ++k) {
+        cv[k] = e[EP * lane + k];
+        own += cv[k];
+      }
+      unsigned incl = own;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const unsigned u = __shfl_up(incl, o);
         if (lane >= o) incl += u;
       }
-      const unsigned excl = incl - (a + b);
-      e[2 * lane] = excl;
-      e[2 * lane + 1] = excl + a;
+      unsigned run = incl - own;
+#pragma unroll
+      for (int k = 0; k < EP; ++k) {
+        e[EP * lane + k] = run;
+        run += cv[k];
+      }
       const unsigned total = __shfl(incl, 63);
       if (lane == 0) {
         sbase = atomicAdd(gb.counter, total);
@@ -806,22 +823,35 @@ hipError_t launch_block_rms_finish(const float* partial, int nw_total, int first
   return hipGetLastError();
 }
 
-hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
-                            const float* winv, int n_dm, float thresh, GiantBuffers& gb,
-                            hipStream_t s, const GroupArgs& ga) {
-  if (!ga.gd) return hipErrorInvalidValue;
-  gb.chunk = TH_CHUNK;
+size_t threshold_chunk(const WidthTable& wt) {
   unsigned amax = 0, bmax = 0;
   for (int j = 0; j < wt.nw; ++j) {
     amax = wt.wi[j].ahead > amax ? wt.wi[j].ahead : amax;
     bmax = wt.wi[j].behind > bmax ? wt.wi[j].behind : bmax;
   }
-  const size_t tlen = (size_t)TH_CHUNK + amax + bmax;
+  return (size_t)FB_BS * (amax + bmax <= TH_SHORT_HALO ? TH_PER_SHORT : TH_PER_LONG);
+}
+
+hipError_t launch_threshold(const float* S, size_t spitch, const WidthTable& wt,
+                            const float* winv, int n_dm, float thresh, GiantBuffers& gb,
+                            hipStream_t s, const GroupArgs& ga) {
+  if (!ga.gd) return hipErrorInvalidValue;
+  gb.chunk = threshold_chunk(wt);
+  unsigned amax = 0, bmax = 0;
+  for (int j = 0; j < wt.nw; ++j) {
+    amax = wt.wi[j].ahead > amax ? wt.wi[j].ahead : amax;
+    bmax = wt.wi[j].behind > bmax ? wt.wi[j].behind : bmax;
+  }
+  const size_t tlen = gb.chunk + amax + bmax;
   const size_t lds = (tlen + 8) * sizeof(float);  // + the 16-byte alignment of the staging
   if (lds > 65536) return hipErrorInvalidValue;   // boxcar_max beyond ~12000
   dim3 grid((unsigned)gb.nchunks_per_w, n_dm, ga.G);
-  hipLaunchKernelGGL(k_threshold, grid, dim3(FB_BS), lds, s, S, spitch, ga.gd, wt, amax, bmax,
-                     winv, thresh, gb);
+  if (gb.chunk == (size_t)FB_BS * TH_PER_SHORT)
+    hipLaunchKernelGGL(k_threshold<TH_PER_SHORT>, grid, dim3(FB_BS), lds, s, S, spitch, ga.gd, wt,
+                       amax, bmax, winv, thresh, gb);
+  else
+    hipLaunchKernelGGL(k_threshold<TH_PER_LONG>, grid, dim3(FB_BS), lds, s, S, spitch, ga.gd, wt,
+                       amax, bmax, winv, thresh, gb);
   return hipGetLastError();
 }
 
@@ -850,7 +880,6 @@ hipError_t launch_exclusive_scan_u32(const unsigned* in, unsigned* out, size_t n
   return hipGetLastError();
 }
 
-size_t threshold_chunk() { return TH_CHUNK; }
 
 hipError_t launch_order_samples(GiantBuffers& gb, size_t nkeys, hipStream_t s) {
   hipLaunchKernelGGL(k_order_samples, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s, gb,
