@@ -114,6 +114,7 @@ struct hd_pipeline_t {
   bool adaptive = false;
   DevBuf adelays;
   std::vector<int> adelays_h;
+  std::vector<int> delays_h;  // host copy of the delay table (rows of live channels)
   // device buffers
   DevBuf fb, x, live, delays, plane, pts, rms, brms, S, partial, wrms, winv;
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
@@ -160,6 +161,9 @@ struct hd_pipeline_t {
   std::map<std::tuple<int, int, int, int>, std::array<int, 5>> tune;
   hipEvent_t tev[2] = {};
   int last_db = 16, last_remap = 1, last_lanes8 = 0, last_cc = 4, last_wide = 0;
+  // double-buffered dedispersion staging where the windows allow
+  // (HD_DEDISP_DBUF=0 turns it off)
+  bool use_dbuf = true;
   // the stage chain as a HIP graph (hd_set_graph)
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -481,6 +485,7 @@ hd_error build_live_and_delays(hd_pipeline pl) {
   HD_CHECK(pl->delays.ensure(d.size() * sizeof(int)));
   HD_CHECK(hipMemcpy(pl->live.p, pl->live_list.data(), nl * sizeof(int), hipMemcpyHostToDevice));
   HD_CHECK(hipMemcpy(pl->delays.p, d.data(), d.size() * sizeof(int), hipMemcpyHostToDevice));
+  pl->delays_h = d;
   if (pl->adaptive) {
     // each trial's delays in samples of its own resolution (the slope over a
     // power of two is exact in float)
@@ -756,6 +761,28 @@ void apply_tile(hd_pipeline pl, hd::DedispGeom& dg, int db, int remap, int lanes
   dg.wide = lanes8 && db == 32 ? wide : 0;
 }
 
+// The leading DM blocks of a byte-staged launch of 32 trials on 1024 threads
+// whose windows fit two staging buffers (rows r0 .. r0 + rows - 1 of the
+// host delay table d, nl live channels per row): they run double-buffered.
+void plan_dbuf(hd::DedispGeom& dg, const std::vector<int>& d, size_t r0, size_t rows,
+               size_t nl) {
+  dg.dbuf_blocks = 0;
+  dg.wlen8_dbuf = 0;
+  if (!(dg.widen8 > 0 && dg.wide && dg.db == 32) || nl == 0 || d.size() < (r0 + rows) * nl)
+    return;
+  int wmax = 0;
+  for (size_t m0 = 0; m0 < rows; m0 += 32) {
+    const size_t a = (r0 + m0) * nl, b = (r0 + std::min(m0 + 32, rows) - 1) * nl;
+    int spread = 0;
+    for (size_t l = 0; l < nl; ++l) spread = std::max(spread, d[b + l] - (d[a + l] & ~3));
+    const int w = std::max(wmax, hd::dedisp8_window_dwords(spread));
+    if (hd::dedisp_lds_bytes_dbuf(dg, w) > 160 * 1024) break;
+    wmax = w;
+    dg.dbuf_blocks = (int)(m0 / 32) + 1;
+  }
+  dg.wlen8_dbuf = wmax;
+}
+
 hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
                             size_t nbits, hipStream_t s, const int** pairs) {
   // trials per block, remap, byte staging, channels per step, 1024 threads
@@ -779,6 +806,13 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
     const auto key = std::make_tuple(dg.n_dm, gr.gd ? 1 : 0, dg.n_live, (int)nbits);
     auto it = pl->tune.find(key);
     if (it == pl->tune.end()) {
+      // candidates timed as they will run: with double-buffered staging for
+      // the leading DM blocks when every group starts at the same trial
+      bool dbuf_ok = pl->use_dbuf && gr.host;
+      for (int g = 0; dbuf_ok && g < gr.G; ++g)
+        dbuf_ok = gr.host[g].trial0 == gr.host[0].trial0;
+      const size_t r0 = dbuf_ok ? gr.host[0].trial0 : 0;
+      const size_t nl = (size_t)std::max(dg.n_live, 1);
       if (!pl->tev[0]) {
         HD_CHECK(hipEventCreate(&pl->tev[0]));
         HD_CHECK(hipEventCreate(&pl->tev[1]));
@@ -796,6 +830,7 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
               // to rank the tiles as the full launch does
               hd::DedispGeom t = dg;
               apply_tile(pl, t, 8 << k, r, lanes, cc, wide, nbits);
+              if (dbuf_ok) plan_dbuf(t, pl->delays_h, r0, (size_t)dg.n_dm, nl);
               t.ppitch = std::min(dg.ppitch, (size_t)48 * hd::dedisp_tile_time());
               hd::GroupArgs tg = gr;
               tg.G = std::min(gr.G, 64);
@@ -825,6 +860,7 @@ hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupA
         for (int r = 0; r < 2; ++r) {
           hd::DedispGeom t = dg;
           apply_tile(pl, t, choice[0], r, choice[2], choice[3], choice[4], nbits);
+          if (dbuf_ok) plan_dbuf(t, pl->delays_h, r0, (size_t)dg.n_dm, nl);
           hd::GroupArgs tg = gr;
           tg.G = std::min(gr.G, 64);
           const int* tp = pl->pairs_db[tile_index(choice[0], choice[3])].as<int>();
@@ -1292,6 +1328,15 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   } else {
     hd_error te = choose_dedisp_tile(pl, dg, gr, nbits, s, &pairs);  // may time candidates
     if (te != HD_NO_ERROR) return te;
+  }
+  // double-buffered staging for the leading DM blocks, when every group
+  // starts at the same trial (its blocks then have the same windows)
+  {
+    bool same = true;
+    for (const hd::GroupDev& d : hrg) same = same && d.trial0 == hrg[0].trial0;
+    if (same && pl->use_dbuf)
+      plan_dbuf(dg, dly == pl->delays.as<int>() ? pl->delays_h : pl->adelays_h, hrg[0].trial0,
+                (size_t)R, (size_t)std::max(n_live, 1));
   }
   auto threshold = [&]() -> hipError_t {
     hipError_t e = hd::launch_fill_u32(gb.counter, 1, 0u, s);
@@ -1770,6 +1815,8 @@ extern "C" hd_error hd_create_pipeline(hd_pipeline* pipeline_, hd_params params)
   {
     const char* g = getenv("HD_GRAPH");
     pl->use_graph = g ? atoi(g) != 0 : true;
+    const char* db = getenv("HD_DEDISP_DBUF");
+    pl->use_dbuf = db ? atoi(db) != 0 : true;
   }
   DeviceGuard dg(pl->device);
   if (params.verbosity >= 1)

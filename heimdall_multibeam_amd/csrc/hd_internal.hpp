@@ -160,11 +160,19 @@ struct DedispGeom {
   int wide = 0;        // byte staging at 32 trials: 1024 threads (2 trials per wave) instead of 512
   size_t tiles = 0;    // time tiles of the grid (0: ppitch / tile time); must reach every
                        // group's tile holding its sample nout
+  // byte staging at 32 trials on 1024 threads: DM blocks 0 .. dbuf_blocks - 1
+  // (each block's trials and window of at most wlen8_dbuf dwords the same in
+  // every group) run with two staging buffers; 0: none
+  int dbuf_blocks = 0;
+  int wlen8_dbuf = 0;
+  int mb_off = 0;      // (set by the launcher: the first DM block of a launch)
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
                              const DedispGeom& g, float* plane, hipStream_t s,
                              const GroupArgs& ga = GroupArgs{});
 size_t dedisp_lds_bytes(const DedispGeom& g);  // LDS per block of g's tile (limit 160 KiB)
+// the same with two staging buffers of wlen8 dwords per copy
+size_t dedisp_lds_bytes_dbuf(const DedispGeom& g, int wlen8);
 int dedisp_tile_time();     // outputs per block in time
 int dedisp_tile_dms();      // default DMs per block
 int dedisp_window_dwords(int spread);  // staged dwords per copy for a delay span

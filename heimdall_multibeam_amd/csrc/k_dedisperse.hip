@@ -370,17 +370,25 @@ constexpr int d8_waves(int db, int cc, bool split) {
 }
 constexpr int D8_PF = 2;                  // staging items (4 dwords) prefetched per thread
 
-template <int DB, int CC, bool SPLIT, int BS = D2_BS>
+// DBUF: two staging buffers, for DM blocks whose windows fit twice in the
+// LDS.  Chunk k + 1 is staged into one buffer while chunk k is summed from
+// the other, with one barrier per chunk instead of two; the blocks of a
+// launch are DM blocks g.mb_off .. of the grid's.
+template <int DB, int CC, bool SPLIT, int BS = D2_BS, bool DBUF = false>
 __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void k_dedisperse8(const uint8_t* __restrict__ x,
                                                       const int* __restrict__ delays,
                                                       DedispGeom g, int hw_max, int widen,
                                                       float* __restrict__ out,
                                                       const GroupDev* __restrict__ gd) {
-  extern __shared__ uint32_t lds8[];  // [CC][4][hw_max], then an int2 per live channel
+  // [buffer][CC][4][hw_max] (one buffer, or two with DBUF), then an int2 per
+  // live channel
+  extern __shared__ uint32_t lds8[];
   constexpr int WAVES = BS / 64;
   constexpr int NA = DB / WAVES;
-  __shared__ __align__(16) int sdly[DB][CC];
-  int2* ctab = reinterpret_cast<int2*>(lds8 + (size_t)CC * 4 * hw_max);
+  constexpr int NBUF = DBUF ? 2 : 1;
+  __shared__ __align__(16) int sdly[NBUF][DB][CC];
+  const int stage_words = CC * 4 * hw_max;
+  int2* ctab = reinterpret_cast<int2*>(lds8 + (size_t)NBUF * stage_words);
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
   const int trial0 = gd ? (int)gd[blockIdx.z].trial0 : 0;
   x += (size_t)(gd ? gd[blockIdx.z].xslot : blockIdx.z) * g.n_live * g.xpitch;
@@ -395,7 +403,7 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
     const unsigned q = nblk / 8, r = nblk % 8, xx = lin % 8;
     lin = xx * q + min(xx, r) + lin / 8;
   }
-  const int mb = (int)(lin % nbx);
+  const int mb = (int)(lin % nbx) + g.mb_off;
   const int m0 = mb * DB;
   const int ml = min(m0 + DB, g.n_dm) - 1;
   const size_t t0 = (size_t)(lin / nbx) * D2_TB;
@@ -456,9 +464,9 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
       }
     }
   };
-  // copy k, dword j: bytes 4j+k .. 4j+k+3 of the window
-  auto put4 = [&](int u, const uint32_t (&d)[5]) {
-    uint32_t* C = lds8 + (size_t)(scc * 4) * hw_max + 4 * u;
+  // copy k, dword j: bytes 4j+k .. 4j+k+3 of the window (in buffer `buf`)
+  auto put4 = [&](int buf, int u, const uint32_t (&d)[5]) {
+    uint32_t* C = lds8 + (size_t)buf * stage_words + (size_t)(scc * 4) * hw_max + 4 * u;
     *reinterpret_cast<uint4*>(C) = make_uint4(d[0], d[1], d[2], d[3]);
 #pragma unroll
     for (int k = 1; k < 4; ++k)
@@ -476,8 +484,81 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
       rd = delays[(size_t)(trial0 + mm) * g.n_live + c];
     }
   };
+  // stage chunk kk into buffer buf: its trial offsets from rd, its window
+  // from the prefetched items (and the rest loaded here), or zeros for
+  // channels past the last live one
+  auto stage = [&](int kk, int buf) {
+    const int c0 = kk * CC;
+    if (tid < DB * CC) {
+      const int c = c0 + tid % CC;
+      int p = buf * stage_words + (tid % CC) * 4 * hw_max;  // past the last channel: copy 0, zeros
+      if (c < g.n_live) {
+        const int o = rd - ctab[c].x;
+        p += (o & 3) * hw_max + (o >> 2);
+      }
+      sdly[buf][tid / CC][tid % CC] = p;
+    }
+    if (pf_ng > 0) {
+#pragma unroll
+      for (int r = 0; r < D8_PF; ++r) {
+        const int u = sjj + r * TPC;
+        if (u < pf_ng) put4(buf, u, pf[r]);
+      }
+      if (pf_ng > D8_PF * TPC) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(
+            x + (size_t)(c0 + scc) * g.xpitch + t0 + ctab[c0 + scc].x);
+        for (int u = sjj + D8_PF * TPC; u < pf_ng; u += TPC) {
+          uint32_t d[5];
+#pragma unroll
+          for (int i = 0; i < 5; ++i) d[i] = src[4 * u + i];
+          put4(buf, u, d);
+        }
+      }
+    } else if (pf_ng < 0) {
+      // the 64 * D8_Q dwords a trial reads at offset 0
+      uint32_t* C = lds8 + (size_t)buf * stage_words + (size_t)(scc * 4) * hw_max;
+      for (int u = sjj; u < 64 * D8_Q / 4; u += TPC)
+        *reinterpret_cast<uint4*>(C + 4 * u) = make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
+  auto add = [&](int a, const uint32_t* src) {
+#pragma unroll
+    for (int q = 0; q < D8_Q; ++q) {
+      const uint32_t v = src[64 * q];
+      pa[a][q] += v;
+      if (SPLIT) po[a][SPLIT ? q : 0] += __builtin_amdgcn_perm(0u, v, 0x0C030C01u);
+    }
+  };
+  // sums of the chunk staged in buffer buf
+  auto sum_chunk = [&](int buf) {
+    // four channels at a time: their offsets for the wave's trials in one
+    // 16-byte read per trial
+#pragma unroll
+    for (int v = 0; v < CC / 4; ++v) {
+      int off[NA][4];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) {
+        const int4 o4 = *reinterpret_cast<const int4*>(&sdly[buf][wave + WAVES * a][4 * v]);
+        off[a][0] = o4.x;
+        off[a][1] = o4.y;
+        off[a][2] = o4.z;
+        off[a][3] = o4.w;
+      }
+#pragma unroll
+      for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) add(a, lds8 + off[a][cc] + lane);
+    }
+  };
   prefetch(0);
   load_delay(0);
+  if (DBUF) {
+    // chunk 0 staged ahead of the loop
+    stage(0, 0);
+    prefetch(1);
+    load_delay(1);
+    __syncthreads();
+  }
   // Chunks in runs of widen / CC between widenings (widen >= CC), so that the
   // summing loop has no widening branch; a chunk's channels past the last
   // live one are staged as zeros.
@@ -486,69 +567,23 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
   while (k < nchunk) {
     const int kend = min(nchunk, k + per_widen);
     for (; k < kend; ++k) {
-      const int c0 = k * CC;
-      __syncthreads();
-      if (tid < DB * CC) {
-        const int c = c0 + tid % CC;
-        int p = (tid % CC) * 4 * hw_max;  // past the last channel: copy 0, zeros
-        if (c < g.n_live) {
-          const int o = rd - ctab[c].x;
-          p += (o & 3) * hw_max + (o >> 2);
-        }
-        sdly[tid / CC][tid % CC] = p;
-      }
-      if (pf_ng > 0) {
-#pragma unroll
-        for (int r = 0; r < D8_PF; ++r) {
-          const int u = sjj + r * TPC;
-          if (u < pf_ng) put4(u, pf[r]);
-        }
-        if (pf_ng > D8_PF * TPC) {
-          const uint32_t* src = reinterpret_cast<const uint32_t*>(
-              x + (size_t)(c0 + scc) * g.xpitch + t0 + ctab[c0 + scc].x);
-          for (int u = sjj + D8_PF * TPC; u < pf_ng; u += TPC) {
-            uint32_t d[5];
-#pragma unroll
-            for (int i = 0; i < 5; ++i) d[i] = src[4 * u + i];
-            put4(u, d);
-          }
-        }
-      } else if (pf_ng < 0) {
-        // the 64 * D8_Q dwords a trial reads at offset 0
-        uint32_t* C = lds8 + (size_t)(scc * 4) * hw_max;
-        for (int u = sjj; u < 64 * D8_Q / 4; u += TPC)
-          *reinterpret_cast<uint4*>(C + 4 * u) = make_uint4(0u, 0u, 0u, 0u);
-      }
-      // the next chunk's loads are issued before the barrier, as early as
-      // this chunk's staging registers are free
-      prefetch(k + 1);
-      load_delay(k + 1);
-      __syncthreads();
-      auto add = [&](int a, const uint32_t* src) {
-#pragma unroll
-        for (int q = 0; q < D8_Q; ++q) {
-          const uint32_t v = src[64 * q];
-          pa[a][q] += v;
-          if (SPLIT) po[a][SPLIT ? q : 0] += __builtin_amdgcn_perm(0u, v, 0x0C030C01u);
-        }
-      };
-      // four channels at a time: their offsets for the wave's trials in one
-      // 16-byte read per trial
-#pragma unroll
-      for (int v = 0; v < CC / 4; ++v) {
-        int off[NA][4];
-#pragma unroll
-        for (int a = 0; a < NA; ++a) {
-          const int4 o4 = *reinterpret_cast<const int4*>(&sdly[wave + WAVES * a][4 * v]);
-          off[a][0] = o4.x;
-          off[a][1] = o4.y;
-          off[a][2] = o4.z;
-          off[a][3] = o4.w;
-        }
-#pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
-#pragma unroll
-          for (int a = 0; a < NA; ++a) add(a, lds8 + off[a][cc] + lane);
+      if (DBUF) {
+        // chunk k + 1 into the other buffer (every read of it, chunk k - 1,
+        // ended at the previous barrier), then chunk k's sums
+        if (k + 1 < nchunk) stage(k + 1, (k + 1) & 1);
+        prefetch(k + 2);
+        load_delay(k + 2);
+        sum_chunk(k & 1);
+        __syncthreads();
+      } else {
+        __syncthreads();
+        stage(k, 0);
+        // the next chunk's loads are issued before the barrier, as early as
+        // this chunk's staging registers are free
+        prefetch(k + 1);
+        load_delay(k + 1);
+        __syncthreads();
+        sum_chunk(0);
       }
     }
 #pragma unroll
@@ -663,7 +698,7 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
 // Raises a kernel's dynamic-LDS limit once (never during a later capture:
 // every shape is launched eagerly before its chain is captured).
 hipError_t lds_limit(const void* fn, int slot, size_t lds) {
-  static size_t set[33] = {0};
+  static size_t set[40] = {0};
   if (lds <= 64 * 1024 || lds <= set[slot]) return hipSuccess;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess) set[slot] = lds;
@@ -678,9 +713,9 @@ template <int DB, int CC>
 const void* fn16() {
   return reinterpret_cast<const void*>(&k_dedisperse16<DB, CC>);
 }
-template <int DB, int CC, bool SPLIT, int BS = D2_BS>
+template <int DB, int CC, bool SPLIT, int BS = D2_BS, bool DBUF = false>
 const void* fn8() {
-  return reinterpret_cast<const void*>(&k_dedisperse8<DB, CC, SPLIT, BS>);
+  return reinterpret_cast<const void*>(&k_dedisperse8<DB, CC, SPLIT, BS, DBUF>);
 }
 const void* kernel16(int slot) {
   static const void* const t[9] = {fn16<8, 4>(),  fn16<16, 4>(),  fn16<32, 4>(),
@@ -708,11 +743,22 @@ const void* kernel8_wide(int cc, bool split) {
   const int k = cc == 4 ? 0 : (cc == 8 ? 1 : 2);
   return split ? ts[k] : t[k];
 }
+// the same with two staging buffers
+const void* kernel8_wide_dbuf(int cc, bool split) {
+  static const void* const t[3] = {fn8<32, 4, false, 1024, true>(),
+                                   fn8<32, 8, false, 1024, true>(),
+                                   fn8<32, 16, false, 1024, true>()};
+  static const void* const ts[3] = {fn8<32, 4, true, 1024, true>(), fn8<32, 8, true, 1024, true>(),
+                                    fn8<32, 16, true, 1024, true>()};
+  const int k = cc == 4 ? 0 : (cc == 8 ? 1 : 2);
+  return split ? ts[k] : t[k];
+}
 
-// Dynamic LDS of the byte kernel: the staged copies, then each live
-// channel's window (int2).
-static size_t dedisp8_dynamic_lds(const DedispGeom& g) {
-  return (size_t)g.cc * 4 * (size_t)g.wlen8 * sizeof(uint32_t) + (size_t)g.n_live * 8;
+// Dynamic LDS of the byte kernel: the staged copies (nbuf buffers of wlen8
+// dwords per copy), then each live channel's window (int2).
+static size_t dedisp8_dynamic_lds(const DedispGeom& g, int nbuf = 1, int wlen8 = 0) {
+  return (size_t)nbuf * g.cc * 4 * (size_t)(wlen8 ? wlen8 : g.wlen8) * sizeof(uint32_t) +
+         (size_t)g.n_live * 8;
 }
 
 // LDS of a tile: the staged copies (dynamic) plus the trial offsets (static).
@@ -720,6 +766,9 @@ size_t dedisp_lds_bytes(const DedispGeom& g) {
   const size_t offs = (size_t)g.db * g.cc * sizeof(int);
   if (g.widen8 > 0) return dedisp8_dynamic_lds(g) + offs;
   return (size_t)g.cc * 2 * (size_t)g.wlen * sizeof(uint32_t) + offs;
+}
+size_t dedisp_lds_bytes_dbuf(const DedispGeom& g, int wlen8) {
+  return dedisp8_dynamic_lds(g, 2, wlen8) + 2 * (size_t)g.db * g.cc * sizeof(int);
 }
 
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,
@@ -744,18 +793,38 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
     if (g.split8 && g.widen8 > 256) return hipErrorInvalidValue;   // 16-bit lanes would carry
     const size_t lds8 = dedisp8_dynamic_lds(g);
     const bool wide = g.wide && db == 32;
+    using K8 = void (*)(const uint8_t*, const int*, DedispGeom, int, int, float*, const GroupDev*);
+    // the leading g.dbuf_blocks DM blocks (windows of at most g.wlen8_dbuf
+    // dwords) on the double-buffered kernel, the rest on the single one
+    const int nb_dbuf = (wide && g.wlen8_dbuf >= 64 * D8_Q && g.wlen8_dbuf % 64 == 0 &&
+                         dedisp_lds_bytes_dbuf(g, g.wlen8_dbuf) <= 160 * 1024)
+                            ? min(g.dbuf_blocks, (int)grid.x)
+                            : 0;
+    if (nb_dbuf > 0) {
+      const int ci = cc == 4 ? 0 : cc == 8 ? 1 : 2;
+      const void* fd = kernel8_wide_dbuf(cc, g.split8 != 0);
+      const size_t ldsd = dedisp8_dynamic_lds(g, 2, g.wlen8_dbuf);
+      const hipError_t ed = lds_limit(fd, 33 + ci + (g.split8 ? 3 : 0), ldsd);
+      if (ed != hipSuccess) return ed;
+      DedispGeom gd1 = g;
+      gd1.mb_off = 0;
+      hipLaunchKernelGGL(reinterpret_cast<K8>(const_cast<void*>(fd)),
+                         dim3((unsigned)nb_dbuf, grid.y, grid.z), dim3(1024), ldsd, s, x, d_delays,
+                         gd1, g.wlen8_dbuf, g.widen8, plane, ga.gd);
+      const hipError_t el = hipGetLastError();
+      if (el != hipSuccess || nb_dbuf == (int)grid.x) return el;
+    }
     const void* f8 = wide ? kernel8_wide(cc, g.split8 != 0) : kernel8(slot, g.split8 != 0);
     const hipError_t e =
         lds_limit(f8, wide ? 27 + (cc == 4 ? 0 : cc == 8 ? 1 : 2) + (g.split8 ? 3 : 0)
                            : (g.split8 ? 18 : 9) + slot,
                   lds8);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(reinterpret_cast<void (*)(const uint8_t*, const int*, DedispGeom, int, int,
-                                                 float*, const GroupDev*)>(
-                           const_cast<void*>(f8)),
-                       grid, dim3(wide ? 1024 : D2_BS), lds8, s, x, d_delays, g, g.wlen8, g.widen8,
-                       plane,
-                       ga.gd);
+    DedispGeom g2 = g;
+    g2.mb_off = nb_dbuf;
+    hipLaunchKernelGGL(reinterpret_cast<K8>(const_cast<void*>(f8)),
+                       dim3(grid.x - (unsigned)nb_dbuf, grid.y, grid.z), dim3(wide ? 1024 : D2_BS),
+                       lds8, s, x, d_delays, g2, g.wlen8, g.widen8, plane, ga.gd);
     return hipGetLastError();
   }
   const size_t lds = (size_t)cc * 2 * (size_t)g.wlen * sizeof(uint32_t);
