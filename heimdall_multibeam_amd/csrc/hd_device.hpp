@@ -83,8 +83,12 @@ struct StretchBeam {
 
 // reblock_abs_beam_functor index (Pipeline/get_rms.cu:214-229):
 // (size_t)((float)beam*delta + (float)o), beam = o/length.
+// (o < 2^32: the quotient is 0 below the first period, and a 32-bit
+// division gives it when the length fits 32 bits; a 64-bit division is a
+// long instruction sequence.)
 __device__ __forceinline__ size_t reblock_pos(unsigned o, size_t length, float delta) {
-  const size_t beam = o / length;
+  const size_t beam =
+      (size_t)o < length ? 0 : (length > 0xffffffffull ? 0 : (size_t)(o / (unsigned)length));
   const float p = (float)beam * delta;
   const float q = p + (float)o;
   return (size_t)q;

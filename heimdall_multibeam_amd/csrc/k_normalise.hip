@@ -159,7 +159,9 @@ __global__ __launch_bounds__(NB_BS) void k_beam_rms_partials(const float* __rest
   if (b >= sh.nbeams) return;
   const int L0 = beam_rms_levels(sh.bc);
   const size_t chunk = pow5(L0);
-  const size_t np = sh.bc / chunk;
+  // (a 32-bit division when bc fits: the 64-bit one is a long sequence)
+  const size_t np =
+      sh.bc <= 0xffffffffull ? (size_t)((unsigned)sh.bc / (unsigned)chunk) : sh.bc / chunk;
   const size_t c = blockIdx.x;
   if (c >= np) return;
   const int m = blockIdx.y;
