@@ -160,9 +160,10 @@ def dedisp_block(name, t_dd, adds, b_in, plane, tile, t_step, rows):
     channel and trial), with its LDS and HBM views."""
     traffic = None
     if rows:
-        for r in rows:
-            if r["kernel"].startswith("k_dedisperse"):
-                traffic = r["fetch_bytes"] + r["write_bytes"]
+        # every launch of the stage (double- and single-buffered DM blocks)
+        dd = [r for r in rows if r["kernel"].startswith("k_dedisperse")]
+        if dd:
+            traffic = sum((r["fetch_bytes"] + r["write_bytes"]) * r["calls_per_step"] for r in dd)
     per = 1 if tile[2] else 2     # LDS bytes read per sample-add (byte or 16-bit staging)
     return {"kernel": name, "ms": round(t_dd * 1e3, 4), "share_of_step": round(t_dd / t_step, 4),
             "bound": "valu", "unit": "int32 adds/s", "achieved": adds / t_dd,

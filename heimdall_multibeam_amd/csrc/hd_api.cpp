@@ -783,6 +783,39 @@ void plan_dbuf(hd::DedispGeom& dg, const std::vector<int>& d, size_t r0, size_t 
   dg.wlen8_dbuf = wmax;
 }
 
+// The same per group, for groups that start at different trials (the gated
+// search's units): a group whose DM blocks' windows all fit two staging
+// buffers runs double-buffered (GroupDev::dbuf).  Returns whether any does.
+bool plan_dbuf_groups(hd::DedispGeom& dg, std::vector<hd::GroupDev>& groups,
+                      const std::vector<int>& d, size_t rows, size_t nl) {
+  dg.dbuf_blocks = 0;
+  dg.wlen8_dbuf = 0;
+  dg.dbuf_groups = 0;
+  for (hd::GroupDev& g : groups) g.dbuf = 0;
+  if (!(dg.widen8 > 0 && dg.wide && dg.db == 32) || nl == 0) return false;
+  int wlim = 0;  // the widest window two buffers hold
+  while (hd::dedisp_lds_bytes_dbuf(dg, wlim + 64) <= 160 * 1024) wlim += 64;
+  int wmax = 0;
+  for (hd::GroupDev& g : groups) {
+    if (d.size() < (g.trial0 + rows) * nl) continue;
+    int w = 0;
+    for (size_t m0 = 0; m0 < rows; m0 += 32) {
+      const size_t a = (g.trial0 + m0) * nl, b = (g.trial0 + std::min(m0 + 32, rows) - 1) * nl;
+      int spread = 0;
+      for (size_t l = 0; l < nl; ++l) spread = std::max(spread, d[b + l] - (d[a + l] & ~3));
+      w = std::max(w, hd::dedisp8_window_dwords(spread));
+    }
+    if (w <= wlim) {
+      g.dbuf = 1;
+      wmax = std::max(wmax, w);
+    }
+  }
+  if (wmax == 0) return false;
+  dg.dbuf_groups = 1;
+  dg.wlen8_dbuf = wmax;
+  return true;
+}
+
 hd_error choose_dedisp_tile(hd_pipeline pl, hd::DedispGeom& dg, const hd::GroupArgs& gr,
                             size_t nbits, hipStream_t s, const int** pairs) {
   // trials per block, remap, byte staging, channels per step, 1024 threads
@@ -1334,9 +1367,15 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
   {
     bool same = true;
     for (const hd::GroupDev& d : hrg) same = same && d.trial0 == hrg[0].trial0;
-    if (same && pl->use_dbuf)
-      plan_dbuf(dg, dly == pl->delays.as<int>() ? pl->delays_h : pl->adelays_h, hrg[0].trial0,
-                (size_t)R, (size_t)std::max(n_live, 1));
+    const std::vector<int>& table = dly == pl->delays.as<int>() ? pl->delays_h : pl->adelays_h;
+    if (same && pl->use_dbuf) {
+      plan_dbuf(dg, table, hrg[0].trial0, (size_t)R, (size_t)std::max(n_live, 1));
+    } else if (pl->use_dbuf &&
+               plan_dbuf_groups(dg, hrg, table, (size_t)R, (size_t)std::max(n_live, 1))) {
+      // the groups' flags go to the device table
+      HD_CHECK(hipMemcpyAsync(pl->rdev.p, hrg.data(), (size_t)Gr * sizeof(hd::GroupDev),
+                              hipMemcpyHostToDevice, s));
+    }
   }
   auto threshold = [&]() -> hipError_t {
     hipError_t e = hd::launch_fill_u32(gb.counter, 1, 0u, s);

@@ -95,6 +95,9 @@ struct GroupDev {
   uint32_t rows_valid = 0xFFFFFFFFu;
   uint32_t xslot = 0;
   uint32_t cgroup = 0;
+  // dedispersion with two staging buffers for this group (when the launch
+  // chooses per group, DedispGeom::dbuf_groups)
+  uint32_t dbuf = 0;
 };
 
 // Group arguments of a launch: gd == nullptr means one group described by
@@ -165,6 +168,9 @@ struct DedispGeom {
   // every group) run with two staging buffers; 0: none
   int dbuf_blocks = 0;
   int wlen8_dbuf = 0;
+  // instead, per group: the groups whose GroupDev::dbuf is set (every DM
+  // block's window at most wlen8_dbuf dwords) run with two buffers
+  int dbuf_groups = 0;
   int mb_off = 0;      // (set by the launcher: the first DM block of a launch)
 };
 hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d_chunk_pairs,

@@ -389,6 +389,8 @@ __global__ __launch_bounds__(BS, BS == D2_BS ? d8_waves(DB, CC, SPLIT) : 4) void
   __shared__ __align__(16) int sdly[NBUF][DB][CC];
   const int stage_words = CC * 4 * hw_max;
   int2* ctab = reinterpret_cast<int2*>(lds8 + (size_t)NBUF * stage_words);
+  // groups chosen per group: this instance takes only its own kind
+  if (g.dbuf_groups && gd && (gd[blockIdx.z].dbuf != 0) != DBUF) return;
   const size_t nout = gd ? gd[blockIdx.z].sh.nout : g.nout;
   const int trial0 = gd ? (int)gd[blockIdx.z].trial0 : 0;
   x += (size_t)(gd ? gd[blockIdx.z].xslot : blockIdx.z) * g.n_live * g.xpitch;
@@ -796,10 +798,30 @@ hipError_t launch_dedisperse(const uint8_t* x, const int* d_delays, const int* d
     using K8 = void (*)(const uint8_t*, const int*, DedispGeom, int, int, float*, const GroupDev*);
     // the leading g.dbuf_blocks DM blocks (windows of at most g.wlen8_dbuf
     // dwords) on the double-buffered kernel, the rest on the single one
-    const int nb_dbuf = (wide && g.wlen8_dbuf >= 64 * D8_Q && g.wlen8_dbuf % 64 == 0 &&
-                         dedisp_lds_bytes_dbuf(g, g.wlen8_dbuf) <= 160 * 1024)
-                            ? min(g.dbuf_blocks, (int)grid.x)
-                            : 0;
+    const bool dbuf_fits = wide && g.wlen8_dbuf >= 64 * D8_Q && g.wlen8_dbuf % 64 == 0 &&
+                           dedisp_lds_bytes_dbuf(g, g.wlen8_dbuf) <= 160 * 1024;
+    if (g.dbuf_groups) {
+      // per group: both instances over the whole grid, each taking its groups
+      if (!dbuf_fits || !ga.gd) return hipErrorInvalidValue;
+      const int ci = cc == 4 ? 0 : cc == 8 ? 1 : 2;
+      const void* fd = kernel8_wide_dbuf(cc, g.split8 != 0);
+      const size_t ldsd = dedisp8_dynamic_lds(g, 2, g.wlen8_dbuf);
+      hipError_t ed = lds_limit(fd, 33 + ci + (g.split8 ? 3 : 0), ldsd);
+      if (ed != hipSuccess) return ed;
+      DedispGeom g1 = g;
+      g1.mb_off = 0;
+      hipLaunchKernelGGL(reinterpret_cast<K8>(const_cast<void*>(fd)), grid, dim3(1024), ldsd, s, x,
+                         d_delays, g1, g.wlen8_dbuf, g.widen8, plane, ga.gd);
+      ed = hipGetLastError();
+      if (ed != hipSuccess) return ed;
+      const void* f8 = kernel8_wide(cc, g.split8 != 0);
+      ed = lds_limit(f8, 27 + ci + (g.split8 ? 3 : 0), lds8);
+      if (ed != hipSuccess) return ed;
+      hipLaunchKernelGGL(reinterpret_cast<K8>(const_cast<void*>(f8)), grid, dim3(1024), lds8, s,
+                         x, d_delays, g1, g.wlen8, g.widen8, plane, ga.gd);
+      return hipGetLastError();
+    }
+    const int nb_dbuf = dbuf_fits ? min(g.dbuf_blocks, (int)grid.x) : 0;
     if (nb_dbuf > 0) {
       const int ci = cc == 4 ? 0 : cc == 8 ? 1 : 2;
       const void* fd = kernel8_wide_dbuf(cc, g.split8 != 0);
