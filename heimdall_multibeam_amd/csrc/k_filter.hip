@@ -152,6 +152,19 @@ __global__ __launch_bounds__(FB_BS) void k_block_rms_partials(
       const size_t span = bs.chunk + (ws.ahead + ws.behind) / ws.ts;
       if (span > tile_words || (ws.ahead + ws.behind) % ws.ts != 0) src_j = -1;
     }
+    // A stale chunk holds width src_j's normalised values |F * inv|, and the
+    // chunk is all fresh for src_j, whose partial (an earlier launch) is the
+    // level-L0 median of |F| here.  The pyramid only selects values and
+    // rounding x * inv is monotone for a positive finite inv, so the median
+    // of |F * inv| is that partial times inv, bit for bit.
+    if (src_j >= 0 && src_j < first_w) {
+      const float sv = partial[(row * wt.nw + src_j) * np_max + c];
+      const float sc = winv[src_j];
+      if (sc > 0.f && sc < INFINITY && sv < INFINITY) {  // (sv >= 0; NaN fails both)
+        if (threadIdx.x == 0) partial[(row * wt.nw + first_w) * np_max + c] = sv * sc;
+        return;
+      }
+    }
   }
   const size_t tlen = use_tile ? bs.chunk + amax + bmax : (src_j >= 0 ? tile_words : 0);
   float* T = lds;
