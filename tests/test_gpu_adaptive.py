@@ -98,3 +98,23 @@ def test_adaptive_full_1024ch_gulp(oracle):
     assert err == r.error == H.HD_NO_ERROR
     assert len(r.groups) > 10
     _compare(gpu, r, oracle, w)
+
+
+def test_gated_search_refuses_adaptive_dt():
+    """The gated search runs at full resolution only: with scrunching on and a
+    proposal set, hd_execute returns HD_INVALID_PIPELINE; with the proposal
+    off again the same pipeline searches adaptively."""
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import proposal as P, synth
+    from tests.test_gpu_proposal import _random_params
+    w = SCRUNCH_128
+    fb = synth.make_gulp(w, 3)
+    pl = H.Pipeline(params_for(w, **ON_GPU))
+    pl.set_output(False)
+    P.set_proposal(pl, _random_params(8), -1e30)
+    err, _ = pl.execute(fb, w.nsamps, w.nbits, allow=(H.HD_INVALID_PIPELINE,))
+    assert err == H.HD_INVALID_PIPELINE
+    P.set_proposal(pl, None, 0)
+    err, _ = pl.execute(fb, w.nsamps, w.nbits)
+    assert err == H.HD_NO_ERROR and len(pl.giants()["peak"]) > 0
+    pl.close()
