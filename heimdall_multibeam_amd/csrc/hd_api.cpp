@@ -122,6 +122,7 @@ struct hd_pipeline_t {
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
   DevBuf c_sind, c_sfd;
   DevBuf c_ecount, c_eoff, c_adj;  // coincidence lists
+  DevBuf c_gb;                     // per clustering group: first giant, first list entry
   DevBuf o_peak, o_ind, o_begin, o_end, o_filter, o_dm, o_members;
   DevBuf clean_out, clean_work;
   DevBuf gdev, g_group, o_group;        // group table, group of each giant / candidate
@@ -1717,6 +1718,12 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
       return nullptr;
     return pl->c_adj.as<uint32_t>();
   };
+  // the giants' groups are the gulps, in index order (giants are kept in
+  // (gulp, trial, width, time) order)
+  const unsigned n_cg = (unsigned)pl->hgd.size();
+  HD_CHECK(pl->c_gb.ensure((2 * (size_t)n_cg + 3) * sizeof(uint32_t)));
+  cb.gbounds = pl->c_gb.as<uint32_t>();
+  cb.n_cgroups = n_cg;
   const unsigned max_filter = wt.wi[wt.nw - 1].filter_idx;
   int iters = 0;
   HD_CHECK(hd::launch_cluster(pl->g_ind.as<uint64_t>(), pl->g_filter.as<uint32_t>(),

@@ -296,6 +296,13 @@ struct ClusterBuffers {
   uint32_t* ecount = nullptr;
   uint32_t* eoff = nullptr;
   std::function<uint32_t*(size_t)> adj_alloc;
+  // clustering groups (optional, with the lists): n_cgroups groups (the
+  // gulps of a call, or 1 without a group column) whose giants are
+  // contiguous in index order; gbounds holds 2 * n_cgroups + 3 entries.  The
+  // lists are then relaxed per group, one block each with the group's labels
+  // in LDS, when every group fits there.
+  uint32_t* gbounds = nullptr;
+  unsigned n_cgroups = 0;
 };
 // With group (per giant) and gd, giants of different groups never coincide
 // and each group uses its own beam length; otherwise nsamps_beam applies.

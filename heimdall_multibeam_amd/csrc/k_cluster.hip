@@ -19,6 +19,7 @@
 // with the maximum S/N and its member count is the sum.  A 64-bit atomic max
 // over (orderable S/N, ~index) picks that giant.
 #include <algorithm>
+#include <vector>
 
 #include "hd_device.hpp"
 #include "hd_internal.hpp"
@@ -446,38 +447,84 @@ __global__ __launch_bounds__(256) void k_relax_lists(unsigned n,
 
 __global__ void k_clear_flag(unsigned* flag) { *flag = 0u; }
 
-// The same relaxation run to its fixed point inside one 1024-thread block,
-// for up to kRelaxBlock giants: the passes repeat (each the update of
-// k_relax_lists for every giant) until one changes no label, with a
-// block-wide barrier between passes instead of a host check of the flag.
-// Labels only decrease, so the loop ends; the fixed point (each giant
-// labelled with the smallest index reachable along decreasing coincident
-// chains) does not depend on the order of the updates.  After pass k every
-// giant whose longest decreasing chain has at most k links holds its final
-// label, so at most n + 1 passes run (the last one changes nothing): the
-// pass bound below is never what ends the loop.  Every wave runs the same
-// number of rounds: the bound is a multiple of the 128 giants a round covers.
-constexpr unsigned kRelaxBlock = 16384;
-constexpr int kRelaxBlockPasses = 100000;
-static_assert(kRelaxBlock + 2 <= (unsigned)kRelaxBlockPasses, "pass bound below n + 2");
-__global__ __launch_bounds__(1024) void k_relax_lists_block(unsigned n,
-                                                            const uint32_t* __restrict__ eoff,
-                                                            const uint32_t* __restrict__ adj,
-                                                            int32_t* labels) {
+// Clustering groups (the gulps of a call) lie further apart than any
+// coincidence window, so every list entry of a giant is a giant of its own
+// group; their giants are contiguous in index order.  Thread i (0 .. n)
+// books the groups that start at giant i: gb[2g] = first giant of group g,
+// gb[2g + 1] = first list entry of that giant (gb[2G], gb[2G + 1] close the
+// last group).  gb[2G + 2] (cleared before) is set when the group column
+// decreases or holds a group >= G: the ranges are then not used.
+__global__ void k_group_bounds(const uint32_t* __restrict__ group, unsigned n, unsigned G,
+                               const uint32_t* __restrict__ eoff, uint32_t* gb) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const long long gp = i == 0 ? -1 : (long long)(group ? group[i - 1] : 0u);
+  const long long gc = i == n ? (long long)G : (long long)(group ? group[i] : 0u);
+  if (gc < gp || (i < n && gc >= (long long)G)) gb[2 * (size_t)G + 2] = 1u;
+  const uint32_t e = eoff[i];
+  for (long long g = gp + 1; g <= gc && g <= (long long)G; ++g) {
+    gb[2 * g] = i;
+    gb[2 * g + 1] = e;
+  }
+}
+
+// The list relaxation of one clustering group with its labels in LDS: one
+// 1024-thread block per group (giants gb[2b] .. gb[2b + 2] - 1, at most
+// kLdsGroup; labels local to the group).  A pass updates every giant in
+// index order, 128 giants per round with 8 lanes each and a barrier between
+// rounds, so that a round sees the labels of the rounds before it and
+// chains that run forward in index order settle in one pass.  Passes repeat
+// until one changes no label.  Labels only decrease, so the loop ends, and
+// the fixed point (each giant labelled with the smallest index reachable
+// along decreasing coincident chains) does not depend on the order of the
+// updates.  After pass k every giant whose longest decreasing chain has at
+// most k links holds its final label, so at most ng + 1 passes run for a
+// group of ng giants (the last one changes nothing): the pass bound ng + 2
+// is never what ends the loop.
+constexpr unsigned kLdsGroup = 16384;  // giants per group (a 32-bit label each)
+__global__ __launch_bounds__(1024) void k_relax_lists_lds(const uint32_t* __restrict__ gb,
+                                                          const uint32_t* __restrict__ eoff,
+                                                          const uint32_t* __restrict__ adj,
+                                                          int32_t* __restrict__ labels) {
+  extern __shared__ int32_t lab[];
   constexpr unsigned kPer = 1024 / kListLanes;  // giants per round
+  constexpr int U = 4;                          // list entries in flight per lane
+  const unsigned a = gb[2 * blockIdx.x], e = gb[2 * blockIdx.x + 2];
+  if (a >= e) return;
+  const unsigned n = e - a;
+  for (unsigned k = threadIdx.x; k < n; k += 1024) lab[k] = (int32_t)k;
+  __syncthreads();
   const unsigned sub = threadIdx.x % kListLanes;
   const unsigned npad = (n + kPer - 1) / kPer * kPer;
-  for (int pass = 0; pass < kRelaxBlockPasses; ++pass) {
+  for (unsigned pass = 0; pass < n + 2; ++pass) {
     bool changed = false;
     for (unsigned i = threadIdx.x / kListLanes; i < npad; i += kPer) {
       const bool live = i < n;
-      const int32_t cur = live ? __atomic_load_n(&labels[i], __ATOMIC_RELAXED) : 0x7fffffff;
+      const int32_t cur = live ? __atomic_load_n(&lab[i], __ATOMIC_RELAXED) : 0x7fffffff;
       int32_t best = cur;
-      if (live && sub == 0) best = follow_labels(labels, cur);
-      const unsigned p1 = live ? eoff[i + 1] : 0u;
-      for (unsigned p = (live ? eoff[i] : 0u) + sub; p < p1; p += kListLanes) {
-        const int32_t lj = __atomic_load_n(&labels[adj[p]], __ATOMIC_RELAXED);
-        if (lj < best) best = lj;
+      if (live && sub == 0) {  // follow_labels over the group's labels
+        int32_t r = cur;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int32_t nx = __atomic_load_n(&lab[r], __ATOMIC_RELAXED);
+          if (nx >= r) break;
+          r = nx;
+        }
+        best = r;
+      }
+      const unsigned p0 = live ? eoff[a + i] : 0u, p1 = live ? eoff[a + i + 1] : 0u;
+      for (unsigned p = p0 + sub; p < p1; p += kListLanes * U) {
+        uint32_t jj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const unsigned q = p + (unsigned)u * kListLanes;
+          jj[u] = adj[q < p1 ? q : p] - a;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int32_t lj = __atomic_load_n(&lab[jj[u]], __ATOMIC_RELAXED);
+          if (lj < best) best = lj;
+        }
       }
 #pragma unroll
       for (int o = kListLanes / 2; o > 0; o >>= 1) {
@@ -485,12 +532,14 @@ __global__ __launch_bounds__(1024) void k_relax_lists_block(unsigned n,
         best = other < best ? other : best;
       }
       if (live && sub == 0 && best < cur) {
-        __atomic_store_n(&labels[i], best, __ATOMIC_RELAXED);
+        __atomic_store_n(&lab[i], best, __ATOMIC_RELAXED);
         changed = true;
       }
+      __syncthreads();  // the next round sees this round's labels
     }
     if (!__syncthreads_or(changed)) break;
   }
+  for (unsigned k = threadIdx.x; k < n; k += 1024) labels[a + k] = (int32_t)a + lab[k];
 }
 
 __global__ void k_merge_init(unsigned n, const int32_t* __restrict__ labels, uint64_t* best,
@@ -548,9 +597,10 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
   (void)members_in;
   *iterations = 0;
   if (n == 0) return hipSuccess;
+  const unsigned T = 256;
+  hipError_t e = hipSuccess;
   size_t npad = 1;
   while (npad < n) npad <<= 1;
-  const unsigned T = 256;
   // banded windows need the sorted copies, a packed filter and coordinates
   // below 2^27 (the key holds filter:5 | coordinate:27 | index:32), and
   // per-giant window arrays of nb entries (lo/hi sized by the caller)
@@ -592,7 +642,7 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     hipLaunchKernelGGL(k_windows, dim3((n + T - 1) / T), dim3(T), 0, s, ind, group, gd, n,
                        cb.sort_keys, span, cb.lo, cb.hi, cb.labels);
   }
-  hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   // the sorted copies need filter < 32 and dm < 2^27 to pack
   const bool sorted = cb.s_ind && cb.s_fd && max_filter < 32;
@@ -603,7 +653,10 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
   // (at most kMaxEdges entries; beyond that the passes scan the windows)
   constexpr size_t kMaxEdges = size_t(1) << 26;
   uint32_t* adj = nullptr;
-  size_t n_edges = 0;
+  // per clustering group (a search without a group column is one group):
+  // its range of giants and of list entries, read back with the edge count
+  const unsigned G = group ? cb.n_cgroups : 1u;
+  std::vector<uint32_t> gbh;
   if (banded && cb.ecount && cb.eoff && cb.adj_alloc) {
     hipLaunchKernelGGL(k_edges_banded_wave<0>, dim3((n + 3) / 4), dim3(256), 0, s, ind, filter,
                        dm, n, nb, cb.sort_keys, cb.s_ind, cb.s_fd, cb.lo, cb.hi, time_tol,
@@ -612,12 +665,20 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
     e = launch_fill_u32(cb.ecount + n, 1, 0u, s);
     if (e == hipSuccess) e = launch_exclusive_scan_u32(cb.ecount, cb.eoff, (size_t)n + 1,
                                                        cb.scan_tmp, s);
+    if (e == hipSuccess && cb.gbounds && G >= 1) {
+      e = launch_fill_u32(cb.gbounds + 2 * (size_t)G + 2, 1, 0u, s);
+      hipLaunchKernelGGL(k_group_bounds, dim3((n + 1 + T - 1) / T), dim3(T), 0, s, group, n, G,
+                         cb.eoff, cb.gbounds);
+      gbh.resize(2 * (size_t)G + 3);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(gbh.data(), cb.gbounds, gbh.size() * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, s);
+    }
     if (e == hipSuccess)
       e = hipMemcpyAsync(h_flag_pinned, cb.eoff + n, sizeof(unsigned), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     const size_t edges = *h_flag_pinned;
-    n_edges = edges;
     if (edges <= kMaxEdges) adj = cb.adj_alloc(edges);
     if (!adj) (void)hipGetLastError();  // a failed allocation leaves the windows path
     if (adj)
@@ -626,17 +687,25 @@ hipError_t launch_cluster(const uint64_t* ind, const uint32_t* filter, const uin
                          time_tol, filter_tol, dm_tol, nsamps_beam, group, gd, cb.ecount,
                          cb.eoff, adj);
   }
-  // one block for small searches (lists of up to 2^20 entries: about a
-  // thousand reads per thread and pass)
-  if (adj && n <= kRelaxBlock && n_edges <= (size_t(1) << 20)) {
-    hipLaunchKernelGGL(k_relax_lists_block, dim3(1), dim3(1024), 0, s, n, cb.eoff, adj,
-                       cb.labels);
-    *iterations = 0;  // passes are counted inside the block only
-    return hipGetLastError();
+  // One block per clustering group with its labels in LDS, when the groups'
+  // ranges are sound and every group has at most kLdsGroup giants.
+  if (adj && !gbh.empty()) {
+    bool fits = gbh[2 * (size_t)G + 2] == 0 && gbh[0] == 0 && gbh[2 * (size_t)G] == n;
+    uint32_t most = 0;
+    for (unsigned g = 0; fits && g < G; ++g) {
+      if (gbh[2 * g + 2] < gbh[2 * g] || gbh[2 * g + 3] < gbh[2 * g + 1]) fits = false;
+      else most = std::max(most, gbh[2 * g + 2] - gbh[2 * g]);
+    }
+    if (fits && most <= kLdsGroup) {
+      const size_t lds = std::max<size_t>(most, 1) * sizeof(int32_t);
+      hipLaunchKernelGGL(k_relax_lists_lds, dim3(G), dim3(1024), lds, s, cb.gbounds, cb.eoff,
+                         adj, cb.labels);
+      return hipGetLastError();
+    }
   }
   // relaxation passes run in batches between host checks of the change flag
   // (list passes are cheap next to a check, window passes are not).  At most
-  // n + 1 passes are needed (see k_relax_lists_block), so a flag still set
+  // n + 1 passes are needed (see k_relax_lists_lds), so a flag still set
   // after n + 2 passes means the labels are wrong: that is reported.
   const int kBatch = adj ? 2 : 4;
   const size_t max_passes = (size_t)n + 2;
