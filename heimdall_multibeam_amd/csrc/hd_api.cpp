@@ -2076,7 +2076,8 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     DevBuf* bufs[] = {&pl->clean_out, &pl->clean_work, &pl->fb, &pl->x, &pl->live, &pl->delays, &pl->plane, &pl->pts, &pl->rms, &pl->brms,
                       &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
-                      &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->okey, &pl->oval,
+                      &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->c_gb,
+                      &pl->okey, &pl->oval,
                       &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,
                       &pl->g_begin, &pl->g_end, &pl->g_filter, &pl->g_dm, &pl->g_best, &pl->c_keys,
                       &pl->c_lo, &pl->c_hi, &pl->c_labels, &pl->c_flag, &pl->c_best,

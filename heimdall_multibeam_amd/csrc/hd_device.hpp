@@ -193,6 +193,32 @@ __device__ __forceinline__ float* lds_pyramid(float* src, float* dst, int n, int
   return src;
 }
 
+// Finishes a median-of-5 pyramid held in A (n values) with the reference's
+// loop `for (size = n; size > 1; size /= 5)` (get_rms.cu:83-87, 119-123):
+// groups of five while size > 4, then the small-count rule; no value (n = 0)
+// gives 0, the zeroed buffer's first entry.  Returns the final value to
+// every thread of the block.
+template <int BS>
+__device__ __forceinline__ float lds_finish(float* A, float* B, size_t n) {
+  __shared__ float result;
+  while (n > 4) {
+    const size_t m = n / 5;
+    __syncthreads();
+    for (size_t j = threadIdx.x; j < m; j += BS) {
+      const float* p = A + 5 * j;
+      B[j] = med5(p[0], p[1], p[2], p[3], p[4]);
+    }
+    float* t = A;
+    A = B;
+    B = t;
+    n = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) result = (n >= 1) ? med_tail(A, (int)n) : 0.f;
+  __syncthreads();
+  return result;
+}
+
 // Median of 5^L consecutive leaves starting at `first` (L in 0..3), leaves
 // produced by the functor `leaf(k)`.
 template <typename Leaf>

@@ -62,27 +62,6 @@ __device__ __forceinline__ float filtered_at(const float* S, const WidthTable& w
   return 0.f;
 }
 
-template <int BS>
-__device__ __forceinline__ float lds_finish_f(float* A, float* B, size_t n) {
-  __shared__ float result;
-  while (n > 4) {
-    const size_t m = n / 5;
-    __syncthreads();
-    for (size_t j = threadIdx.x; j < m; j += BS) {
-      const float* p = A + 5 * j;
-      B[j] = med5(p[0], p[1], p[2], p[3], p[4]);
-    }
-    float* t = A;
-    A = B;
-    B = t;
-    n = m;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) result = (n >= 1) ? med_tail(A, (int)n) : 0.f;
-  __syncthreads();
-  return result;
-}
-
 // dst[t] = src[t * stride] for t < n (LDS from global), with SU loads in
 // flight per thread before their LDS writes: a plain loop waits out the
 // memory latency once per element.
@@ -281,7 +260,7 @@ __global__ __launch_bounds__(64) void k_block_rms_finish(const float* __restrict
   float* B = lds + np;
   for (size_t q = threadIdx.x; q < np; q += 64) A[q] = p[q];
   __syncthreads();
-  const float med = lds_finish_f<64>(A, B, np);
+  const float med = lds_finish<64>(A, B, np);
   if (threadIdx.x == 0) {
     const float r = (float)((double)med * 1.4862);  // get_rms.cu:127-128
     wrms[row * nw_total + j] = r;

@@ -31,30 +31,6 @@ __device__ __forceinline__ size_t pow5(int L) {
   return p;
 }
 
-// Finishes a median-of-5 pyramid held in A (n values) with the reference's
-// loop `for (size = n; size > 1; size /= 5)`: groups of five while size > 4,
-// then the small-count rule.  Returns the final value to every thread.
-template <int BS>
-__device__ __forceinline__ float lds_finish(float* A, float* B, size_t n) {
-  __shared__ float result;
-  while (n > 4) {
-    const size_t m = n / 5;
-    __syncthreads();
-    for (size_t j = threadIdx.x; j < m; j += BS) {
-      const float* p = A + 5 * j;
-      B[j] = med5(p[0], p[1], p[2], p[3], p[4]);
-    }
-    float* t = A;
-    A = B;
-    B = t;
-    n = m;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) result = (n >= 1) ? med_tail(A, (int)n) : 0.f;
-  __syncthreads();
-  return result;
-}
-
 // blockIdx: (beam, trial, group); the group's rows are g*n_dm + m and each
 // row holds prow baseline points.
 __global__ __launch_bounds__(NB_BS) void k_baseline_points(const float* __restrict__ plane,
