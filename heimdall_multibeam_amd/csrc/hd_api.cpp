@@ -381,6 +381,10 @@ int series_shape(float cur_dt, float baseline_length, SeriesShape& sh) {
     int Ld = 0;
     while (Ld < std::min(sh.nscr, 3) && sh.count_round / pow5_host(Ld) > 12288) ++Ld;
     if (sh.count_round / pow5_host(Ld) > 12288) return HD_INVALID_STRIDE;
+    // at least the first level straight from the leaves (the same medians)
+    // when every thread of a 256-thread block still keeps a value: five loads
+    // in flight per value for short series
+    if (Ld == 0 && sh.nscr >= 1 && sh.count_round / 5 >= 256) Ld = 1;
     sh.bl_direct = Ld;
     sh.step_rs_beam = (float)(sh.stride - 1) / (float)sh.count_round;
     sh.step_rs_last = (float)(sh.bc - 1) / (float)(sh.count_round - 1);
