@@ -8,10 +8,21 @@
 // to the front of the buffer and the next gulp is read behind them
 // (heimdall.C:176-292, here for one beam).  A gulp shorter than nsamps_gulp
 // ends the loop without being searched, as in the reference.
+//
+// The gulps live in device memory, in two buffers used in turn: while gulp k
+// is searched (hd_execute_device, the same search as hd_execute), a second
+// thread reads the next nsamps_gulp samples from the file into page-locked
+// memory and copies them to the device on a stream of its own.  After the
+// call the next gulp is put together on the device from the unfinished tail
+// and those samples, so neither the file read nor the host-to-device copy
+// waits for the search or delays it.
+#include <hip/hip_runtime_api.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <iostream>
+#include <thread>
 #include <vector>
 
 #include "hd/default_params.h"
@@ -72,22 +83,68 @@ int main(int argc, char* argv[]) {
     return -1;
   }
   const size_t max_overlap = hd_get_max_overlap(pipeline, params.nbeams);
-  std::vector<hd_byte> buf((nsamps_gulp + max_overlap) * stride);
+  // device buffers of (nsamps_gulp + max_overlap) samples for the gulps, one
+  // of nsamps_gulp for the next gulp's new samples, page-locked host memory
+  // for the file reads, and the copy stream (default priority)
+  const size_t gulp_bytes = (nsamps_gulp + max_overlap) * stride;
+  const size_t new_bytes = nsamps_gulp * stride;
+  hd_byte* dev[2] = {nullptr, nullptr};
+  hd_byte* stage = nullptr;
+  hd_byte* pin = nullptr;
+  hipStream_t cs = nullptr;
+  auto release = [&]() {
+    if (cs) (void)hipStreamDestroy(cs);
+    for (hd_byte* d : dev)
+      if (d) (void)hipFree(d);
+    if (stage) (void)hipFree(stage);
+    if (pin) (void)hipHostFree(pin);
+    hd_destroy_pipeline(pipeline);
+    hd_sigproc_close(src);
+  };
+  if (hipSetDevice((int)params.gpu_id) != hipSuccess ||
+      hipMalloc((void**)&dev[0], gulp_bytes) != hipSuccess ||
+      hipMalloc((void**)&dev[1], gulp_bytes) != hipSuccess ||
+      hipMalloc((void**)&stage, new_bytes) != hipSuccess ||
+      hipHostMalloc((void**)&pin, new_bytes, 0) != hipSuccess ||
+      hipStreamCreate(&cs) != hipSuccess) {
+    std::cerr << "ERROR: device buffers for the gulps could not be allocated" << std::endl;
+    release();
+    return -1;
+  }
   if (params.verbosity >= 1)
     std::cout << "Beginning data processing, requesting " << nsamps_gulp << " samples"
               << std::endl;
 
   size_t total_nsamps = 0;
   size_t overlap = 0;
-  size_t nsamps_read = hd_sigproc_read(src, nsamps_gulp, buf.data());
+  int cur = 0;
+  size_t nsamps_read = hd_sigproc_read(src, nsamps_gulp, pin);
+  if (nsamps_read == nsamps_gulp &&
+      (hipMemcpyAsync(dev[cur], pin, new_bytes, hipMemcpyHostToDevice, cs) != hipSuccess ||
+       hipStreamSynchronize(cs) != hipSuccess)) {
+    std::cerr << "ERROR: copy of the first gulp to the device failed" << std::endl;
+    release();
+    return -1;
+  }
   while (nsamps_read == nsamps_gulp) {
     const size_t nsamps_to_process = nsamps_gulp + overlap;
     hd_size nsamps_processed = 0;
     if (params.verbosity >= 1)
       std::cout << "Executing pipeline on new gulp of " << nsamps_read
                 << " samples, nsamps_to_process=" << nsamps_to_process << std::endl;
-    error = hd_execute(pipeline, buf.data(), nsamps_to_process, nbits, total_nsamps,
-                       params.nbeams, &nsamps_processed);
+    // the next gulp's new samples, read and copied beside the search
+    size_t nsamps_next = 0;
+    hipError_t copy_err = hipSuccess;
+    std::thread reader([&]() {
+      copy_err = hipSetDevice((int)params.gpu_id);
+      nsamps_next = hd_sigproc_read(src, nsamps_gulp, pin);
+      if (copy_err == hipSuccess && nsamps_next == nsamps_gulp)
+        copy_err = hipMemcpyAsync(stage, pin, new_bytes, hipMemcpyHostToDevice, cs);
+    });
+    error = hd_execute_device(pipeline, dev[cur], nsamps_to_process, nbits, total_nsamps,
+                              params.nbeams, &nsamps_processed);
+    reader.join();
+    if (copy_err == hipSuccess) copy_err = hipStreamSynchronize(cs);
     if (error == HD_NO_ERROR) {
       if (params.verbosity >= 1)
         std::cout << "Processed " << nsamps_processed << " samples." << std::endl;
@@ -98,8 +155,12 @@ int main(int argc, char* argv[]) {
     } else {
       std::cerr << "ERROR: Pipeline execution failed" << std::endl;
       std::cerr << "       " << hd_get_error_string(error) << std::endl;
-      hd_destroy_pipeline(pipeline);
-      hd_sigproc_close(src);
+      release();
+      return -1;
+    }
+    if (copy_err != hipSuccess) {
+      std::cerr << "ERROR: copy of the next gulp to the device failed" << std::endl;
+      release();
       return -1;
     }
     overlap = nsamps_to_process - nsamps_processed;   // heimdall.C:210
@@ -107,20 +168,28 @@ int main(int argc, char* argv[]) {
     if (overlap > max_overlap) {
       std::cerr << "ERROR: overlap " << overlap << " exceeds the maximum " << max_overlap
                 << std::endl;
-      hd_destroy_pipeline(pipeline);
-      hd_sigproc_close(src);
+      release();
       return -1;
     }
-    // keep the unprocessed tail, then read the next gulp behind it
-    std::copy(buf.begin() + (ptrdiff_t)(nsamps_processed * stride),
-              buf.begin() + (ptrdiff_t)(nsamps_to_process * stride), buf.begin());
-    nsamps_read = hd_sigproc_read(src, nsamps_gulp, buf.data() + overlap * stride);
+    nsamps_read = nsamps_next;
+    if (nsamps_read != nsamps_gulp) break;  // a short last gulp is not searched
+    // the next gulp: the unprocessed tail, then the new samples behind it
+    const int nxt = 1 - cur;
+    if (hipMemcpyAsync(dev[nxt], dev[cur] + nsamps_processed * stride, overlap * stride,
+                       hipMemcpyDeviceToDevice, cs) != hipSuccess ||
+        hipMemcpyAsync(dev[nxt] + overlap * stride, stage, new_bytes, hipMemcpyDeviceToDevice,
+                       cs) != hipSuccess ||
+        hipStreamSynchronize(cs) != hipSuccess) {
+      std::cerr << "ERROR: assembling the next gulp on the device failed" << std::endl;
+      release();
+      return -1;
+    }
+    cur = nxt;
   }
   if (params.verbosity >= 1)
     std::cout << "Successfully processed a total of " << total_nsamps << " samples."
               << std::endl;
-  hd_destroy_pipeline(pipeline);
-  hd_sigproc_close(src);
+  release();
   if (params.verbosity >= 1) std::cout << "All done." << std::endl;
   return 0;
 }

@@ -30,6 +30,9 @@ HIP_SOURCES = ["hd_api.cpp", "dedisp_plan.cpp", "formats.cpp", "k_dedisperse.hip
 APPS = {"heimdall": "heimdall_main.cpp", "generate_dmlist": "generate_dmlist.cpp",
         "coincidencer": "coincidencer.cpp", "candidate_profiler": "candidate_profiler.cpp"}
 BIN = os.path.join(HERE, "bin")
+# tools that also call the HIP runtime (device buffers and a copy stream)
+APP_HIP = {"heimdall"}
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -67,9 +70,13 @@ def build(verbose=False):
     os.makedirs(BIN, exist_ok=True)
     for name, src in APPS.items():
         out = os.path.join(BIN, name)
-        _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
-              os.path.join(HERE, "app", src), "-o", out + ".tmp", "-L" + HERE,
-              "-lheimdall_mi355x", "-Wl,-rpath,$ORIGIN/.."])
+        hip = (["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
+               if name in APP_HIP else [])
+        hip_lib = (["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-pthread"]
+                   if name in APP_HIP else [])
+        _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include")] + hip +
+             [os.path.join(HERE, "app", src), "-o", out + ".tmp", "-L" + HERE,
+              "-lheimdall_mi355x"] + hip_lib + ["-Wl,-rpath,$ORIGIN/.."])
         os.replace(out + ".tmp", out)
     if verbose:
         print("built", LIB, SYNTH, BIN)

@@ -135,3 +135,36 @@ def copy_device(dst, src, nbytes):
 
 def synchronize():
     check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    """A HIP stream at the default priority, for copies that run beside the
+    pipeline's own stream."""
+
+    def __init__(self):
+        h = hip()
+        h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        h.hipStreamDestroy.argtypes = [C.c_void_p]
+        h.hipStreamSynchronize.argtypes = [C.c_void_p]
+        h.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        self.ptr = C.c_void_p()
+        check(h.hipStreamCreate(C.byref(self.ptr)), "hipStreamCreate")
+
+    def copy(self, dst, src, nbytes, kind):
+        """Asynchronous copy of nbytes (addresses as integers) on this stream."""
+        check(hip().hipMemcpyAsync(C.c_void_p(dst), C.c_void_p(src), nbytes, kind, self.ptr),
+              "hipMemcpyAsync")
+
+    def synchronize(self):
+        check(hip().hipStreamSynchronize(self.ptr), "hipStreamSynchronize")
+
+    def destroy(self):
+        if self.ptr:
+            hip().hipStreamDestroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
