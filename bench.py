@@ -16,13 +16,15 @@ Prints one JSON line:
                  peak (it is compute-bound), its LDS read rate, its HBM view
   kernels_hbm    per-kernel HBM GB/s from the committed rocprofv3 counters
   host_path      hd_execute from page-locked host memory (the rate including
-                 the PCIe copy; never the headline)
+                 the PCIe copy; never the headline), and the gulp driver's
+                 double buffering that hides the copy behind the search
   adaptive_dt    the same gulp with use_scrunching on (the reference's default
                  search: trials at their scrunch factor)
   batch          the ragged 352-beam batch (320 channels, 655.36 us) in one
                  hd_execute_batch per step, with its own roofline
   cpu_baseline   the CPU oracle on a bounded sample of the same gulp;
   cpu_baseline_small the CPU oracle on whole 512-channel gulps
+  small_gpu      the same 512-channel gulps searched on the GPU
   proposal       proposal-gated vs full batched search on 4096-channel beams
 """
 import argparse
@@ -548,6 +550,40 @@ def host_path(pl, w, fb, steps, beam_s):
                                    "stream while the current one is searched"}}
 
 
+def small_path(steps, warmup, seed=1):
+    """The CPU-runnable configuration (512 channels, 65536 samples of 256 us,
+    DM 0-500) on the GPU: a whole gulp resident in HBM, graph replays."""
+    import heimdall_multibeam_amd as H
+    from heimdall_multibeam_amd import hip, synth
+    w = synth.CONFIGS["small_512ch"]
+    fb = synth.make_gulp(w, seed)
+    params = H.default_params(nchans=w.nchans, dt=w.dt, f0=w.f0, df=w.df, dm_min=w.dm_min,
+                              dm_max=w.dm_max, boxcar_max=w.boxcar_max,
+                              detect_thresh=w.detect_thresh, use_scrunching=False,
+                              min_tscrunch_width=w.min_tscrunch_width)
+    pl = H.Pipeline(params)
+    pl.set_output(False)
+    beam_s = (w.nsamps - w.boxcar_max - pl.max_delay()) * w.dt
+    dbuf = hip.DeviceBuffer(fb.nbytes)
+    dbuf.upload(fb)
+    for _ in range(max(warmup, 1)):
+        pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pl.execute(dbuf.address, w.nsamps, w.nbits, 0, 1, device=True)
+    hip.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    n_dm = len(pl.dm_list())
+    pl.close()
+    dbuf.free()
+    return {"value": round(beam_s / t, 3), "unit": "beam-seconds searched/s",
+            "ms_per_step": round(t * 1e3, 3),
+            "workload": "small_512ch: 1 beam, 512 ch x 8-bit, fch1 1500 MHz, foff -0.5 MHz, "
+                        "tsamp 256 us, gulp 65536, DM 0-500 (%d trials), boxcars 1..256, "
+                        "seed %d, input resident in HBM" % (n_dm, seed)}
+
+
 def adaptive_path(w, dbuf, steps, warmup, beam_s):
     """The same HBM-resident gulp searched with adaptive time resolution
     (use_scrunching, the reference's default): each trial at its scrunch
@@ -683,10 +719,16 @@ def main():
         "adaptive_dt": None,
         "cpu_baseline": None,
         "cpu_baseline_small": None,
+        "small_gpu": None,
     }
     if not a.no_host_path:
         out["host_path"] = host_path(pl, w, fb, a.steps, beam_s)
     pl.close()
+    if a.workload == "full_1024ch":
+        try:
+            out["small_gpu"] = small_path(a.steps, a.warmup)
+        except Exception as e:  # reported, never hides the main measurement
+            out["small_gpu"] = {"error": "%s: %s" % (type(e).__name__, e)}
     if not a.no_adaptive:
         try:
             out["adaptive_dt"] = adaptive_path(w, dbuf, a.steps, a.warmup, beam_s)
@@ -697,6 +739,9 @@ def main():
         print("timing CPU baselines", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, fb, a.cpu_stride, n_dm, beam_s)
         out["cpu_baseline_small"] = cpu_baseline_small()
+        sg = out.get("small_gpu")
+        if sg and "value" in sg and out["cpu_baseline_small"]["value"] > 0:
+            sg["over_cpu_baseline_small"] = round(sg["value"] / out["cpu_baseline_small"]["value"], 1)
     if not a.no_batch:
         try:
             out["batch"] = batch_measure(a, a.steps, a.warmup)

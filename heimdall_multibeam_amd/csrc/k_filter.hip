@@ -364,20 +364,35 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const unsigned tneed = tavail <= 0 ? 1u : (tavail < (long long)tlen ? (unsigned)tavail : tlen);
   const unsigned n4 = (tneed + sh + 3) / 4;
   constexpr int SU = 12;  // (the widest tile, 8192 + 4096 floats, in one round)
+  // Plain loads, no per-piece tests, when the pieces lie within three floats
+  // of the row (a thread past the last piece loads the last one again and
+  // stores nothing).  Rows of S start 3 floats into their allocation and are
+  // padded to a multiple of 64 floats, with 5 spare floats after the last, so
+  // those floats are readable; the tile entries they fill (before S[0], past
+  // S[s_len - 1]) are read only for outputs that are never kept.
+  const bool whole = tal >= -3 && tal + 4ll * n4 <= slen + 3;
   for (unsigned v0 = threadIdx.x; v0 < n4; v0 += SU * FB_BS) {
     float4 r[SU];
+    if (whole) {
 #pragma unroll
-    for (int u = 0; u < SU; ++u) {
-      const unsigned v = v0 + u * FB_BS;
-      const long long si = tal + 4ll * v;
-      if (v < n4) {
-        if (si >= 0 && si + 3 < slen) {
-          r[u] = *reinterpret_cast<const float4*>(S + si);
-        } else {
-          r[u].x = (si >= 0 && si < slen) ? S[si] : 0.f;
-          r[u].y = (si + 1 >= 0 && si + 1 < slen) ? S[si + 1] : 0.f;
-          r[u].z = (si + 2 >= 0 && si + 2 < slen) ? S[si + 2] : 0.f;
-          r[u].w = (si + 3 >= 0 && si + 3 < slen) ? S[si + 3] : 0.f;
+      for (int u = 0; u < SU; ++u) {
+        const unsigned v = v0 + u * FB_BS;
+        r[u] = *reinterpret_cast<const float4*>(S + tal + 4ll * (v < n4 ? v : n4 - 1));
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < SU; ++u) {
+        const unsigned v = v0 + u * FB_BS;
+        const long long si = tal + 4ll * v;
+        if (v < n4) {
+          if (si >= 0 && si + 3 < slen) {
+            r[u] = *reinterpret_cast<const float4*>(S + si);
+          } else {
+            r[u].x = (si >= 0 && si < slen) ? S[si] : 0.f;
+            r[u].y = (si + 1 >= 0 && si + 1 < slen) ? S[si + 1] : 0.f;
+            r[u].z = (si + 2 >= 0 && si + 2 < slen) ? S[si + 2] : 0.f;
+            r[u].w = (si + 3 >= 0 && si + 3 < slen) ? S[si + 3] : 0.f;
+          }
         }
       }
     }
