@@ -119,6 +119,8 @@ struct hd_pipeline_t {
   DevBuf fb, x, live, delays, plane, pts, rms, brms, S, partial, wrms, winv;
   DevBuf skey, sval, counter, ccount, coff, cpos, okey, oval, head, segid, scan_tmp;
   DevBuf g_peak, g_ind, g_begin, g_end, g_filter, g_dm, g_best;
+  // the giants of adaptive-resolution runs, gathered on the device run by run
+  DevBuf a_peak, a_ind, a_begin, a_end, a_filter, a_dm, a_group;
   DevBuf c_keys, c_lo, c_hi, c_labels, c_flag, c_best, c_members, c_rflag, c_rpos, c_tmp;
   DevBuf c_sind, c_sfd;
   DevBuf c_ecount, c_eoff, c_adj;  // coincidence lists
@@ -1540,10 +1542,32 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
     // Adaptive time resolution (pipeline.cu:488-790): the runs of trials of
     // equal scrunch factor in turn, each over every gulp with the shapes of
     // its factor, from the input at its resolution.  The giants are gathered
-    // per gulp in trial order, as the reference's DM loop appends them.
-    std::vector<float> a_peak;
-    std::vector<uint64_t> a_ind, a_begin, a_end;
-    std::vector<uint32_t> a_filter, a_dm, a_group;
+    // per gulp in trial order, as the reference's DM loop appends them: each
+    // run's giants (in gulp, then trial order) are appended on the device,
+    // and with several gulps put in gulp order at the end.
+    size_t acc = 0;  // giants gathered so far
+    // room for `need` elements of `elem` bytes, keeping the first acc
+    auto room = [&](DevBuf& b, size_t elem, size_t need) -> hipError_t {
+      if (b.p && b.bytes >= need * elem) return hipSuccess;
+      DevBuf nb;
+      hipError_t e = nb.ensure(std::max(need * elem, 2 * b.bytes));
+      if (e == hipSuccess && acc)
+        e = hipMemcpyAsync(nb.p, b.p, acc * elem, hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) {
+        nb.release();
+        return e;
+      }
+      b.release();  // (hipFree waits for the copy above)
+      b = nb;
+      return hipSuccess;
+    };
+    auto append = [&](DevBuf& dst, const DevBuf& src, size_t elem, size_t n) -> hipError_t {
+      hipError_t e = room(dst, elem, acc + n);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(dst.as<char>() + acc * elem, src.p, n * elem,
+                           hipMemcpyDeviceToDevice, s);
+      return e;
+    };
     const size_t nl = std::max(n_live, 1);
     for (const auto& k : pl->classes) {
       if (k.factor > W) break;  // no width to search (:657-659), here or later
@@ -1583,23 +1607,40 @@ hd_error run_gulps(hd_pipeline pl, const std::vector<GulpIn>& in, size_t nbits) 
                                      &tile, crg[0].sh, ng);
       if (e != HD_NO_ERROR) return e;
       if (ng == 0) continue;
-      const size_t o = a_peak.size();
-      a_peak.resize(o + ng); a_ind.resize(o + ng); a_begin.resize(o + ng); a_end.resize(o + ng);
-      a_filter.resize(o + ng); a_dm.resize(o + ng); a_group.resize(o + ng);
-      HD_CHECK(hipMemcpyAsync(a_peak.data() + o, pl->g_peak.p, ng * 4, hipMemcpyDeviceToHost, s));
-      HD_CHECK(hipMemcpyAsync(a_ind.data() + o, pl->g_ind.p, ng * 8, hipMemcpyDeviceToHost, s));
-      HD_CHECK(hipMemcpyAsync(a_begin.data() + o, pl->g_begin.p, ng * 8, hipMemcpyDeviceToHost, s));
-      HD_CHECK(hipMemcpyAsync(a_end.data() + o, pl->g_end.p, ng * 8, hipMemcpyDeviceToHost, s));
-      HD_CHECK(hipMemcpyAsync(a_filter.data() + o, pl->g_filter.p, ng * 4, hipMemcpyDeviceToHost, s));
-      HD_CHECK(hipMemcpyAsync(a_dm.data() + o, pl->g_dm.p, ng * 4, hipMemcpyDeviceToHost, s));
-      HD_CHECK(hipMemcpyAsync(a_group.data() + o, pl->g_group.p, ng * 4, hipMemcpyDeviceToHost, s));
-      HD_CHECK(hipStreamSynchronize(s));
+      HD_CHECK(append(pl->a_peak, pl->g_peak, 4, ng));
+      HD_CHECK(append(pl->a_ind, pl->g_ind, 8, ng));
+      HD_CHECK(append(pl->a_begin, pl->g_begin, 8, ng));
+      HD_CHECK(append(pl->a_end, pl->g_end, 8, ng));
+      HD_CHECK(append(pl->a_filter, pl->g_filter, 4, ng));
+      HD_CHECK(append(pl->a_dm, pl->g_dm, 4, ng));
+      HD_CHECK(append(pl->a_group, pl->g_group, 4, ng));
+      acc += ng;
     }
     // the stage events of the runs are not split: the whole search counts as
     // dedispersion
     for (int ev = 2; ev <= 6; ++ev) record(pl, ev);
-    n_giants = a_peak.size();
-    if (n_giants > 0) {
+    n_giants = acc;
+    if (n_giants > 0 && G == 1) {
+      // one gulp: the runs appended in trial order are the giants
+      std::swap(pl->g_peak, pl->a_peak);
+      std::swap(pl->g_ind, pl->a_ind);
+      std::swap(pl->g_begin, pl->a_begin);
+      std::swap(pl->g_end, pl->a_end);
+      std::swap(pl->g_filter, pl->a_filter);
+      std::swap(pl->g_dm, pl->a_dm);
+      std::swap(pl->g_group, pl->a_group);
+    } else if (n_giants > 0) {
+      std::vector<float> a_peak(n_giants);
+      std::vector<uint64_t> a_ind(n_giants), a_begin(n_giants), a_end(n_giants);
+      std::vector<uint32_t> a_filter(n_giants), a_dm(n_giants), a_group(n_giants);
+      HD_CHECK(hipMemcpyAsync(a_peak.data(), pl->a_peak.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_ind.data(), pl->a_ind.p, n_giants * 8, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_begin.data(), pl->a_begin.p, n_giants * 8, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_end.data(), pl->a_end.p, n_giants * 8, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_filter.data(), pl->a_filter.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_dm.data(), pl->a_dm.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipMemcpyAsync(a_group.data(), pl->a_group.p, n_giants * 4, hipMemcpyDeviceToHost, s));
+      HD_CHECK(hipStreamSynchronize(s));
       // gulp by gulp (stable: each gulp's giants stay in trial order)
       std::vector<size_t> ord(n_giants);
       for (size_t i = 0; i < n_giants; ++i) ord[i] = i;
@@ -2081,6 +2122,8 @@ extern "C" void hd_destroy_pipeline(hd_pipeline pl) {
     DevBuf* bufs[] = {&pl->clean_out, &pl->clean_work, &pl->fb, &pl->x, &pl->live, &pl->delays, &pl->plane, &pl->pts, &pl->rms, &pl->brms,
                       &pl->S, &pl->partial, &pl->wrms, &pl->winv, &pl->skey, &pl->sval,
                       &pl->counter, &pl->ccount, &pl->coff, &pl->cpos, &pl->c_gb,
+                      &pl->a_peak, &pl->a_ind, &pl->a_begin, &pl->a_end, &pl->a_filter,
+                      &pl->a_dm, &pl->a_group,
                       &pl->okey, &pl->oval,
                       &pl->head, &pl->segid, &pl->scan_tmp, &pl->g_peak, &pl->g_ind,
                       &pl->g_begin, &pl->g_end, &pl->g_filter, &pl->g_dm, &pl->g_best, &pl->c_keys,
