@@ -14,9 +14,11 @@ namespace hd {
 // Medians.  The reference's comparison networks (Pipeline/median_filter.cu:
 // 21-75) return the order statistic (median4: mean of the middle two).
 // Written here as min/max networks: the extreme elements of a pair of pairs
-// can never be the median, which leaves a median of three.
+// can never be the median, which leaves a median of three, one v_med3_f32.
+// The values searched are finite with no negative zero (integer sums, and
+// absolute deviations), where every median network gives the same bits.
 __device__ __forceinline__ float med3(float a, float b, float c) {
-  return fmaxf(fminf(a, b), fminf(fmaxf(a, b), c));
+  return __builtin_amdgcn_fmed3f(a, b, c);
 }
 __device__ __forceinline__ float med4(float a, float b, float c, float d) {
   const float lo = fmaxf(fminf(a, b), fminf(c, d));
