@@ -77,15 +77,20 @@ __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ fb,
   }
 }
 
-// 8-bit samples with every channel live: 64 time x 256 channel tiles moved as
-// 32-bit words both ways (four channels per word read, four time samples per
-// word written).
-constexpr int UP_T = 64, UP_C = 256;
+// 8-bit samples with every channel live: tiles of 64 time samples x tcw
+// channels moved as 32-bit words both ways (four channels per word read, four
+// time samples per word written).  tcw is a multiple of 4: the whole row when
+// it has at most 512 channels, so that a tile is one stretch of the input
+// and no block reads part of a cache line another block also fetches; 256
+// otherwise.  Dynamic LDS: 64 rows of tcw / 4 + 1 words.
+constexpr int UP_T = 64;
+constexpr int UP_ROW_MAX = 512;
 __global__ __launch_bounds__(256) void k_unpack8(const uint8_t* __restrict__ fb,
                                                  size_t stride_bytes, size_t nsamps, int nchans,
                                                  uint8_t* __restrict__ x, size_t xpitch,
-                                                 const GroupDev* __restrict__ gd, size_t trim) {
-  __shared__ uint32_t tile[UP_T][UP_C / 4 + 1];
+                                                 const GroupDev* __restrict__ gd, size_t trim,
+                                                 int tcw) {
+  extern __shared__ uint32_t tile[];
   if (gd) {
     fb = gd[blockIdx.z].fb;
     nsamps = gd[blockIdx.z].nsamps;
@@ -94,41 +99,57 @@ __global__ __launch_bounds__(256) void k_unpack8(const uint8_t* __restrict__ fb,
   const size_t t0 = (size_t)blockIdx.x * UP_T;
   const size_t tlim = row_limit(nsamps, xpitch, trim);
   if (t0 >= tlim) return;
-  const int c0 = blockIdx.y * UP_C;
+  const int c0 = blockIdx.y * tcw;
   const int tid = threadIdx.x;
+  const int nw = tcw / 4, rw = nw + 1;  // words per tile row, with the pad
   const bool words = (stride_bytes % 4 == 0) && (reinterpret_cast<uintptr_t>(fb) % 4 == 0);
+  // read: word idx of the tile is row idx / nw, word idx % nw; U loads in
+  // flight per thread before their LDS writes
+  constexpr int U = 8;
+  const int nread = UP_T * nw;
+  for (int i0 = tid; i0 < nread; i0 += U * 256) {
+    uint32_t v[U];
 #pragma unroll
-  for (int k = 0; k < UP_T * UP_C / 4 / 256; ++k) {
-    const int idx = tid + 256 * k;
-    const int r = idx / (UP_C / 4), dw = idx % (UP_C / 4);
-    const size_t t = t0 + r;
-    const int c = c0 + 4 * dw;
-    uint32_t v = 0;
-    if (t < nsamps && c < nchans) {
-      const uint8_t* row = fb + t * stride_bytes;
-      if (words && c + 3 < nchans) {
-        v = *reinterpret_cast<const uint32_t*>(row + c);
-      } else {
-        for (int i = 0; i < 4 && c + i < nchans; ++i) v |= (uint32_t)row[c + i] << (8 * i);
+    for (int u = 0; u < U; ++u) {
+      const int idx = i0 + u * 256;
+      const int r = idx / nw, dw = idx - r * nw;
+      const size_t t = t0 + r;
+      const int c = c0 + 4 * dw;
+      v[u] = 0;
+      if (idx < nread && t < nsamps && c < nchans) {
+        const uint8_t* row = fb + t * stride_bytes;
+        if (words && c + 3 < nchans) {
+          v[u] = *reinterpret_cast<const uint32_t*>(row + c);
+        } else {
+          for (int i = 0; i < 4 && c + i < nchans; ++i) v[u] |= (uint32_t)row[c + i] << (8 * i);
+        }
       }
     }
-    tile[r][dw] = v;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int idx = i0 + u * 256;
+      if (idx < nread) {
+        const int r = idx / nw;
+        tile[r * rw + (idx - r * nw)] = v[u];
+      }
+    }
   }
   __syncthreads();
-  const uint8_t* tb = reinterpret_cast<const uint8_t*>(&tile[0][0]);
-  constexpr int RB = (UP_C / 4 + 1) * 4;  // bytes per tile row
-#pragma unroll
-  for (int k = 0; k < UP_T * UP_C / 4 / 256; ++k) {
-    const int idx = tid + 256 * k;
+  // write: channel idx / 16 of the tile, time word idx % 16 (16 lanes write
+  // 64 consecutive bytes of one channel row)
+  const uint8_t* tb = reinterpret_cast<const uint8_t*>(tile);
+  const int RB = rw * 4;  // bytes per tile row
+  const int cend = min(tcw, nchans - c0);
+  const int nwrite = cend * (UP_T / 4);
+  for (int idx = tid; idx < nwrite; idx += 256) {
     const int cc = idx / (UP_T / 4), q = idx % (UP_T / 4);
-    const int c = c0 + cc;
     const size_t t = t0 + 4 * q;
-    if (c >= nchans || t >= tlim) continue;
+    if (t >= tlim) continue;
     const uint32_t v = (uint32_t)tb[(4 * q + 0) * RB + cc] |
                        ((uint32_t)tb[(4 * q + 1) * RB + cc] << 8) |
                        ((uint32_t)tb[(4 * q + 2) * RB + cc] << 16) |
                        ((uint32_t)tb[(4 * q + 3) * RB + cc] << 24);
-    *reinterpret_cast<uint32_t*>(x + (size_t)c * xpitch + t) = v;
+    *reinterpret_cast<uint32_t*>(x + (size_t)(c0 + cc) * xpitch + t) = v;
   }
 }
 
@@ -685,10 +706,12 @@ hipError_t launch_unpack(const uint8_t* fb, size_t stride_bytes, int nbits, size
                          bool trim_rows) {
   const size_t trim = trim_rows ? (size_t)2 * D2_TB : 0;
   if (nbits == 8 && live_is_identity && xpitch % 4 == 0) {
-    dim3 grid((unsigned)((xpitch + UP_T - 1) / UP_T), (unsigned)((n_live + UP_C - 1) / UP_C),
+    const int tcw = n_live <= UP_ROW_MAX ? std::max(4, (n_live + 3) / 4 * 4) : 256;
+    dim3 grid((unsigned)((xpitch + UP_T - 1) / UP_T), (unsigned)((n_live + tcw - 1) / tcw),
               (unsigned)ga.G);
-    hipLaunchKernelGGL(k_unpack8, grid, dim3(256), 0, s, fb, stride_bytes, nsamps, n_live, x,
-                       xpitch, ga.gd, trim);
+    const size_t lds = (size_t)UP_T * (tcw / 4 + 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_unpack8, grid, dim3(256), lds, s, fb, stride_bytes, nsamps, n_live, x,
+                       xpitch, ga.gd, trim, tcw);
     return hipGetLastError();
   }
   dim3 grid((unsigned)((xpitch + 63) / 64), (unsigned)((n_live + 63) / 64), (unsigned)ga.G);

@@ -336,14 +336,15 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
   const size_t p0 = c * TH_CHUNK;                 // first position of the chunk
   {
     // chunks past every width's output (shorter groups of a batch) only
-    // record empty counts (and padding rows of a gated group)
-    bool any = false;
-    if (blockIdx.y < gd[blockIdx.z].rows_valid)
-      for (int j = 0; j < wt.nw; ++j) {
-        const unsigned ts = wt.wi[j].ts;
-        if ((ts == 1 ? p0 : (p0 + ts - 1) / ts) < nwv[j]) any = true;
-      }
-    if (!any) {
+    // record empty counts (and padding rows of a gated group); lane j of
+    // each wave tests width j
+    bool mine = false;
+    const int lj = (int)(threadIdx.x & 63);
+    if (lj < wt.nw && blockIdx.y < gd[blockIdx.z].rows_valid) {
+      const unsigned ts = wt.wi[lj].ts;
+      mine = (ts == 1 ? p0 : (p0 + ts - 1) / ts) < nwv[lj];
+    }
+    if (!__ballot(mine)) {
       for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
         gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
       return;
@@ -529,13 +530,19 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
 #pragma unroll
       for (int q = 0; q < TH_PER; ++q) mx = fmaxf(mx, Ta[q * FB_BS] - Tb[q * FB_BS]);
     } else {
-      // a row ending in the chunk (short gulps of a batch): the steps past
-      // its last kept output are skipped, block-uniformly
+      // a row ending in the chunk (short gulps of a batch): steps of four
+      // outputs up to its last kept one, block-uniformly; inside a step the
+      // reads are all issued (every index lies in the tile) and the outputs
+      // past the row are selected away
 #pragma unroll
-      for (int q = 0; q < TH_PER; ++q) {
-        if ((unsigned)q * FB_BS >= pk) break;
-        const float d = Ta[q * FB_BS] - Tb[q * FB_BS];
-        if ((unsigned)q * FB_BS + tid < pk) mx = fmaxf(mx, d);
+      for (int q0 = 0; q0 < TH_PER; q0 += 4) {
+        if ((unsigned)q0 * FB_BS >= pk) break;
+        float d[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) d[u] = Ta[(q0 + u) * FB_BS] - Tb[(q0 + u) * FB_BS];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          mx = fmaxf(mx, (unsigned)(q0 + u) * FB_BS + tid < pk ? d[u] : -INFINITY);
       }
     }
     return mx * inv > thresh;
@@ -546,8 +553,13 @@ __global__ __launch_bounds__(FB_BS) void k_threshold(
     unsigned hw = 0;  // this thread's widths with a hit (nw <= 24 bits)
     for (int j = 0; j < wt.nw; ++j)
       if (screen_width(j)) hw |= 1u << j;
-    if (hw) atomicOr(&wmask, hw);
-    if (!__syncthreads_or(hw != 0)) {
+    // the wave's widths with a hit, one ballot per width, booked by one lane
+    unsigned wb = 0;
+    if (__ballot(hw != 0))
+      for (int j = 0; j < wt.nw; ++j)
+        if (__ballot((hw >> j) & 1u)) wb |= 1u << j;
+    if (lane == 0 && wb) atomicOr(&wmask, wb);
+    if (!__syncthreads_or(wb != 0)) {
       for (int j = threadIdx.x; j < wt.nw; j += FB_BS)
         gb.ccount[(row * wt.nw + j) * gb.nchunks_per_w + c] = 0;
       return;
