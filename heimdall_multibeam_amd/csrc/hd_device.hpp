@@ -46,19 +46,20 @@ __device__ __forceinline__ bool frac_gt(float x) { return x > 1e-5f; }
 // linear_stretch_functor (Pipeline/median_filter.cu:451-464) for output i;
 // in_end is the element one past the input, read when rounding lands the
 // last output at the end of the input.
+// Both forms below are written without branches: the second element is read
+// at j + 1 only when the fraction counts (as in the reference), else element j
+// is read twice, and the result is selected.
 struct Stretch {
   float step;  // float(in_count-1)/(out_count-1)
   __device__ __forceinline__ float operator()(const float* in, unsigned i) const {
     const float x = (float)i * step;
     const unsigned j = (unsigned)x;
     const float fr = x - (float)j;
-    if (frac_gt(fr)) {
-      const float a = in[j];
-      const float d = in[j + 1] - a;
-      const float p = fr * d;
-      return a + p;
-    }
-    return in[j] + 0.f;
+    const bool lerp = frac_gt(fr);
+    const float a = in[j];
+    const float b = in[lerp ? j + 1 : j];
+    const float p = fr * (b - a);
+    return lerp ? a + p : a + 0.f;
   }
 };
 
@@ -73,13 +74,11 @@ struct StretchBeam {
     const float xb = x + (float)beam;
     const unsigned k = (unsigned)xb;
     const float o = (x - (float)k) + (float)beam;
-    if (frac_gt(o)) {
-      const float a = in[k];
-      const float d = in[k + 1] - a;
-      const float p = o * d;
-      return a + p;
-    }
-    return in[k] + 0.f;
+    const bool lerp = frac_gt(o);
+    const float a = in[k];
+    const float b = in[lerp ? k + 1 : k];
+    const float p = o * (b - a);
+    return lerp ? a + p : a + 0.f;
   }
 };
 

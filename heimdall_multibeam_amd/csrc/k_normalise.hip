@@ -254,6 +254,7 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_scan_seg(
   __shared__ float sP[STAGED ? SC_PMAX + 1 : 1];
   __shared__ float sR[STAGED ? SC_RMAX : 1];
   __shared__ float sT[2][SQ_WAVES];  // chunk totals of a round (two rounds alternate)
+  __shared__ __align__(16) float sQ[SQ_WAVES][68];  // each wave's segment totals
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int ls = wave / WPS, lw = wave % WPS;  // the wave's series in the block, its turn
@@ -319,24 +320,46 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_scan_seg(
       x[4 * i + 2] = va[i].z;
       x[4 * i + 3] = va[i].w;
     }
-    // P: in order inside the segment (samples past the series are not added)
+    // P: in order inside the segment (samples past the series are not added;
+    // a whole segment, the usual case, without the per-sample tests)
     float p = 0.f;
+    if (nv == SQ_SEG) {
 #pragma unroll
-    for (int i = 0; i < SQ_SEG; ++i) {
-      if (i < nv) {
+      for (int i = 0; i < SQ_SEG; ++i) {
         const float v = rms ? normalised_at(x[i], P, R, sh, t0 + i) : x[i];
         p = p + v;
         x[i] = p;
       }
-    }
-    // Q: the chunk's segment totals in lane order, as a wave-uniform chain
-    float q = 0.f, mine = 0.f;
+    } else {
 #pragma unroll
-    for (int s = 0; s < 64; ++s) {
-      const float ts = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p), s));
-      if (lane == s) mine = q;
-      q = q + ts;
+      for (int i = 0; i < SQ_SEG; ++i) {
+        if (i < nv) {
+          const float v = rms ? normalised_at(x[i], P, R, sh, t0 + i) : x[i];
+          p = p + v;
+          x[i] = p;
+        }
+      }
     }
+    // Q: the chunk's segment totals in lane order, one chain of adds that
+    // lane 0 runs over the totals staged in LDS; entry s becomes the sum of
+    // totals 0 .. s-1 and entry 64 the chunk's total
+    float* qb = sQ[wave];
+    qb[lane] = p;
+    wave_sync();
+    if (lane == 0) {
+      float acc = 0.f;
+#pragma unroll
+      for (int s = 0; s < 64; ++s) {
+        const float ts = qb[s];
+        qb[s] = acc;
+        acc = acc + ts;
+      }
+      qb[64] = acc;
+    }
+    wave_sync();
+    const float mine = qb[lane];
+    const float q = qb[64];
+    wave_sync();  // qb is rewritten by the next chunk
     // O of this chunk: the round's carry plus the earlier chunks of the round
     // in order; a chunk past the series adds a zero total after every real one
     float o = O;
