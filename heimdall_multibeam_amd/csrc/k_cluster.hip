@@ -470,17 +470,18 @@ __global__ void k_group_bounds(const uint32_t* __restrict__ group, unsigned n, u
 
 // The list relaxation of one clustering group with its labels in LDS: one
 // 1024-thread block per group (giants gb[2b] .. gb[2b + 2] - 1, at most
-// kLdsGroup; labels local to the group).  A pass updates every giant in
-// index order, 128 giants per round with 8 lanes each and a barrier between
-// rounds, so that a round sees the labels of the rounds before it and
-// chains that run forward in index order settle in one pass.  Passes repeat
-// until one changes no label.  Labels only decrease, so the loop ends, and
-// the fixed point (each giant labelled with the smallest index reachable
-// along decreasing coincident chains) does not depend on the order of the
-// updates.  After pass k every giant whose longest decreasing chain has at
-// most k links holds its final label, so at most ng + 1 passes run for a
-// group of ng giants (the last one changes nothing): the pass bound ng + 2
-// is never what ends the loop.
+// kLdsGroup; labels local to the group).  The giants are taken in index
+// order, 128 per round with 8 lanes each.  A giant's list holds earlier
+// giants only, so once the rounds before it are final a round depends on
+// itself alone: it is updated again until an update changes none of its
+// labels (once, when no giant of the round lists another one of it), and
+// then it is final too.  The result is the fixed point (each giant
+// labelled with the smallest index reachable along decreasing coincident
+// chains), which does not depend on the order of the updates.  After k
+// updates of a round every giant whose longest decreasing chain inside the
+// round has at most k links holds its final label, so at most 128 + 1
+// updates run (the last one changes nothing): the bound 128 + 2 is never
+// what ends the loop.
 constexpr unsigned kLdsGroup = 16384;  // giants per group (a 32-bit label each)
 __global__ __launch_bounds__(1024) void k_relax_lists_lds(const uint32_t* __restrict__ gb,
                                                           const uint32_t* __restrict__ eoff,
@@ -496,10 +497,11 @@ __global__ __launch_bounds__(1024) void k_relax_lists_lds(const uint32_t* __rest
   __syncthreads();
   const unsigned sub = threadIdx.x % kListLanes;
   const unsigned npad = (n + kPer - 1) / kPer * kPer;
-  for (unsigned pass = 0; pass < n + 2; ++pass) {
-    bool changed = false;
-    for (unsigned i = threadIdx.x / kListLanes; i < npad; i += kPer) {
-      const bool live = i < n;
+  for (unsigned r0 = 0; r0 < npad; r0 += kPer) {
+    const unsigned i = r0 + threadIdx.x / kListLanes;
+    const bool live = i < n;
+    const unsigned p0 = live ? eoff[a + i] : 0u, p1 = live ? eoff[a + i + 1] : 0u;
+    for (unsigned it = 0; it < kPer + 2; ++it) {
       const int32_t cur = live ? __atomic_load_n(&lab[i], __ATOMIC_RELAXED) : 0x7fffffff;
       int32_t best = cur;
       if (live && sub == 0) {  // follow_labels over the group's labels
@@ -512,7 +514,7 @@ __global__ __launch_bounds__(1024) void k_relax_lists_lds(const uint32_t* __rest
         }
         best = r;
       }
-      const unsigned p0 = live ? eoff[a + i] : 0u, p1 = live ? eoff[a + i + 1] : 0u;
+      bool intra = false;  // a neighbour in this round
       for (unsigned p = p0 + sub; p < p1; p += kListLanes * U) {
         uint32_t jj[U];
 #pragma unroll
@@ -524,6 +526,7 @@ __global__ __launch_bounds__(1024) void k_relax_lists_lds(const uint32_t* __rest
         for (int u = 0; u < U; ++u) {
           const int32_t lj = __atomic_load_n(&lab[jj[u]], __ATOMIC_RELAXED);
           if (lj < best) best = lj;
+          intra = intra || jj[u] >= r0;
         }
       }
 #pragma unroll
@@ -531,13 +534,17 @@ __global__ __launch_bounds__(1024) void k_relax_lists_lds(const uint32_t* __rest
         const int32_t other = __shfl_xor(best, o);
         best = other < best ? other : best;
       }
+      bool changed = false;
       if (live && sub == 0 && best < cur) {
         __atomic_store_n(&lab[i], best, __ATOMIC_RELAXED);
         changed = true;
       }
-      __syncthreads();  // the next round sees this round's labels
+      // again only when a label of the round changed and a giant of the
+      // round reads another one's (the later iterations repeat only the
+      // first test)
+      if (!__syncthreads_or(changed)) break;
+      if (it == 0 && !__syncthreads_or(intra)) break;
     }
-    if (!__syncthreads_or(changed)) break;
   }
   for (unsigned k = threadIdx.x; k < n; k += 1024) labels[a + k] = (int32_t)a + lab[k];
 }
