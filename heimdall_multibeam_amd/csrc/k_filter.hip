@@ -268,6 +268,31 @@ __global__ __launch_bounds__(64) void k_block_rms_finish(const float* __restrict
   }
 }
 
+// The same with at most four partials per (trial, width), the short series of
+// a ragged batch: no median level remains, only the tail, so one thread per
+// (row, width) takes it in registers.  Items: rows x n_group, width-fastest.
+__global__ __launch_bounds__(256) void k_block_rms_finish_short(
+    const float* __restrict__ partial, const GroupDev* __restrict__ gd, size_t np_one,
+    size_t np_max, int n_dm, size_t n_items, int nw_total, int first_w, int n_group,
+    float* __restrict__ wrms, float* __restrict__ winv) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_items) return;
+  const size_t row = idx / (unsigned)n_group;
+  const int j = first_w + (int)(idx - row * (unsigned)n_group);
+  const size_t np = gd ? gd[row / (unsigned)n_dm].bs.np : np_one;  // <= 4
+  const float* p = partial + (row * nw_total + j) * np_max;
+  const float v0 = np > 0 ? p[0] : 0.f, v1 = np > 1 ? p[1] : 0.f;
+  const float v2 = np > 2 ? p[2] : 0.f, v3 = np > 3 ? p[3] : 0.f;
+  // med_tail of the np values (the pyramid's end, median_filter.cu:217-235)
+  const float med = np >= 4 ? med4(v0, v1, v2, v3)
+                    : np == 3 ? med3(v0, v1, v2)
+                    : np == 2 ? 0.5f * (v0 + v1)
+                    : np == 1 ? v0 : 0.f;
+  const float r = (float)((double)med * 1.4862);  // get_rms.cu:127-128
+  wrms[row * nw_total + j] = r;
+  winv[row * nw_total + j] = 1.0f / r;            // get_rms.cu:195
+}
+
 // Block-wide exclusive scan of one unsigned per thread; returns the prefix,
 // writes the total to *total.
 template <int BS>
@@ -834,6 +859,13 @@ hipError_t launch_block_rms_finish(const float* partial, int nw_total, int first
   if ((!ga.gd && !single) || n_group == 0) return hipSuccess;
   const size_t np_max = ga.gd ? block_rms_np_max(ga) : single->np;
   if (np_max == 0) return hipSuccess;
+  if (np_max <= 4) {
+    const size_t n_items = (size_t)n_dm * (ga.gd ? ga.G : 1) * n_group;
+    hipLaunchKernelGGL(k_block_rms_finish_short, dim3((unsigned)((n_items + 255) / 256)),
+                       dim3(256), 0, s, partial, ga.gd, single ? single->np : (size_t)0, np_max,
+                       n_dm, n_items, nw_total, first_w, n_group, wrms, winv);
+    return hipGetLastError();
+  }
   const size_t lds = (np_max + np_max / 5 + 8) * sizeof(float);
   if (lds > 65536) return hipErrorInvalidValue;  // pooled count beyond ~40M samples
   hipLaunchKernelGGL(k_block_rms_finish, dim3(n_group, n_dm, ga.gd ? ga.G : 1), dim3(64), lds, s,
