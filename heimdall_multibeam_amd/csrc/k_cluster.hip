@@ -489,7 +489,7 @@ __global__ __launch_bounds__(1024) void k_relax_lists_lds(const uint32_t* __rest
                                                           int32_t* __restrict__ labels) {
   extern __shared__ int32_t lab[];
   constexpr unsigned kPer = 1024 / kListLanes;  // giants per round
-  constexpr int U = 4;                          // list entries in flight per lane
+  constexpr int U = 16;                         // list entries in flight per lane
   const unsigned a = gb[2 * blockIdx.x], e = gb[2 * blockIdx.x + 2];
   if (a >= e) return;
   const unsigned n = e - a;
