@@ -116,11 +116,21 @@ __global__ void k_windows(const uint64_t* __restrict__ ind, const uint32_t* __re
 
 // are_coincident_beam (are_coincident.cuh:46-68) with the reference's int
 // casts: time tolerance scaled by the wider of the two boxcars.
+// Giant i's beam as a sample range: ij / nsamps_beam == ii / nsamps_beam
+// exactly when ij lies in it, so the 64-bit division is taken once per giant
+// instead of once per pair.
+struct BeamRange {
+  uint64_t lo, hi;
+};
+__device__ __forceinline__ BeamRange beam_range(uint64_t ii, uint64_t nsamps_beam) {
+  const uint64_t lo = ii / nsamps_beam * nsamps_beam;
+  return BeamRange{lo, lo + nsamps_beam};
+}
 __device__ __forceinline__ bool coincident(uint64_t ii, uint64_t ij, uint32_t fi, uint32_t fj,
                                            uint32_t di, uint32_t dj, uint64_t time_tol,
                                            uint64_t filter_tol, uint64_t dm_tol,
-                                           uint64_t nsamps_beam) {
-  if (ii / nsamps_beam != ij / nsamps_beam) return false;
+                                           const BeamRange& br) {
+  if (ij < br.lo || ij >= br.hi) return false;
   const int wi = 1 << (int)fi, wj = 1 << (int)fj;
   const uint64_t tt = time_tol * (uint64_t)(wi > wj ? wi : wj);
   const int dt = abs((int)ij - (int)ii);
@@ -145,12 +155,13 @@ __global__ void k_relax(const uint64_t* __restrict__ ind, const uint32_t* __rest
   const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
   if (jump < best) best = jump;
   const uint64_t ii = ind[i];
+  const BeamRange br = beam_range(ii, nsamps_beam);
   const uint32_t fi = filter[i], di = dm[i];
   for (unsigned p = lo[i]; p < hi[i]; ++p) {
     const unsigned j = (unsigned)(keys[p] & 0xffffffffull);
     if (j >= i) continue;
     if (!coincident(ii, ind[j], fi, filter[j], di, dm[j], time_tol, filter_tol, dm_tol,
-                    nsamps_beam))
+                    br))
       continue;
     const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
     if (lj < best) best = lj;
@@ -182,6 +193,7 @@ __global__ void k_relax_sorted(const uint64_t* __restrict__ ind,
   const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
   if (jump < best) best = jump;
   const uint64_t ii = ind[i];
+  const BeamRange br = beam_range(ii, nsamps_beam);
   const uint32_t fi = filter[i], di = dm[i];
   const unsigned p1 = hi[i];
   for (unsigned p = lo[i]; p < p1; ++p) {
@@ -189,7 +201,7 @@ __global__ void k_relax_sorted(const uint64_t* __restrict__ ind,
     if (j >= i) continue;
     const uint32_t fd = s_fd[p];
     if (!coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol, dm_tol,
-                    nsamps_beam))
+                    br))
       continue;
     const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
     if (lj < best) best = lj;
@@ -288,6 +300,7 @@ __global__ void k_relax_banded(const uint64_t* __restrict__ ind,
   const int32_t jump = __atomic_load_n(&labels[cur], __ATOMIC_RELAXED);
   if (jump < best) best = jump;
   const uint64_t ii = ind[i];
+  const BeamRange br = beam_range(ii, nsamps_beam);
   const uint32_t fi = filter[i], di = dm[i];
   for (unsigned b = 0; b < nb; ++b) {
     const unsigned p1 = hi[(size_t)i * nb + b];
@@ -296,7 +309,7 @@ __global__ void k_relax_banded(const uint64_t* __restrict__ ind,
       if (j >= i) continue;
       const uint32_t fd = s_fd[p];
       if (!coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol, dm_tol,
-                      nsamps_beam))
+                      br))
         continue;
       const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
       if (lj < best) best = lj;
@@ -329,6 +342,7 @@ __global__ __launch_bounds__(256) void k_relax_banded_wave(
     if (jump < best) best = jump;
   }
   const uint64_t ii = ind[i];
+  const BeamRange br = beam_range(ii, nsamps_beam);
   const uint32_t fi = filter[i], di = dm[i];
   for (unsigned b = 0; b < nb; ++b) {
     const unsigned p1 = hi[(size_t)i * nb + b];
@@ -337,7 +351,7 @@ __global__ __launch_bounds__(256) void k_relax_banded_wave(
       if (j >= i) continue;
       const uint32_t fd = s_fd[p];
       if (!coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol, dm_tol,
-                      nsamps_beam))
+                      br))
         continue;
       const int32_t lj = __atomic_load_n(&labels[j], __ATOMIC_RELAXED);
       if (lj < best) best = lj;
@@ -371,6 +385,7 @@ __global__ __launch_bounds__(256) void k_edges_banded_wave(
   if (i >= n) return;  // whole wave
   const uint64_t nsamps_beam = group ? gd[group[i]].nsamps_beam : nsamps_beam_one;
   const uint64_t ii = ind[i];
+  const BeamRange br = beam_range(ii, nsamps_beam);
   const uint32_t fi = filter[i], di = dm[i];
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   const unsigned base = MODE == 1 ? eoff[i] : 0u;
@@ -386,7 +401,7 @@ __global__ __launch_bounds__(256) void k_edges_banded_wave(
         if (j < i) {
           const uint32_t fd = s_fd[p];
           hit = coincident(ii, s_ind[p], fi, fd & 31u, di, fd >> 5, time_tol, filter_tol,
-                           dm_tol, nsamps_beam);
+                           dm_tol, br);
         }
       }
       const uint64_t m = __ballot(hit);

@@ -719,10 +719,13 @@ __global__ void k_order_samples_by_sample(GiantBuffers gb, WidthTable wt, unsign
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nsamp) return;
   const uint64_t kv = gb.skey[p];
-  const uint64_t dmw = kv >> 32;
-  const unsigned j = (unsigned)(dmw % (uint64_t)wt.nw);
+  const unsigned dmw = (unsigned)(kv >> 32);  // a 32-bit field: 32-bit divisions
+  const unsigned j = dmw % (unsigned)wt.nw;
   const uint64_t pos = (kv & 0xffffffffull) * wt.wi[j].ts;
-  const size_t key = (size_t)dmw * gb.nchunks_per_w + (size_t)(pos / gb.chunk);
+  const size_t kch = (pos >> 32) == 0 && gb.chunk <= 0xffffffffull
+                         ? (size_t)((unsigned)pos / (unsigned)gb.chunk)
+                         : (size_t)(pos / gb.chunk);
+  const size_t key = (size_t)dmw * gb.nchunks_per_w + kch;
   const size_t dst = (size_t)gb.cpos[key] + (p - gb.coff[key]);
   gb.okey[dst] = kv;
   gb.oval[dst] = gb.sval[p];
@@ -735,7 +738,7 @@ __global__ void k_segment_heads(GiantBuffers gb, WidthTable wt, unsigned nsamp) 
   if (k > 0) {
     const uint64_t prev = gb.okey[k - 1];
     if ((prev >> 32) == (key >> 32)) {
-      const unsigned j = (unsigned)((key >> 32) % (uint64_t)wt.nw);
+      const unsigned j = (unsigned)(key >> 32) % (unsigned)wt.nw;
       const uint64_t i = key & 0xffffffffull;
       const uint64_t ip = prev & 0xffffffffull;
       h = (i > ip + wt.wi[j].merge_dist) ? 1u : 0u;
@@ -770,17 +773,17 @@ __global__ void k_segment_scatter(GiantBuffers gb, WidthTable wt, unsigned nsamp
   const unsigned ng = __shfl_down(g, 1);
   if (!valid) return;
   if (lane == 63 || ng != g) atomicMax((unsigned long long*)&best[g], bk);
-  const uint64_t dmw = key >> 32;
-  const unsigned j = (unsigned)(dmw % (uint64_t)wt.nw);
+  const unsigned dmw = (unsigned)(key >> 32);  // a 32-bit field: 32-bit divisions
+  const unsigned j = dmw % (unsigned)wt.nw;
   const WidthInfo wi = wt.wi[j];
   const uint64_t ro = wi.rel_off, sc = wi.cs;
   const uint64_t i = key & 0xffffffffull;
   if (h) {
-    const uint64_t row = dmw / (uint64_t)wt.nw;  // group * n_dm + trial
+    const unsigned row = dmw / (unsigned)wt.nw;  // group * n_dm + trial
     go.begin[g] = (i + ro) * sc;
     go.filter[g] = wi.filter_idx;
-    const uint32_t grp = (uint32_t)(row / n_dm);
-    go.dm[g] = (uint32_t)(row % n_dm) + (gd ? gd[grp].trial0 : 0u);
+    const uint32_t grp = row / n_dm;
+    go.dm[g] = row % n_dm + (gd ? gd[grp].trial0 : 0u);
     if (go.group) go.group[g] = gd ? gd[grp].cgroup : grp;
   }
   if (k + 1 == nsamp || gb.head[k + 1]) go.end[g] = (i + 1 + ro) * sc;
@@ -792,7 +795,7 @@ __global__ void k_segment_finish(GiantBuffers gb, WidthTable wt, unsigned ngiant
   if (g >= ngiants) return;
   const unsigned k = 0xffffffffu - (unsigned)(best[g] & 0xffffffffull);
   const uint64_t key = gb.okey[k];
-  const WidthInfo wi = wt.wi[(unsigned)((key >> 32) % (uint64_t)wt.nw)];
+  const WidthInfo wi = wt.wi[(unsigned)(key >> 32) % (unsigned)wt.nw];
   go.peak[g] = gb.oval[k];
   go.ind[g] = ((key & 0xffffffffull) + wi.rel_off) * (uint64_t)wi.cs;
 }
