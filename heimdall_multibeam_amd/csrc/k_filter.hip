@@ -64,10 +64,11 @@ __device__ __forceinline__ float filtered_at(const float* S, const WidthTable& w
 
 // dst[t] = src[t * stride] for t < n (LDS from global), with SU loads in
 // flight per thread before their LDS writes: a plain loop waits out the
-// memory latency once per element.
+// memory latency once per element.  24 per thread stage a tile of up to 6144
+// entries (a 3125-leaf chunk and a 2048-sample halo) in one round.
 __device__ __forceinline__ void stage_lds(float* dst, const float* __restrict__ src, size_t n,
                                  size_t stride) {
-  constexpr int SU = 8;
+  constexpr int SU = 24;
   for (size_t t0 = threadIdx.x; t0 < n; t0 += (size_t)SU * FB_BS) {
     float r[SU];
 #pragma unroll
