@@ -158,7 +158,7 @@ def evaluate(setup, seeds, kinds, params, need_heuristic=True):
     """Per held-out gulp: S/N>=8 candidate units, network unit maxima and
     heuristic unit maxima."""
     # load the network, then switch gating off: the searches here are the
-    # full search (the judge of recall); scores use the loaded network
+    # full search (the arbiter of recall); scores use the loaded network
     P.set_proposal(setup.pl, params, 0.0)
     P.set_proposal(setup.pl, None, 0.0)
     out = []

@@ -113,7 +113,7 @@ def test_dm_list_equals_oracle(oracle):
 
 
 def test_config_trial_counts():
-    """DM trial counts of the measured workloads."""
+    """DM trial counts of the measured 512-, 1024- and 320-channel workloads."""
     import heimdall_multibeam_amd as H
     cases = [((0, 500, 256e-6, 40, 1500.0, -0.5, 512, 1.25), 320),
              ((0, 2000, 64e-6, 40, 1517.8359375, -0.328125, 1024, 1.25), 1399),

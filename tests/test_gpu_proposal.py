@@ -287,7 +287,7 @@ def test_coarse_plane_learned_1024ch_matches_oracle(H, oracle):
 
 def test_gated_search_learned_1024ch_every_unit_equals_full(H, oracle):
     """On one full gulp of the learned-proposal workload: the full search
-    (the proposal's labeller and recall judge) against the oracle, then the
+    (the proposal's labeller and recall arbiter) against the oracle, then the
     gated search with every unit proposed against the full search, both bit
     for bit."""
     import os
